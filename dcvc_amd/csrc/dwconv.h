@@ -1,0 +1,19 @@
+#pragma once
+#include <hip/hip_runtime.h>
+#include "common.h"
+
+namespace dcvc {
+
+struct DwArgs {
+    const f16* x = nullptr;  // [n_img][H][W] pixels, ldx halves apart, C channels
+    int ldx = 0;
+    const f16* wt = nullptr; // [9][C] taps (ky * 3 + kx), channel contiguous
+    f16* y = nullptr;
+    int ldy = 0;
+    int n_img = 1, H = 0, W = 0, C = 0;
+};
+
+int dw3x3_validate(const DwArgs& a);
+hipError_t dw3x3_launch(const DwArgs& a, hipStream_t stream);
+
+}  // namespace dcvc

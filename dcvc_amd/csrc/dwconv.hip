@@ -1,0 +1,103 @@
+// Depthwise 3x3 stride-1 pad-1 convolution, NHWC fp16, fp32 accumulate.
+//
+// Replaces d3x3 (cutlass/d3x3.cu:443-446, d3x3_kernel.h:104-200).  The bias
+// of the depthwise conv is folded into the following 1x1 conv at load time,
+// as the reference does (layers_proxy.cpp:175-178), so there is none here.
+//
+// Mapping: one thread owns 8 consecutive channels (one 16-byte vector) of one
+// image column w and walks TH output rows, keeping the 3x3 input window in
+// registers (3 new 16-byte loads per output row).  Consecutive threads take
+// consecutive channel vectors, then consecutive columns, so a wavefront
+// reads contiguous NHWC rows.  The tap order (ky, kx) is fixed, so the
+// result does not depend on the launch shape.
+#include "common.h"
+#include "dwconv.h"
+
+namespace dcvc {
+
+template <int TH>
+__global__ __launch_bounds__(256) void dw3x3_kernel(DwArgs a) {
+    const int CG = a.C >> 3;
+    const long item = (long)blockIdx.x * 256 + threadIdx.x;
+    const int strips = (a.H + TH - 1) / TH;
+    const long total = (long)CG * a.W * strips * a.n_img;
+    if (item >= total) return;
+    const int cg = (int)(item % CG);
+    long rest = item / CG;
+    const int w = (int)(rest % a.W);
+    rest /= a.W;
+    const int hs = (int)(rest % strips);
+    const int img = (int)(rest / strips);
+    const int c0 = cg * 8;
+
+    float wt[9][8];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        const f16x8 v = *reinterpret_cast<const f16x8*>(a.wt + (size_t)t * a.C + c0);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) wt[t][e] = (float)v[e];
+    }
+
+    const f16* base = a.x + (size_t)img * a.H * a.W * a.ldx + c0;
+    const f16x8 zero8 = {0, 0, 0, 0, 0, 0, 0, 0};
+    auto load_row = [&](int h, f16x8 (&row)[3]) {
+        if (h < 0 || h >= a.H) {
+            row[0] = row[1] = row[2] = zero8;
+            return;
+        }
+        const f16* p = base + (size_t)h * a.W * a.ldx;
+        row[0] = (w > 0) ? *reinterpret_cast<const f16x8*>(p + (size_t)(w - 1) * a.ldx) : zero8;
+        row[1] = *reinterpret_cast<const f16x8*>(p + (size_t)w * a.ldx);
+        row[2] = (w + 1 < a.W) ? *reinterpret_cast<const f16x8*>(p + (size_t)(w + 1) * a.ldx) : zero8;
+    };
+
+    const int h0 = hs * TH;
+    const int h1 = min(h0 + TH, a.H);
+    f16x8 r0[3], r1[3], r2[3];
+    load_row(h0 - 1, r0);
+    load_row(h0, r1);
+    f16* obase = a.y + (size_t)img * a.H * a.W * a.ldy + c0;
+    for (int h = h0; h < h1; ++h) {
+        load_row(h + 1, r2);
+        f16x8 o;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            float s = 0.f;
+            s += wt[0][e] * (float)r0[0][e];
+            s += wt[1][e] * (float)r0[1][e];
+            s += wt[2][e] * (float)r0[2][e];
+            s += wt[3][e] * (float)r1[0][e];
+            s += wt[4][e] * (float)r1[1][e];
+            s += wt[5][e] * (float)r1[2][e];
+            s += wt[6][e] * (float)r2[0][e];
+            s += wt[7][e] * (float)r2[1][e];
+            s += wt[8][e] * (float)r2[2][e];
+            o[e] = (f16)s;
+        }
+        *reinterpret_cast<f16x8*>(obase + ((size_t)h * a.W + w) * a.ldy) = o;
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            r0[t] = r1[t];
+            r1[t] = r2[t];
+        }
+    }
+}
+
+int dw3x3_validate(const DwArgs& a) {
+    if (!a.x || !a.y || !a.wt) return DCVC_ERR_ARG;
+    if (a.C <= 0 || a.C % 8 || a.H <= 0 || a.W <= 0 || a.n_img <= 0) return DCVC_ERR_ARG;
+    if (a.ldx < a.C || a.ldy < a.C || a.ldx % 8 || a.ldy % 8) return DCVC_ERR_ARG;
+    if (((uintptr_t)a.x) % 16 || ((uintptr_t)a.y) % 16 || ((uintptr_t)a.wt) % 16) return DCVC_ERR_ARG;
+    return DCVC_OK;
+}
+
+hipError_t dw3x3_launch(const DwArgs& a, hipStream_t stream) {
+    constexpr int TH = 8;
+    const long strips = (a.H + TH - 1) / TH;
+    const long total = (long)(a.C / 8) * a.W * strips * a.n_img;
+    const int blocks = (int)((total + 255) / 256);
+    hipLaunchKernelGGL(dw3x3_kernel<TH>, dim3(blocks), dim3(256), 0, stream, a);
+    return hipGetLastError();
+}
+
+}  // namespace dcvc

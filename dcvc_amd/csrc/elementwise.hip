@@ -1,0 +1,538 @@
+// Bandwidth-bound kernels: layout shuffles, per-channel scaling, the masked
+// quantisation / scale-index boundary and stable stream compaction.
+// Reference kernels replaced: elementwise/stream.cu, shuffle.cu, cat_and_pad.cu.
+#include <cmath>
+#include <cstring>
+
+#include "common.h"
+#include "elementwise.h"
+
+// The quantisation formulas below are specified as separate IEEE float32
+// operations (no fused multiply-add), so that host oracles can reproduce them
+// bit-for-bit.
+#pragma clang fp contract(off)
+
+namespace dcvc {
+
+// ---------------------------------------------------------------------------
+// scale -> index thresholds (def_const.h:6-11 constants, FP32 semantics)
+// ---------------------------------------------------------------------------
+static const float kScaleMin = 0.11f, kScaleMax = 16.f;
+static const float kLogScaleMin = -2.2073f, kLogScaleMax = 2.7726f;
+
+static long scale_index_exact(float s) {
+    const float step = (kLogScaleMax - kLogScaleMin) / (float)(128 - 1);
+    const float recip = 1.f / step;
+    double v = (double)s;
+    v = v < (double)kScaleMin ? (double)kScaleMin : (v > (double)kScaleMax ? (double)kScaleMax : v);
+    return (long)std::floor((std::log(v) - (double)kLogScaleMin) * (double)recip);
+}
+
+void host_scale_index_thresholds(float th[128]) {
+    th[0] = -INFINITY;
+    int32_t lo_bits, hi_bits;
+    const float smin = kScaleMin, smax = kScaleMax;
+    for (int k = 1; k < 128; ++k) {
+        th[k] = INFINITY;
+        if (scale_index_exact(smax) < k) continue;
+        std::memcpy(&lo_bits, &smin, 4);
+        std::memcpy(&hi_bits, &smax, 4);
+        while (lo_bits < hi_bits) {
+            int32_t mid = lo_bits + (hi_bits - lo_bits) / 2;
+            float v;
+            std::memcpy(&v, &mid, 4);
+            if (scale_index_exact(v) >= k)
+                hi_bits = mid;
+            else
+                lo_bits = mid + 1;
+        }
+        std::memcpy(&th[k], &lo_bits, 4);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// element formulas shared by encoder and decoder
+// ---------------------------------------------------------------------------
+DCVC_DEV int quant_elem(float y, float mean, float scale, float thr) {
+    float r = round_half_away(y - mean);
+    if (!(scale > thr)) r = 0.f;
+    r = fminf(fmaxf(r, -128.f), 127.f);
+    return (int)r;
+}
+DCVC_DEV f16 yhat_elem(int yq, float mean) { return (f16)((float)yq + mean); }
+DCVC_DEV float qstep(float q) { return fmaxf(q, 0.5f); }
+
+static inline dim3 grid1d(long n, int bs = 256) { return dim3((unsigned)((n + bs - 1) / bs)); }
+
+// ---------------------------------------------------------------------------
+// layout kernels
+// ---------------------------------------------------------------------------
+// pad_and_unshuffle_8 / pixel_unshuffle (cat_and_pad.cu:7-51, shuffle.cu:134-185):
+// out channel co = ci*f*f + dy*f + dx; replicate padding by clamping.
+__global__ void pad_unshuffle_kernel(const f16* __restrict__ x, int ldx, int n_img, int H, int W, int Cin,
+                                     int f, int Ho, int Wo, f16* __restrict__ y, int ldy) {
+    const int Co = Cin * f * f;
+    const int CG = Co >> 3;
+    const long item = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long total = (long)n_img * Ho * Wo * CG;
+    if (item >= total) return;
+    const int cg = (int)(item % CG);
+    const long pix = item / CG;
+    const int wo = (int)(pix % Wo);
+    const int ho = (int)((pix / Wo) % Ho);
+    const int img = (int)(pix / ((long)Wo * Ho));
+    const f16* xb = x + (size_t)img * H * W * ldx;
+    f16x8 v;
+    const int ff = f * f;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int co = cg * 8 + e;
+        const int ci = co / ff;
+        const int r = co - ci * ff;
+        const int h = min(ho * f + r / f, H - 1);
+        const int w = min(wo * f + r % f, W - 1);
+        v[e] = xb[((size_t)h * W + w) * ldx + ci];
+    }
+    *reinterpret_cast<f16x8*>(y + pix * ldy + cg * 8) = v;
+}
+
+hipError_t pad_unshuffle(const f16* x, int ldx, int n_img, int H, int W, int Cin, int f, int padB,
+                         int padR, f16* y, int ldy, hipStream_t s) {
+    const int Ho = (H + padB) / f, Wo = (W + padR) / f;
+    const long total = (long)n_img * Ho * Wo * (Cin * f * f / 8);
+    hipLaunchKernelGGL(pad_unshuffle_kernel, grid1d(total), dim3(256), 0, s, x, ldx, n_img, H, W, Cin, f,
+                       Ho, Wo, y, ldy);
+    return hipGetLastError();
+}
+
+// pixel_shuffle_8 with clamp to [-0.5, 0.5] (shuffle.cu:224-316)
+__global__ void pixel_shuffle8_kernel(const f16* __restrict__ x, int ldx, int n_img, int H8, int W8, int C,
+                                      f16* __restrict__ y, int ldy) {
+    const int H = H8 * 8, W = W8 * 8;
+    const long pix = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= (long)n_img * H * W) return;
+    const int w = (int)(pix % W);
+    const int h = (int)((pix / W) % H);
+    const int img = (int)(pix / ((long)W * H));
+    const f16* src = x + (((size_t)img * H8 + (h >> 3)) * W8 + (w >> 3)) * ldx + (h & 7) * 8 + (w & 7);
+    for (int c = 0; c < C; ++c) {
+        float v = (float)src[c * 64];
+        v = fminf(fmaxf(v, -0.5f), 0.5f);
+        y[pix * ldy + c] = (f16)v;
+    }
+}
+
+hipError_t pixel_shuffle8_clamp(const f16* x, int ldx, int n_img, int H8, int W8, int C, f16* y, int ldy,
+                                hipStream_t s) {
+    const long total = (long)n_img * H8 * 8 * W8 * 8;
+    hipLaunchKernelGGL(pixel_shuffle8_kernel, grid1d(total), dim3(256), 0, s, x, ldx, n_img, H8, W8, C, y, ldy);
+    return hipGetLastError();
+}
+
+// multiply_with_broadcast_cuda (stream.cu:484-546): y = x * q[qp][c]
+__global__ void mul_channel_kernel(const f16* __restrict__ x, int ldx, long M, int C, const f16* __restrict__ q,
+                                   int qstride, const int* __restrict__ qp, f16* __restrict__ y, int ldy) {
+    const int CG = C >> 3;
+    const long item = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= M * CG) return;
+    const long m = item / CG;
+    const int c0 = (int)(item % CG) * 8;
+    const f16* qr = q + (size_t)(qp ? *qp : 0) * qstride + c0;
+    f16x8 a = *reinterpret_cast<const f16x8*>(x + m * ldx + c0);
+    f16x8 o;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) o[e] = (f16)((float)a[e] * (float)qr[e]);
+    *reinterpret_cast<f16x8*>(y + m * ldy + c0) = o;
+}
+
+hipError_t mul_channel(const f16* x, int ldx, long M, int C, const f16* q, int qstride, const int* qp, f16* y,
+                       int ldy, hipStream_t s) {
+    hipLaunchKernelGGL(mul_channel_kernel, grid1d(M * (C / 8)), dim3(256), 0, s, x, ldx, M, C, q, qstride, qp, y,
+                       ldy);
+    return hipGetLastError();
+}
+
+// round_z_cuda (stream.cu:862-884): clamp(round(z), -64, 63) as fp16 and int8 (NHWC-linear)
+__global__ void round_z_kernel(const f16* __restrict__ z, int ldz, long M, int C, f16* __restrict__ zhat, int ldh,
+                               int8_t* __restrict__ zi8) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M * C) return;
+    const long m = i / C;
+    const int c = (int)(i % C);
+    float v = round_half_away((float)z[m * ldz + c]);
+    v = fminf(fmaxf(v, -64.f), 63.f);
+    zhat[m * ldh + c] = (f16)v;
+    zi8[i] = (int8_t)v;
+}
+
+hipError_t round_z(const f16* z, int ldz, long M, int C, f16* zhat, int ldh, int8_t* zi8, hipStream_t s) {
+    hipLaunchKernelGGL(round_z_kernel, grid1d(M * C), dim3(256), 0, s, z, ldz, M, C, zhat, ldh, zi8);
+    return hipGetLastError();
+}
+
+// int8_to_dtype_cuda (stream.cu:454-482)
+__global__ void i8_to_f16_kernel(const int8_t* __restrict__ zi8, long M, int C, f16* __restrict__ zhat, int ldh) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M * C) return;
+    zhat[(i / C) * ldh + (i % C)] = (f16)(float)zi8[i];
+}
+
+hipError_t i8_to_f16(const int8_t* zi8, long M, int C, f16* zhat, int ldh, hipStream_t s) {
+    hipLaunchKernelGGL(i8_to_f16_kernel, grid1d(M * C), dim3(256), 0, s, zi8, M, C, zhat, ldh);
+    return hipGetLastError();
+}
+
+// replicate_pad_cuda / slice_cuda (cat_and_pad.cu:53-110): y[h][w] = x[min(h,H-1)][min(w,W-1)]
+__global__ void replicate_pad_kernel(const f16* __restrict__ x, int ldx, int n_img, int H, int W, int C, int Ho,
+                                     int Wo, f16* __restrict__ y, int ldy) {
+    const int CG = C >> 3;
+    const long item = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= (long)n_img * Ho * Wo * CG) return;
+    const int cg = (int)(item % CG);
+    const long pix = item / CG;
+    const int wo = (int)(pix % Wo);
+    const int ho = (int)((pix / Wo) % Ho);
+    const int img = (int)(pix / ((long)Wo * Ho));
+    const int h = min(ho, H - 1), w = min(wo, W - 1);
+    *reinterpret_cast<f16x8*>(y + pix * ldy + cg * 8) =
+        *reinterpret_cast<const f16x8*>(x + (((size_t)img * H + h) * W + w) * ldx + cg * 8);
+}
+
+hipError_t replicate_pad(const f16* x, int ldx, int n_img, int H, int W, int C, int Ho, int Wo, f16* y, int ldy,
+                         hipStream_t s) {
+    const long total = (long)n_img * Ho * Wo * (C / 8);
+    hipLaunchKernelGGL(replicate_pad_kernel, grid1d(total), dim3(256), 0, s, x, ldx, n_img, H, W, C, Ho, Wo, y, ldy);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// HT-S quantisation boundary (dmc_hts_proxy.cpp:529-556, 676-696)
+// ---------------------------------------------------------------------------
+// Encoder step k over all (pixel, channel):
+//   active = channel group (c / (C/4)) is the one step k uses at (h&1, w&1)
+//   ys     = y / max(q_dec, 0.5)                          (stream.cu:422-443)
+//   y_q    = clamp(round(ys - mean) * [scale > thr], -128, 127)
+//   y_hat  = y_q + mean (active), unchanged (inactive), 0 for inactive at k=0;
+//   k=3 multiplies every y_hat by max(q_dec, 0.5)          (stream.cu:548-630)
+//   sym    = (y_q << 8) + index(scale), cond = scale > thr   (stream.cu:130-161)
+__global__ void hts_enc_step_kernel(Act y, const f16* __restrict__ scales, int lds, const f16* __restrict__ means,
+                                    int ldm, const f16* __restrict__ qdec, int ldq, f16* __restrict__ yhat, int ldh,
+                                    int16_t* __restrict__ sym, uint8_t* __restrict__ cond,
+                                    const float* __restrict__ th, float thr, int step) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int C = y.C;
+    if (i >= y.pixels() * C) return;
+    const long m = i / C;
+    const int c = (int)(i % C);
+    const int w = (int)(m % y.W);
+    const int h = (int)((m / y.W) % y.H);
+    const int pos = ((h & 1) << 1) | (w & 1);
+    const bool active = mask4_pos(step, c / (C >> 2)) == pos;
+    const float q = qstep((float)qdec[m * ldq + c]);
+    float out;
+    if (active) {
+        const float ys = (float)y.p[m * y.ld + c] * (1.0f / q);
+        const float mean = (float)means[m * ldm + c];
+        const float sc = (float)scales[m * lds + c];
+        const int yq = quant_elem(ys, mean, sc, thr);
+        sym[i] = (int16_t)((yq << 8) + scale_index(sc, th));
+        cond[i] = sc > thr ? 1 : 0;
+        out = (float)yhat_elem(yq, mean);
+    } else {
+        out = step == 0 ? 0.f : (float)yhat[m * ldh + c];
+    }
+    if (step == 3) out = out * q;
+    if (active || step == 0 || step == 3) yhat[m * ldh + c] = (f16)out;
+}
+
+hipError_t hts_enc_step(const Act& y, const f16* scales, int lds, const f16* means, int ldm, const f16* qdec,
+                        int ldq, f16* yhat, int ldh, int16_t* sym, uint8_t* cond, const float* th, float thr,
+                        int step, hipStream_t s) {
+    hipLaunchKernelGGL(hts_enc_step_kernel, grid1d(y.pixels() * y.C), dim3(256), 0, s, y, scales, lds, means, ldm,
+                       qdec, ldq, yhat, ldh, sym, cond, th, thr, step);
+    return hipGetLastError();
+}
+
+// build_index_dec_cuda (stream.cu:89-115): dense NHWC-linear index + condition
+__global__ void hts_dec_index_kernel(Act sc, const float* __restrict__ th, float thr, uint8_t* __restrict__ idx,
+                                     uint8_t* __restrict__ cond) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= sc.pixels() * sc.C) return;
+    const float s = (float)sc.p[(i / sc.C) * sc.ld + (i % sc.C)];
+    idx[i] = (uint8_t)scale_index(s, th);
+    cond[i] = s > thr ? 1 : 0;
+}
+
+hipError_t hts_dec_index(const Act& scales, const float* th, float thr, uint8_t* idx, uint8_t* cond, hipStream_t s) {
+    hipLaunchKernelGGL(hts_dec_index_kernel, grid1d(scales.pixels() * scales.C), dim3(256), 0, s, scales, th, thr,
+                       idx, cond);
+    return hipGetLastError();
+}
+
+// restore_y* (stream.cu:685-754) in the encoder's rounding order
+__global__ void hts_dec_step_kernel(Act g, const int8_t* __restrict__ yq, const f16* __restrict__ means, int ldm,
+                                    const f16* __restrict__ qdec, int ldq, f16* __restrict__ yhat, int ldh, int step) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int C = g.C;
+    if (i >= g.pixels() * C) return;
+    const long m = i / C;
+    const int c = (int)(i % C);
+    const int w = (int)(m % g.W);
+    const int h = (int)((m / g.W) % g.H);
+    const int pos = ((h & 1) << 1) | (w & 1);
+    const bool active = mask4_pos(step, c / (C >> 2)) == pos;
+    float out;
+    if (active) {
+        out = (float)yhat_elem((int)yq[i], (float)means[m * ldm + c]);
+    } else {
+        out = step == 0 ? 0.f : (float)yhat[m * ldh + c];
+    }
+    if (step == 3) out = out * qstep((float)qdec[m * ldq + c]);
+    if (active || step == 0 || step == 3) yhat[m * ldh + c] = (f16)out;
+}
+
+hipError_t hts_dec_step(const Act& g, const int8_t* yq, const f16* means, int ldm, const f16* qdec, int ldq,
+                        f16* yhat, int ldh, int step, hipStream_t s) {
+    hipLaunchKernelGGL(hts_dec_step_kernel, grid1d(g.pixels() * g.C), dim3(256), 0, s, g, yq, means, ldm, qdec, ldq,
+                       yhat, ldh, step);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// DMCI quantisation boundary over quarter tensors (dmci_proxy.cpp:333-393, 497-597)
+// Quarter element (pixel m, channel c' < C/4) maps to channel g*C/4 + c' of the
+// one group g active at the pixel's 2x2 position in step k
+// (single_part_for_writing/reading_4x, stream.cu:896-962).
+// ---------------------------------------------------------------------------
+__global__ void intra_enc_step_kernel(Act y, const f16* __restrict__ scales, int lds, const f16* __restrict__ means,
+                                      int ldm, f16* __restrict__ ysf, int ldsf, const f16* __restrict__ q,
+                                      int qstride, const int* __restrict__ qp, f16* __restrict__ yout, int ldo,
+                                      int16_t* __restrict__ sym, uint8_t* __restrict__ cond,
+                                      const float* __restrict__ th, float thr, int step) {
+    const int C4 = y.C >> 2;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= y.pixels() * C4) return;
+    const long m = i / C4;
+    const int cq = (int)(i % C4);
+    const int w = (int)(m % y.W);
+    const int h = (int)((m / y.W) % y.H);
+    const int pos = ((h & 1) << 1) | (w & 1);
+    const int ga = mask4_group(step, pos);
+    const int ca = ga * C4 + cq;
+    const float mean = (float)means[m * ldm + ca];
+    const float sc = (float)scales[m * lds + ca];
+    const int yq = quant_elem((float)y.p[m * y.ld + ca], mean, sc, thr);
+    sym[i] = (int16_t)((yq << 8) + scale_index(sc, th));
+    cond[i] = sc > thr ? 1 : 0;
+    const f16 yh = yhat_elem(yq, mean);
+    if (step < 3) {
+        if (step == 0) {
+#pragma unroll
+            for (int g = 0; g < 4; ++g) ysf[m * ldsf + g * C4 + cq] = (g == ga) ? yh : (f16)0.f;
+        } else {
+            ysf[m * ldsf + ca] = yh;
+        }
+    } else {
+        const f16* qr = q + (size_t)(qp ? *qp : 0) * qstride;
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const int c = g * C4 + cq;
+            const float v = (g == ga) ? (float)yh : (float)ysf[m * ldsf + c];
+            yout[m * ldo + c] = (f16)(v * (float)qr[c]);
+        }
+    }
+}
+
+hipError_t intra_enc_step(const Act& y, const f16* scales, int lds, const f16* means, int ldm, f16* ysf, int ldsf,
+                          const f16* q, int qstride, const int* qp, f16* yout, int ldo, int16_t* sym, uint8_t* cond,
+                          const float* th, float thr, int step, hipStream_t s) {
+    hipLaunchKernelGGL(intra_enc_step_kernel, grid1d(y.pixels() * (y.C / 4)), dim3(256), 0, s, y, scales, lds, means,
+                       ldm, ysf, ldsf, q, qstride, qp, yout, ldo, sym, cond, th, thr, step);
+    return hipGetLastError();
+}
+
+__global__ void intra_dec_index_kernel(Act g, const f16* __restrict__ scales, int lds, const float* __restrict__ th,
+                                       float thr, int step, uint8_t* __restrict__ idx, uint8_t* __restrict__ cond) {
+    const int C4 = g.C >> 2;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.pixels() * C4) return;
+    const long m = i / C4;
+    const int cq = (int)(i % C4);
+    const int w = (int)(m % g.W);
+    const int h = (int)((m / g.W) % g.H);
+    const int ga = mask4_group(step, ((h & 1) << 1) | (w & 1));
+    const float sc = (float)scales[m * lds + ga * C4 + cq];
+    idx[i] = (uint8_t)scale_index(sc, th);
+    cond[i] = sc > thr ? 1 : 0;
+}
+
+hipError_t intra_dec_index(const Act& g, const f16* scales, int lds, const float* th, float thr, int step,
+                           uint8_t* idx, uint8_t* cond, hipStream_t s) {
+    hipLaunchKernelGGL(intra_dec_index_kernel, grid1d(g.pixels() * (g.C / 4)), dim3(256), 0, s, g, scales, lds, th,
+                       thr, step, idx, cond);
+    return hipGetLastError();
+}
+
+__global__ void intra_dec_step_kernel(Act g, const int8_t* __restrict__ yq, const f16* __restrict__ means, int ldm,
+                                      f16* __restrict__ ysf, int ldsf, const f16* __restrict__ q, int qstride,
+                                      const int* __restrict__ qp, f16* __restrict__ yout, int ldo, int step) {
+    const int C4 = g.C >> 2;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.pixels() * C4) return;
+    const long m = i / C4;
+    const int cq = (int)(i % C4);
+    const int w = (int)(m % g.W);
+    const int h = (int)((m / g.W) % g.H);
+    const int ga = mask4_group(step, ((h & 1) << 1) | (w & 1));
+    const int ca = ga * C4 + cq;
+    const f16 yh = yhat_elem((int)yq[i], (float)means[m * ldm + ca]);
+    if (step < 3) {
+        if (step == 0) {
+#pragma unroll
+            for (int gg = 0; gg < 4; ++gg) ysf[m * ldsf + gg * C4 + cq] = (gg == ga) ? yh : (f16)0.f;
+        } else {
+            ysf[m * ldsf + ca] = yh;
+        }
+    } else {
+        const f16* qr = q + (size_t)(qp ? *qp : 0) * qstride;
+#pragma unroll
+        for (int gg = 0; gg < 4; ++gg) {
+            const int c = gg * C4 + cq;
+            const float v = (gg == ga) ? (float)yh : (float)ysf[m * ldsf + c];
+            yout[m * ldo + c] = (f16)(v * (float)qr[c]);
+        }
+    }
+}
+
+hipError_t intra_dec_step(const Act& g, const int8_t* yq, const f16* means, int ldm, f16* ysf, int ldsf,
+                          const f16* q, int qstride, const int* qp, f16* yout, int ldo, int step, hipStream_t s) {
+    hipLaunchKernelGGL(intra_dec_step_kernel, grid1d(g.pixels() * (g.C / 4)), dim3(256), 0, s, g, yq, means, ldm,
+                       ysf, ldsf, q, qstride, qp, yout, ldo, step);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// stable stream compaction (conditional_index_part1/2, stream.cu:175-357) and
+// expansion (conditional_recover_with_type_conversion, stream.cu:359-420).
+// Tile = 256 threads x 16 consecutive elements; block offsets are an exclusive
+// scan of per-tile counts, so the output is in NHWC-linear order.
+// ---------------------------------------------------------------------------
+constexpr int CPT = 16;  // elements per thread
+static_assert(COMPACT_TILE == 256 * CPT, "tile");
+
+DCVC_DEV int block_exclusive_scan(int v, int* sh, int* total) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        int add = (t >= off) ? sh[t - off] : 0;
+        __syncthreads();
+        sh[t] += add;
+        __syncthreads();
+    }
+    const int incl = sh[t];
+    if (total) *total = sh[255];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ __launch_bounds__(256) void compact_count_kernel(const uint8_t* __restrict__ cond, long N,
+                                                            int* __restrict__ block_counts) {
+    __shared__ int sh[256];
+    const long base = (long)blockIdx.x * COMPACT_TILE + threadIdx.x * CPT;
+    int c = 0;
+    for (int e = 0; e < CPT; ++e) {
+        const long i = base + e;
+        if (i < N) c += cond[i] ? 1 : 0;
+    }
+    int total;
+    block_exclusive_scan(c, sh, &total);
+    if (threadIdx.x == 0) block_counts[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void compact_scan_kernel(int* __restrict__ offs, int nb, int* __restrict__ count) {
+    __shared__ int sh[1024];
+    const int t = threadIdx.x;
+    const int per = (nb + 1023) / 1024;
+    int local = 0;
+    for (int e = 0; e < per; ++e) {
+        const int b = t * per + e;
+        if (b < nb) local += offs[b];
+    }
+    sh[t] = local;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int add = (t >= off) ? sh[t - off] : 0;
+        __syncthreads();
+        sh[t] += add;
+        __syncthreads();
+    }
+    int run = sh[t] - local;
+    if (t == 1023) *count = sh[1023];
+    for (int e = 0; e < per; ++e) {
+        const int b = t * per + e;
+        if (b < nb) {
+            const int v = offs[b];
+            offs[b] = run;
+            run += v;
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void compact_scatter_kernel(const T* __restrict__ vals,
+                                                              const uint8_t* __restrict__ cond, long N,
+                                                              const int* __restrict__ offs, T* __restrict__ out) {
+    __shared__ int sh[256];
+    const long base = (long)blockIdx.x * COMPACT_TILE + threadIdx.x * CPT;
+    int c = 0;
+    for (int e = 0; e < CPT; ++e) {
+        const long i = base + e;
+        if (i < N) c += cond[i] ? 1 : 0;
+    }
+    int dst = offs[blockIdx.x] + block_exclusive_scan(c, sh, nullptr);
+    for (int e = 0; e < CPT; ++e) {
+        const long i = base + e;
+        if (i < N && cond[i]) out[dst++] = vals[i];
+    }
+}
+
+__global__ __launch_bounds__(256) void recover_kernel(const int8_t* __restrict__ dec, const uint8_t* __restrict__ cond,
+                                                      long N, const int* __restrict__ offs,
+                                                      int8_t* __restrict__ out) {
+    __shared__ int sh[256];
+    const long base = (long)blockIdx.x * COMPACT_TILE + threadIdx.x * CPT;
+    int c = 0;
+    for (int e = 0; e < CPT; ++e) {
+        const long i = base + e;
+        if (i < N) c += cond[i] ? 1 : 0;
+    }
+    int src = offs[blockIdx.x] + block_exclusive_scan(c, sh, nullptr);
+    for (int e = 0; e < CPT; ++e) {
+        const long i = base + e;
+        if (i < N) out[i] = cond[i] ? dec[src++] : (int8_t)0;
+    }
+}
+
+hipError_t compact(const void* vals, int esize, const uint8_t* cond, long N, void* out, int* offs, int* count,
+                   hipStream_t s) {
+    const int nb = compact_blocks(N);
+    if (nb > 0) hipLaunchKernelGGL(compact_count_kernel, dim3(nb), dim3(256), 0, s, cond, N, offs);
+    hipLaunchKernelGGL(compact_scan_kernel, dim3(1), dim3(1024), 0, s, offs, nb, count);
+    if (nb == 0) return hipGetLastError();
+    if (esize == 2)
+        hipLaunchKernelGGL(compact_scatter_kernel<int16_t>, dim3(nb), dim3(256), 0, s, (const int16_t*)vals, cond, N,
+                           offs, (int16_t*)out);
+    else
+        hipLaunchKernelGGL(compact_scatter_kernel<uint8_t>, dim3(nb), dim3(256), 0, s, (const uint8_t*)vals, cond, N,
+                           offs, (uint8_t*)out);
+    return hipGetLastError();
+}
+
+hipError_t recover(const int8_t* dec, const uint8_t* cond, long N, const int* offs, int8_t* out, hipStream_t s) {
+    if (N <= 0) return hipSuccess;
+    hipLaunchKernelGGL(recover_kernel, dim3(compact_blocks(N)), dim3(256), 0, s, dec, cond, N, offs, out);
+    return hipGetLastError();
+}
+
+}  // namespace dcvc

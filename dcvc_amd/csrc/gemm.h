@@ -1,0 +1,53 @@
+// Internal interface of the MFMA GEMM family (see gemm.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "common.h"
+
+namespace dcvc {
+
+enum GemmEpi {
+    EPI_BIAS = 0,            // acc + b
+    EPI_WSILU = 1,           // wsilu(acc + b)
+    EPI_SHORTCUT = 2,        // acc + b + S1
+    EPI_SHORTCUT2 = 3,       // acc + b + S1 + S2
+    EPI_SHORTCUT_QUANT = 4,  // (acc + b + S1) * q[n]
+    EPI_QUANT = 5,           // (acc + b) * q[n]
+    EPI_QUAD = 6,            // sum_j wsilu(acc_j + b_j), B rows 4n + j
+};
+
+enum GemmAMode {
+    A_PLAIN = 0,  // A[m, k] = A + m * lda + k
+    A_S2K2 = 1,   // 2x2 stride-2 conv, K = 4 * cin, tap-major
+    A_S2K3 = 2,   // 3x3 stride-2 pad-1 conv, K = 9 * cin, tap-major
+    A_S1K3 = 3,   // 3x3 stride-1 pad-1 conv, K = 9 * cin, tap-major
+};
+
+enum GemmDMode {
+    D_PLAIN = 0,   // D[m, n] = D + m * ldd + n
+    D_SUBPEL = 1,  // n = sub * (N/4) + c stored at pixel (2h + sub/2, 2w + sub%2) of the 2x grid
+};
+
+struct GemmArgs {
+    const f16* A = nullptr;
+    int lda = 0;
+    int M = 0, N = 0, K = 0;
+    // geometry: GEMM rows are pixels of an (outH x outW) grid, images stacked;
+    // implicit-conv modes read an (inH x inW) input grid with cin channels.
+    int inH = 0, inW = 0, outH = 1, outW = 1, cin = 0;
+    const f16* B = nullptr;
+    const float* bias = nullptr;
+    const f16* S1 = nullptr;
+    int lds1 = 0;
+    const f16* S2 = nullptr;
+    int lds2 = 0;
+    const f16* q = nullptr;     // per-QP table, row stride qstride
+    int qstride = 0;
+    const int* qp = nullptr;    // device QP index (nullptr -> row 0)
+    f16* D = nullptr;
+    int ldd = 0;
+};
+
+int gemm_validate(const GemmArgs& g, int epi, int amode, int dmode);
+hipError_t gemm_launch(const GemmArgs& g, int epi, int amode, int dmode, hipStream_t stream);
+
+}  // namespace dcvc

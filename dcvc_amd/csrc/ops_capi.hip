@@ -1,0 +1,286 @@
+// extern "C" kernel-level entry points declared in include/dcvc_ops.h.
+#include "../../include/dcvc_ops.h"
+
+#include "common.h"
+#include "dwconv.h"
+#include "elementwise.h"
+#include "gemm.h"
+
+using namespace dcvc;
+
+static inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+static inline int st(hipError_t e) { return e == hipSuccess ? DCVC_OK : DCVC_ERR_HIP; }
+static inline const f16* H(const void* p) { return reinterpret_cast<const f16*>(p); }
+static inline f16* Hm(void* p) { return reinterpret_cast<f16*>(p); }
+
+static int run_gemm(GemmArgs& g, int epi, int amode, int dmode, void* stream) {
+    const int v = gemm_validate(g, epi, amode, dmode);
+    if (v != DCVC_OK) return v;
+    return st(gemm_launch(g, epi, amode, dmode, S(stream)));
+}
+
+static GemmArgs plain_args(const void* A, int lda, int M, int K, const void* W, int N, const float* bias, void* D,
+                           int ldd) {
+    GemmArgs g;
+    g.A = H(A);
+    g.lda = lda;
+    g.M = M;
+    g.K = K;
+    g.B = H(W);
+    g.N = N;
+    g.bias = bias;
+    g.D = Hm(D);
+    g.ldd = ldd;
+    g.outH = 1;
+    g.outW = M;
+    return g;
+}
+
+extern "C" {
+
+int dcvc_version(void) { return 100; }
+
+int dcvc_scale_index_thresholds(float* th128) {
+    if (!th128) return DCVC_ERR_ARG;
+    host_scale_index_thresholds(th128);
+    return DCVC_OK;
+}
+
+int dcvc_conv1x1_bias(const void* A, int lda, int M, int K, const void* W, int N, const float* bias, void* D,
+                      int ldd, void* stream) {
+    GemmArgs g = plain_args(A, lda, M, K, W, N, bias, D, ldd);
+    return run_gemm(g, EPI_BIAS, A_PLAIN, D_PLAIN, stream);
+}
+
+int dcvc_conv1x1_bias_wsilu(const void* A, int lda, int M, int K, const void* W, int N, const float* bias, void* D,
+                            int ldd, void* stream) {
+    GemmArgs g = plain_args(A, lda, M, K, W, N, bias, D, ldd);
+    return run_gemm(g, EPI_WSILU, A_PLAIN, D_PLAIN, stream);
+}
+
+int dcvc_conv1x1_bias_shortcut(const void* A, int lda, int M, int K, const void* W, int N, const float* bias,
+                               const void* Sc, int lds, void* D, int ldd, void* stream) {
+    GemmArgs g = plain_args(A, lda, M, K, W, N, bias, D, ldd);
+    g.S1 = H(Sc);
+    g.lds1 = lds;
+    return run_gemm(g, EPI_SHORTCUT, A_PLAIN, D_PLAIN, stream);
+}
+
+int dcvc_conv1x1_bias_shortcut2(const void* A, int lda, int M, int K, const void* W, int N, const float* bias,
+                                const void* S1, int lds1, const void* S2, int lds2, void* D, int ldd,
+                                void* stream) {
+    GemmArgs g = plain_args(A, lda, M, K, W, N, bias, D, ldd);
+    g.S1 = H(S1);
+    g.lds1 = lds1;
+    g.S2 = H(S2);
+    g.lds2 = lds2;
+    return run_gemm(g, EPI_SHORTCUT2, A_PLAIN, D_PLAIN, stream);
+}
+
+int dcvc_conv1x1_bias_shortcut_with_quant(const void* A, int lda, int M, int K, const void* W, int N,
+                                          const float* bias, const void* Sc, int lds, const void* q, int qstride,
+                                          const int* qp, void* D, int ldd, void* stream) {
+    GemmArgs g = plain_args(A, lda, M, K, W, N, bias, D, ldd);
+    g.S1 = H(Sc);
+    g.lds1 = lds;
+    g.q = H(q);
+    g.qstride = qstride;
+    g.qp = qp;
+    return run_gemm(g, EPI_SHORTCUT_QUANT, A_PLAIN, D_PLAIN, stream);
+}
+
+int dcvc_conv1x1_bias_with_quant(const void* A, int lda, int M, int K, const void* W, int N, const float* bias,
+                                 const void* q, int qstride, const int* qp, void* D, int ldd, void* stream) {
+    GemmArgs g = plain_args(A, lda, M, K, W, N, bias, D, ldd);
+    g.q = H(q);
+    g.qstride = qstride;
+    g.qp = qp;
+    return run_gemm(g, EPI_QUANT, A_PLAIN, D_PLAIN, stream);
+}
+
+int dcvc_conv1x1_bias_wsilu_chunk_add(const void* A, int lda, int M, int K, const void* W, int N,
+                                      const float* bias, void* D, int ldd, void* stream) {
+    GemmArgs g = plain_args(A, lda, M, K, W, N, bias, D, ldd);
+    return run_gemm(g, EPI_QUAD, A_PLAIN, D_PLAIN, stream);
+}
+
+int dcvc_conv_bias(const void* X, int ldx, int n_img, int H_, int W_, int Cin, const void* Wt, int N,
+                   const float* bias, int kernel, int stride, void* D, int ldd, void* stream) {
+    int amode;
+    int Ho, Wo;
+    if (kernel == 2 && stride == 2) {
+        amode = A_S2K2;
+        if (H_ % 2 || W_ % 2) return DCVC_ERR_ARG;
+        Ho = H_ / 2;
+        Wo = W_ / 2;
+    } else if (kernel == 3 && stride == 2) {
+        amode = A_S2K3;
+        Ho = (H_ + 1) / 2;
+        Wo = (W_ + 1) / 2;
+    } else if (kernel == 3 && stride == 1) {
+        amode = A_S1K3;
+        Ho = H_;
+        Wo = W_;
+    } else {
+        return DCVC_ERR_ARG;
+    }
+    GemmArgs g;
+    g.A = H(X);
+    g.lda = ldx;
+    g.inH = H_;
+    g.inW = W_;
+    g.outH = Ho;
+    g.outW = Wo;
+    g.cin = Cin;
+    g.M = n_img * Ho * Wo;
+    g.K = kernel * kernel * Cin;
+    g.B = H(Wt);
+    g.N = N;
+    g.bias = bias;
+    g.D = Hm(D);
+    g.ldd = ldd;
+    return run_gemm(g, EPI_BIAS, amode, D_PLAIN, stream);
+}
+
+int dcvc_transposed_conv(const void* X, int ldx, int n_img, int H_, int W_, int Cin, const void* Wt, int Cout,
+                         const float* bias, void* D, int ldd, void* stream) {
+    GemmArgs g;
+    g.A = H(X);
+    g.lda = ldx;
+    g.outH = H_;
+    g.outW = W_;
+    g.M = n_img * H_ * W_;
+    g.K = Cin;
+    g.B = H(Wt);
+    g.N = 4 * Cout;
+    g.bias = bias;
+    g.D = Hm(D);
+    g.ldd = ldd;
+    if (!bias) return DCVC_ERR_ARG;  // callers pass a zero vector for bias-free layers
+    return run_gemm(g, EPI_BIAS, A_PLAIN, D_SUBPEL, stream);
+}
+
+int dcvc_d3x3(const void* X, int ldx, int n_img, int H_, int W_, int C, const void* Wt, void* Y, int ldy,
+              void* stream) {
+    DwArgs a;
+    a.x = H(X);
+    a.ldx = ldx;
+    a.wt = H(Wt);
+    a.y = Hm(Y);
+    a.ldy = ldy;
+    a.n_img = n_img;
+    a.H = H_;
+    a.W = W_;
+    a.C = C;
+    const int v = dw3x3_validate(a);
+    if (v != DCVC_OK) return v;
+    return st(dw3x3_launch(a, S(stream)));
+}
+
+int dcvc_pad_and_unshuffle(const void* X, int ldx, int n_img, int H_, int W_, int Cin, int f, int padB, int padR,
+                           void* Y, int ldy, void* stream) {
+    if (!X || !Y || f <= 0 || padB < 0 || padR < 0 || (H_ + padB) % f || (W_ + padR) % f) return DCVC_ERR_ARG;
+    if ((Cin * f * f) % 8 || ldy % 8 || ldy < Cin * f * f || ldx < Cin) return DCVC_ERR_ARG;
+    return st(pad_unshuffle(H(X), ldx, n_img, H_, W_, Cin, f, padB, padR, Hm(Y), ldy, S(stream)));
+}
+
+int dcvc_pixel_shuffle_8(const void* X, int ldx, int n_img, int H8, int W8, int C, void* Y, int ldy, void* stream) {
+    if (!X || !Y || ldx < C * 64 || ldy < C) return DCVC_ERR_ARG;
+    return st(pixel_shuffle8_clamp(H(X), ldx, n_img, H8, W8, C, Hm(Y), ldy, S(stream)));
+}
+
+int dcvc_multiply_with_broadcast(const void* X, int ldx, long M, int C, const void* q, int qstride, const int* qp,
+                                 void* Y, int ldy, void* stream) {
+    if (!X || !Y || !q || C % 8 || ldx % 8 || ldy % 8 || ldx < C || ldy < C) return DCVC_ERR_ARG;
+    return st(mul_channel(H(X), ldx, M, C, H(q), qstride, qp, Hm(Y), ldy, S(stream)));
+}
+
+int dcvc_round_z(const void* Z, int ldz, long M, int C, void* Zhat, int ldh, int8_t* Zi8, void* stream) {
+    if (!Z || !Zhat || !Zi8) return DCVC_ERR_ARG;
+    return st(round_z(H(Z), ldz, M, C, Hm(Zhat), ldh, Zi8, S(stream)));
+}
+
+int dcvc_int8_to_half(const int8_t* Zi8, long M, int C, void* Zhat, int ldh, void* stream) {
+    if (!Zi8 || !Zhat) return DCVC_ERR_ARG;
+    return st(i8_to_f16(Zi8, M, C, Hm(Zhat), ldh, S(stream)));
+}
+
+int dcvc_replicate_pad(const void* X, int ldx, int n_img, int H_, int W_, int C, int Ho, int Wo, void* Y, int ldy,
+                       void* stream) {
+    if (!X || !Y || C % 8 || ldx % 8 || ldy % 8) return DCVC_ERR_ARG;
+    return st(replicate_pad(H(X), ldx, n_img, H_, W_, C, Ho, Wo, Hm(Y), ldy, S(stream)));
+}
+
+static Act act(const void* p, int ld, int n_img, int H_, int W_, int C) {
+    Act a;
+    a.p = const_cast<f16*>(H(p));
+    a.ld = ld;
+    a.n_img = n_img;
+    a.H = H_;
+    a.W = W_;
+    a.C = C;
+    return a;
+}
+
+int dcvc_hts_quant_step(const void* Y, int ldy, int n_img, int H_, int W_, int C, const void* Sc, int lds,
+                        const void* Mn, int ldm, const void* Qd, int ldq, void* Yhat, int ldh, int16_t* sym,
+                        uint8_t* cond, const float* th, float thr, int step, void* stream) {
+    if (!Y || !Sc || !Mn || !Qd || !Yhat || !sym || !cond || !th || step < 0 || step > 3 || C % 4) return DCVC_ERR_ARG;
+    return st(hts_enc_step(act(Y, ldy, n_img, H_, W_, C), H(Sc), lds, H(Mn), ldm, H(Qd), ldq, Hm(Yhat), ldh, sym,
+                           cond, th, thr, step, S(stream)));
+}
+
+int dcvc_hts_build_index_dec(const void* Sc, int lds, int n_img, int H_, int W_, int C, const float* th, float thr,
+                             uint8_t* idx, uint8_t* cond, void* stream) {
+    if (!Sc || !idx || !cond || !th) return DCVC_ERR_ARG;
+    return st(hts_dec_index(act(Sc, lds, n_img, H_, W_, C), th, thr, idx, cond, S(stream)));
+}
+
+int dcvc_hts_restore_step(const int8_t* yq, int n_img, int H_, int W_, int C, const void* Mn, int ldm,
+                          const void* Qd, int ldq, void* Yhat, int ldh, int step, void* stream) {
+    if (!yq || !Mn || !Qd || !Yhat || step < 0 || step > 3 || C % 4) return DCVC_ERR_ARG;
+    return st(hts_dec_step(act(nullptr, 0, n_img, H_, W_, C), yq, H(Mn), ldm, H(Qd), ldq, Hm(Yhat), ldh, step,
+                           S(stream)));
+}
+
+int dcvc_intra_quant_step(const void* Y, int ldy, int n_img, int H_, int W_, int C, const void* Sc, int lds,
+                          const void* Mn, int ldm, void* Ysf, int ldsf, const void* q, int qstride, const int* qp,
+                          void* Yout, int ldo, int16_t* sym, uint8_t* cond, const float* th, float thr, int step,
+                          void* stream) {
+    if (!Y || !Sc || !Mn || !Ysf || !sym || !cond || !th || step < 0 || step > 3 || C % 4) return DCVC_ERR_ARG;
+    if (step == 3 && (!q || !Yout)) return DCVC_ERR_ARG;
+    return st(intra_enc_step(act(Y, ldy, n_img, H_, W_, C), H(Sc), lds, H(Mn), ldm, Hm(Ysf), ldsf, H(q), qstride,
+                             qp, Hm(Yout), ldo, sym, cond, th, thr, step, S(stream)));
+}
+
+int dcvc_intra_build_index_dec(const void* Sc, int lds, int n_img, int H_, int W_, int C, const float* th,
+                               float thr, int step, uint8_t* idx, uint8_t* cond, void* stream) {
+    if (!Sc || !idx || !cond || !th || step < 0 || step > 3 || C % 4) return DCVC_ERR_ARG;
+    return st(intra_dec_index(act(nullptr, 0, n_img, H_, W_, C), H(Sc), lds, th, thr, step, idx, cond, S(stream)));
+}
+
+int dcvc_intra_restore_step(const int8_t* yq, int n_img, int H_, int W_, int C, const void* Mn, int ldm, void* Ysf,
+                            int ldsf, const void* q, int qstride, const int* qp, void* Yout, int ldo, int step,
+                            void* stream) {
+    if (!yq || !Mn || !Ysf || step < 0 || step > 3 || C % 4) return DCVC_ERR_ARG;
+    if (step == 3 && (!q || !Yout)) return DCVC_ERR_ARG;
+    return st(intra_dec_step(act(nullptr, 0, n_img, H_, W_, C), yq, H(Mn), ldm, Hm(Ysf), ldsf, H(q), qstride, qp,
+                             Hm(Yout), ldo, step, S(stream)));
+}
+
+int dcvc_compact_blocks(long N) { return compact_blocks(N); }
+
+int dcvc_conditional_index(const void* vals, int esize, const uint8_t* cond, long N, void* out, int* block_offsets,
+                           int* count, void* stream) {
+    if ((esize != 1 && esize != 2) || N < 0 || !count || (N > 0 && (!vals || !cond || !out || !block_offsets)))
+        return DCVC_ERR_ARG;
+    return st(compact(vals, esize, cond, N, out, block_offsets, count, S(stream)));
+}
+
+int dcvc_conditional_recover(const int8_t* dec, const uint8_t* cond, long N, const int* block_offsets, int8_t* out,
+                             void* stream) {
+    if (N < 0 || (N > 0 && (!dec || !cond || !block_offsets || !out))) return DCVC_ERR_ARG;
+    return st(recover(dec, cond, N, block_offsets, out, S(stream)));
+}
+
+}  // extern "C"

@@ -1,0 +1,160 @@
+/*
+ * dcvc_ops.h — kernel-level C ABI of the MI355X DCVC-UF hot path.
+ *
+ * Each entry point replaces one free function of the reference's
+ * inference_extensions_cuda operator layer (src/layers/extensions/inference/
+ * def_cutlass.h:10-39 and def_elementwise.h:10-78), taking plain device
+ * pointers, element strides and a hipStream_t (as void*) instead of
+ * at::Tensor.  Activations are NHWC fp16: a feature map of C channels is an
+ * [pixels x C] matrix whose rows are `ld` halves apart (ld >= C), so channel
+ * slices of a concatenation buffer are passed as (pointer + offset, ld).
+ *
+ * All functions are asynchronous on `stream` and return 0 (DCVC_OK) or a
+ * nonzero status (1 bad argument, 2 HIP failure).  Nothing here allocates.
+ */
+#ifndef DCVC_OPS_H
+#define DCVC_OPS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Library version (major * 10000 + minor * 100 + patch). */
+int dcvc_version(void);
+
+/* Fills th[0..127] with the float32 scale->CDF-index thresholds:
+ * index(s) = #{k >= 1 : s >= th[k]}, the FP32 value of
+ * floor((ln(clamp(s, 0.11, 16)) + 2.2073) * 127 / 4.9799)
+ * (def_const.h:6-11, elementwise/stream.cu:77-87).  Host-only. */
+int dcvc_scale_index_thresholds(float* th128);
+
+/* ---------------- 1x1-conv GEMMs (MFMA, fp32 accumulate) ----------------
+ * A [M x K] fp16 (row stride lda), W [N x K] fp16 (PyTorch [N,K,1,1]),
+ * bias [N] fp32, D [M x N] fp16 (row stride ldd).  K % 64 == 0.
+ * D may alias S1 (in-place shortcut), as the reference's out_buf does.
+ * q: per-QP table of fp16 rows (row stride qstride); qp: device int* row
+ * index, or NULL for row 0. */
+
+/* conv1x1_bias (cutlass/conv1x1_bias.cu:524-540): D = A W^T + b */
+int dcvc_conv1x1_bias(const void* A, int lda, int M, int K, const void* W, int N, const float* bias,
+                      void* D, int ldd, void* stream);
+/* conv1x1_bias_wsilu (conv1x1_bias_wsilu.cu:230-248): D = wsilu(A W^T + b) */
+int dcvc_conv1x1_bias_wsilu(const void* A, int lda, int M, int K, const void* W, int N,
+                            const float* bias, void* D, int ldd, void* stream);
+/* conv1x1_bias_shortcut (conv1x1_bias_shortcut.cu:230-247): D = A W^T + b + S */
+int dcvc_conv1x1_bias_shortcut(const void* A, int lda, int M, int K, const void* W, int N,
+                               const float* bias, const void* S, int lds, void* D, int ldd,
+                               void* stream);
+/* conv1x1_bias_shortcut2 (conv1x1_bias_shortcut2.cu:81-100): D = A W^T + b + S1 + S2 */
+int dcvc_conv1x1_bias_shortcut2(const void* A, int lda, int M, int K, const void* W, int N,
+                                const float* bias, const void* S1, int lds1, const void* S2,
+                                int lds2, void* D, int ldd, void* stream);
+/* conv1x1_bias_shortcut_with_quant (…_with_quant.cu:50-70): D = (A W^T + b + S) * q[n] */
+int dcvc_conv1x1_bias_shortcut_with_quant(const void* A, int lda, int M, int K, const void* W,
+                                          int N, const float* bias, const void* S, int lds,
+                                          const void* q, int qstride, const int* qp, void* D,
+                                          int ldd, void* stream);
+/* conv1x1_bias_with_quant (conv1x1_bias_with_quant.cu:30-48): D = (A W^T + b) * q[n] */
+int dcvc_conv1x1_bias_with_quant(const void* A, int lda, int M, int K, const void* W, int N,
+                                 const float* bias, const void* q, int qstride, const int* qp,
+                                 void* D, int ldd, void* stream);
+/* conv1x1_bias_wsilu_chunk_add, the quad GEMM
+ * (conv1x1_bias_wsilu_chunk_add.cu:356-390): W [4N x K] and bias [4N] in the
+ * reference ffn.0 order; D[m, n] = sum_{j<4} wsilu(A W[4n+j]^T + b[4n+j]). */
+int dcvc_conv1x1_bias_wsilu_chunk_add(const void* A, int lda, int M, int K, const void* W, int N,
+                                      const float* bias, void* D, int ldd, void* stream);
+
+/* conv_bias (cutlass/conv_bias.cu:131-151): k x k conv, stride s, pad (k-1)/2,
+ * over n_img NHWC images of H x W x Cin.  (k, s) in {(2,2), (3,2), (3,1)}.
+ * W is [N][k*k][Cin] (tap-major, i.e. OHWI).  Output [n_img, Ho, Wo, N]. */
+int dcvc_conv_bias(const void* X, int ldx, int n_img, int H, int W, int Cin, const void* Wt, int N,
+                   const float* bias, int kernel, int stride, void* D, int ldd, void* stream);
+
+/* transposed_conv (cutlass/transposed_conv.cu:101-119): SubpelConv2x with a
+ * 1x1 kernel, i.e. 1x1 conv to 4*Cout channels then pixel_shuffle(2).
+ * W is [4][Cout][Cin] (sub-pixel major: sub = dy*2+dx); bias [4*Cout] in the
+ * same order, or NULL for none.  Output [n_img, 2H, 2W, Cout]. */
+int dcvc_transposed_conv(const void* X, int ldx, int n_img, int H, int W, int Cin, const void* Wt,
+                         int Cout, const float* bias, void* D, int ldd, void* stream);
+
+/* d3x3 (cutlass/d3x3.cu:443-446): depthwise 3x3, pad 1, no bias.
+ * Wt is [9][C] (tap ky*3+kx major). */
+int dcvc_d3x3(const void* X, int ldx, int n_img, int H, int W, int C, const void* Wt, void* Y,
+              int ldy, void* stream);
+
+/* ---------------- layout / scaling ---------------- */
+/* pad_and_unshuffle_8_cuda / pixel_unshuffle_cuda (cat_and_pad.cu:7-51,
+ * shuffle.cu:134-185): replicate-pad by (padB, padR) then unshuffle by f. */
+int dcvc_pad_and_unshuffle(const void* X, int ldx, int n_img, int H, int W, int Cin, int f,
+                           int padB, int padR, void* Y, int ldy, void* stream);
+/* pixel_shuffle_8_cuda with clamp to [-0.5, 0.5] (shuffle.cu:224-316) */
+int dcvc_pixel_shuffle_8(const void* X, int ldx, int n_img, int H8, int W8, int C, void* Y,
+                         int ldy, void* stream);
+/* multiply_with_broadcast_cuda (stream.cu:484-546): Y = X * q[qp][c] */
+int dcvc_multiply_with_broadcast(const void* X, int ldx, long M, int C, const void* q, int qstride,
+                                 const int* qp, void* Y, int ldy, void* stream);
+/* round_z_cuda (stream.cu:862-894): clamp(round(z), -64, 63) -> fp16 + int8 (NHWC-linear) */
+int dcvc_round_z(const void* Z, int ldz, long M, int C, void* Zhat, int ldh, int8_t* Zi8,
+                 void* stream);
+/* int8_to_dtype_cuda (stream.cu:454-482) */
+int dcvc_int8_to_half(const int8_t* Zi8, long M, int C, void* Zhat, int ldh, void* stream);
+/* replicate_pad_cuda / slice_cuda (cat_and_pad.cu:53-110): Ho >= H pads, Ho <= H crops */
+int dcvc_replicate_pad(const void* X, int ldx, int n_img, int H, int W, int C, int Ho, int Wo,
+                       void* Y, int ldy, void* stream);
+
+/* ---------------- quantisation / index boundary ----------------
+ * th: device pointer to the 128-float threshold table; thr: skip threshold.
+ * sym: int16 (y_q << 8) + index, cond: uint8 (scale > thr), NHWC-linear. */
+
+/* HT-S encoder mask step k in 0..3 (process_with_mask_no_scale*_cuda,
+ * stream.cu:548-683, fused with divide_with_clamp_min :422-443 and
+ * build_index_enc :130-161).  Y = raw y; S, Mn, Qd = scales, means, q_dec;
+ * Yhat is accumulated in place (written at active positions; step 0 zeroes
+ * the rest; step 3 scales everything by max(q_dec, 0.5)). */
+int dcvc_hts_quant_step(const void* Y, int ldy, int n_img, int H, int W, int C, const void* S,
+                        int lds, const void* Mn, int ldm, const void* Qd, int ldq, void* Yhat,
+                        int ldh, int16_t* sym, uint8_t* cond, const float* th, float thr, int step,
+                        void* stream);
+/* build_index_dec_cuda (stream.cu:89-128): dense index + condition */
+int dcvc_hts_build_index_dec(const void* S, int lds, int n_img, int H, int W, int C,
+                             const float* th, float thr, uint8_t* idx, uint8_t* cond, void* stream);
+/* restore_y_cuda / restore_y_and_add(_multiply)_inplace_cuda (stream.cu:685-754) */
+int dcvc_hts_restore_step(const int8_t* yq, int n_img, int H, int W, int C, const void* Mn,
+                          int ldm, const void* Qd, int ldq, void* Yhat, int ldh, int step,
+                          void* stream);
+/* DMCI encoder mask step (process_with_mask_cuda + single_part_for_writing_4x
+ * + build_index_enc, dmci_proxy.cpp:333-393): quarter outputs sym/cond of
+ * n_img*H*W*C/4 elements; Ysf = y_hat_so_far (in place); step 3 writes
+ * Yout = (Ysf + y_hat_3) * q[qp][c]. */
+int dcvc_intra_quant_step(const void* Y, int ldy, int n_img, int H, int W, int C, const void* S,
+                          int lds, const void* Mn, int ldm, void* Ysf, int ldsf, const void* q,
+                          int qstride, const int* qp, void* Yout, int ldo, int16_t* sym,
+                          uint8_t* cond, const float* th, float thr, int step, void* stream);
+/* single_part_for_reading_4x + build_index_dec (dmci_proxy.cpp:667-676) */
+int dcvc_intra_build_index_dec(const void* S, int lds, int n_img, int H, int W, int C,
+                               const float* th, float thr, int step, uint8_t* idx, uint8_t* cond,
+                               void* stream);
+/* restore_y_4x* (stream.cu:756-860) */
+int dcvc_intra_restore_step(const int8_t* yq, int n_img, int H, int W, int C, const void* Mn,
+                            int ldm, void* Ysf, int ldsf, const void* q, int qstride,
+                            const int* qp, void* Yout, int ldo, int step, void* stream);
+
+/* conditional_index_part1_cuda (stream.cu:175-340): stable compaction of N
+ * elements of esize (1 or 2) bytes where cond != 0; *count (device int)
+ * receives the kept count.  block_offsets: dcvc_compact_blocks(N) ints. */
+int dcvc_compact_blocks(long N);
+int dcvc_conditional_index(const void* vals, int esize, const uint8_t* cond, long N, void* out,
+                           int* block_offsets, int* count, void* stream);
+/* conditional_recover_with_type_conversion_cuda (stream.cu:359-420): dense
+ * int8 from compacted decoded symbols, 0 where cond == 0; block_offsets as
+ * left by dcvc_conditional_index over the same cond. */
+int dcvc_conditional_recover(const int8_t* dec, const uint8_t* cond, long N,
+                             const int* block_offsets, int8_t* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DCVC_OPS_H */

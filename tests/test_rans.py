@@ -1,0 +1,205 @@
+"""CPU entropy coder: byte-exact against the reference's MLCodec_extensions_cpp
+(built from /root/reference sources into oracle/_ref by oracle/build_ref.sh),
+round trips, escape symbols, stream merging for every parallelism, and
+failure handling on truncated / corrupted streams."""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from dcvc_amd import entropy
+
+REF_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref")
+
+
+def ref_module():
+    if REF_DIR not in sys.path:
+        sys.path.insert(0, REF_DIR)
+    try:
+        import MLCodec_extensions_cpp as m  # reference coder, test oracle only
+    except ImportError:
+        pytest.skip("oracle/_ref reference coder not built (oracle/build_ref.sh)")
+    return m
+
+
+def rng(s):
+    return np.random.Generator(np.random.PCG64(s))
+
+
+@pytest.fixture(scope="module")
+def tables():
+    g_cdf, g_len = entropy.gaussian_cdf_table()
+    r = rng(5)
+    h = torch.from_numpy(r.standard_normal((64, 128, 4)).astype(np.float32) * 0.5)
+    b = torch.from_numpy(r.standard_normal((64, 128, 4)).astype(np.float32) * 0.5)
+    a = torch.from_numpy(r.standard_normal((64, 128, 3)).astype(np.float32) * 0.5)
+    z_cdf, z_len = entropy.bit_estimator_cdf_table(h, b, a)
+    return g_cdf, g_len, z_cdf, z_len
+
+
+def random_symbols(r, n, heavy=False):
+    idx = r.integers(0, 128, size=n)
+    scale = np.exp(np.log(0.11) + idx / 127.0 * (np.log(16) - np.log(0.11)))
+    s = np.clip(np.round(r.standard_normal(n) * scale * (3 if heavy else 1)), -128, 127).astype(np.int64)
+    if heavy:
+        s[r.integers(0, n, size=max(1, n // 50))] = r.integers(-128, 128, size=max(1, n // 50))
+    return ((s << 8) + idx).astype(np.int16)
+
+
+def test_pmf_to_quantized_cdf_matches_reference():
+    m = ref_module()
+    r = rng(0)
+    for trial in range(300):
+        n = int(r.integers(2, 40))
+        p = r.random(n).astype(np.float32)
+        if trial % 3 == 0:
+            p[r.integers(0, n, size=n // 2)] = 1e-9  # forces frequency stealing
+        p = p / p.sum()
+        assert entropy.pmf_to_quantized_cdf(p) == list(m.pmf_to_quantized_cdf(p.tolist()))
+
+
+def test_gaussian_pmf_closed_form(tables):
+    """The Gaussian table's pmf is N(0, s) mass of [k-0.5, k+0.5]; checked
+    against the erfc closed form in float64 through the first nonzero row."""
+    g_cdf, g_len, _, _ = tables
+    scales = entropy.gaussian_scale_table().double().numpy()
+    for i in (0, 40, 90, 127):
+        L = int(g_len[i]) - 2
+        half = (L - 1) // 2
+        k = np.arange(-half, half + 1)
+        pmf = 0.5 * (np.vectorize(math.erfc)(-(k + 0.5) / (scales[i] * math.sqrt(2)))
+                     - np.vectorize(math.erfc)(-(k - 0.5) / (scales[i] * math.sqrt(2))))
+        cdf = g_cdf[i, : L + 2].astype(np.int64)
+        freq = np.diff(cdf) / 65536.0
+        order = [half] + [v for j in range(1, half + 1) for v in (half + j, half - j)]
+        # quantisation: rounding (1/65536 each) plus min-frequency steals ((L+1)/65536 at most)
+        assert np.allclose(freq[:L], pmf[order], atol=(L + 3) / 65536.0)
+        assert cdf[-1] == 65536 and np.all(np.diff(cdf) > 0)
+
+
+@pytest.mark.parametrize("parallel", [1, 2, 3, 4, 5, 7, 8])
+def test_stream_bytes_match_reference(tables, parallel):
+    m = ref_module()
+    g_cdf, g_len, z_cdf, z_len = tables
+    r = rng(parallel)
+    y = random_symbols(r, 50000 + parallel * 777, heavy=True)
+    z = np.clip(np.round(r.standard_normal(128 * 17 * 30) * 3), -64, 63).astype(np.int8)
+    qp = 37
+    ours = entropy.RansEncoder()
+    theirs = m.RansEncoder()
+    for enc in (ours, theirs):
+        enc.set_cdf(z_cdf, z_len, 0)
+        enc.set_cdf(g_cdf, g_len, 1)
+        enc.reset()
+        enc.set_entropy_coder_parallel(parallel)
+        enc.encode_y(y[: len(y) // 2])
+        enc.encode_y(y[len(y) // 2:])
+        enc.encode_z(z, qp * 128, 128)
+        enc.flush()
+    a = ours.get_encoded_stream()
+    b = np.asarray(theirs.get_encoded_stream())
+    assert a.tobytes() == b.tobytes()
+    # decode with ours
+    dec = entropy.RansDecoder()
+    dec.set_cdf(z_cdf, z_len, 0)
+    dec.set_cdf(g_cdf, g_len, 1)
+    dec.set_entropy_coder_parallel(parallel)
+    dec.set_stream(a)
+    assert np.array_equal(dec.decode_z(z.size, qp * 128, 128), z)
+    idx = (y.astype(np.int64) & 0xFF).astype(np.uint8)
+    sym = (y.astype(np.int64) >> 8).astype(np.int8)
+    h = len(y) // 2
+    # rANS is LIFO: batches come back in the reverse of their encode order
+    assert np.array_equal(dec.decode_y(idx[h:]), sym[h:])
+    assert np.array_equal(dec.decode_y(idx[:h]), sym[:h])
+
+
+def test_round_trip_without_reference(tables):
+    g_cdf, g_len, z_cdf, z_len = tables
+    r = rng(99)
+    for parallel in (1, 2, 6, 8):
+        y = random_symbols(r, 20000, heavy=True)
+        enc = entropy.RansEncoder()
+        enc.set_cdf(g_cdf, g_len, 1)
+        enc.set_cdf(z_cdf, z_len, 0)
+        enc.reset()
+        enc.set_entropy_coder_parallel(parallel)
+        enc.encode_y(y)
+        enc.flush()
+        s = enc.get_encoded_stream()
+        dec = entropy.RansDecoder()
+        dec.set_cdf(g_cdf, g_len, 1)
+        dec.set_cdf(z_cdf, z_len, 0)
+        dec.set_entropy_coder_parallel(parallel)
+        dec.set_stream(s)
+        out = dec.decode_y((y.astype(np.int64) & 0xFF).astype(np.uint8))
+        assert np.array_equal(out, (y.astype(np.int64) >> 8).astype(np.int8))
+
+
+def test_empty_and_tiny_inputs(tables):
+    g_cdf, g_len, _, _ = tables
+    for n, parallel in [(0, 1), (0, 3), (1, 1), (2, 8), (7, 8)]:
+        y = random_symbols(rng(n), n)
+        enc = entropy.RansEncoder()
+        enc.set_cdf(g_cdf, g_len, 1)
+        enc.reset()
+        enc.set_entropy_coder_parallel(parallel)
+        enc.encode_y(y)
+        enc.flush()
+        s = enc.get_encoded_stream()
+        dec = entropy.RansDecoder()
+        dec.set_cdf(g_cdf, g_len, 1)
+        dec.set_entropy_coder_parallel(parallel)
+        dec.set_stream(s)
+        out = dec.decode_y((y.astype(np.int64) & 0xFF).astype(np.uint8))
+        assert np.array_equal(out, (y.astype(np.int64) >> 8).astype(np.int8))
+
+
+def test_truncated_and_corrupted_streams_raise(tables):
+    g_cdf, g_len, _, _ = tables
+    y = random_symbols(rng(3), 40000)
+    idx = (y.astype(np.int64) & 0xFF).astype(np.uint8)
+    for parallel in (1, 4):
+        enc = entropy.RansEncoder()
+        enc.set_cdf(g_cdf, g_len, 1)
+        enc.reset()
+        enc.set_entropy_coder_parallel(parallel)
+        enc.encode_y(y)
+        enc.flush()
+        s = enc.get_encoded_stream()
+        dec = entropy.RansDecoder()
+        dec.set_cdf(g_cdf, g_len, 1)
+        dec.set_entropy_coder_parallel(parallel)
+        with pytest.raises(Exception):
+            dec.set_stream(s[: len(s) // 3])
+            dec.decode_y(idx)
+        with pytest.raises(Exception):
+            dec.set_stream(s[:2])
+        bad = s.copy()
+        bad[: 4 * (parallel // 2)] = 0xFF  # corrupt header offsets
+        if parallel > 2:
+            with pytest.raises(Exception):
+                dec.set_stream(bad)
+
+
+def test_bad_arguments_raise(tables):
+    g_cdf, g_len, _, _ = tables
+    enc = entropy.RansEncoder()
+    with pytest.raises(Exception):
+        enc.set_entropy_coder_parallel(9)
+    with pytest.raises(Exception):
+        enc.encode_y(np.zeros(4, np.int16))  # cdf table not set
+    enc.set_cdf(g_cdf, g_len, 1)
+    enc.reset()
+    enc.encode_y(np.array([(1 << 8) + 200], np.int16))  # cdf index 200 >= 128
+    with pytest.raises(Exception):
+        enc.flush()
+
+
+def test_compute_ec_parallel():
+    assert entropy.compute_ec_parallel(0) == 1
+    assert entropy.compute_ec_parallel(3 * 32768 + 5) == 3
+    assert entropy.compute_ec_parallel(2088960) == 8
