@@ -65,10 +65,35 @@ SIGNATURES = {
     "dcvc_rans_decoder_set_stream": [P, P, L],
     "dcvc_rans_decoder_decode_z": [P, L, I, I, P],
     "dcvc_rans_decoder_decode_y": [P, P, L, P],
+    # codec proxies
+    "dcvc_dmci_create": [],
+    "dcvc_dmci_destroy": [P],
+    "dcvc_dmci_set_param": [P, I, P, P, P, P, F],
+    "dcvc_dmci_prepare": [P, I, I, P, I],
+    "dcvc_dmci_compress": [P, P, I, I, I, I, I, I, P, P, P, P],
+    "dcvc_dmci_decompress": [P, P, L, I, I, I, I, P],
+    "dcvc_dmci_set_use_graphs": [P, I],
+    "dcvc_dmci_step_symbol_count": [P, I],
+    "dcvc_dmci_debug_copy": [P, I, P, L],
+    "dcvc_hts_create": [],
+    "dcvc_hts_destroy": [P],
+    "dcvc_hts_set_param": [P, I, P, P, P, P, F],
+    "dcvc_hts_share_weights": [P, P],
+    "dcvc_hts_prepare": [P, I, I, P, I],
+    "dcvc_hts_add_ref_feature_from_frame": [P, P, I, I, I, I, P],
+    "dcvc_hts_compress": [P, P, I, I, I, I, I, I, I, P, P, P, P],
+    "dcvc_hts_decompress": [P, P, L, I, I, I, I, I, P],
+    "dcvc_hts_set_use_graphs": [P, I],
+    "dcvc_hts_debug_copy": [P, I, P, L],
+    "dcvc_hts_last_symbol_count": [P],
+    "dcvc_trace_begin": [],
+    "dcvc_trace_end": [P, I],
 }
 _RESTYPE = {"dcvc_last_error": C.c_char_p, "dcvc_rans_encoder_create": C.c_void_p,
             "dcvc_rans_decoder_create": C.c_void_p, "dcvc_rans_encoder_destroy": None,
-            "dcvc_rans_decoder_destroy": None}
+            "dcvc_rans_decoder_destroy": None, "dcvc_dmci_create": C.c_void_p, "dcvc_hts_create": C.c_void_p,
+            "dcvc_dmci_destroy": None, "dcvc_hts_destroy": None, "dcvc_hts_last_symbol_count": C.c_long,
+            "dcvc_dmci_step_symbol_count": C.c_long}
 
 
 def declared_symbols():

@@ -1,0 +1,178 @@
+// extern "C" entry points of the codec proxies (include/dcvc_codec.h, part 2).
+#include "../../include/dcvc_codec.h"
+#include "capi_util.h"
+#include "proxy_dmci.h"
+#include "proxy_hts.h"
+
+using namespace dcvc;
+
+static HostDict make_dict(int n, const char** names, const float** data, const int64_t* shapes, const int* ndims) {
+    if (n <= 0 || !names || !data || !shapes || !ndims) throw std::invalid_argument("empty state dict");
+    HostDict d;
+    for (int i = 0; i < n; ++i) {
+        if (!names[i] || !data[i] || ndims[i] < 0 || ndims[i] > 4) throw std::invalid_argument("bad state dict entry");
+        HostTensor t;
+        t.shape.assign(shapes + 4 * i, shapes + 4 * i + ndims[i]);
+        const int64_t numel = t.numel();
+        if (numel < 0) throw std::invalid_argument("bad shape");
+        t.data.assign(data[i], data[i] + numel);
+        d.emplace(names[i], std::move(t));
+    }
+    return d;
+}
+
+static hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+template <class T>
+static void* create() {
+    try {
+        return new T();
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return nullptr;
+    }
+}
+
+extern "C" {
+
+void* dcvc_dmci_create(void) { return create<DMCIProxy>(); }
+void dcvc_dmci_destroy(void* h) { delete static_cast<DMCIProxy*>(h); }
+
+int dcvc_dmci_set_param(void* h, int n, const char** names, const float** data, const int64_t* shapes,
+                        const int* ndims, float skip_thres) {
+    return guarded([&] { checked<DMCIProxy>(h)->set_param(make_dict(n, names, data, shapes, ndims), skip_thres); });
+}
+
+int dcvc_dmci_prepare(void* h, int Hp, int Wp, void* x_hat, int ld_out) {
+    return guarded([&] {
+        if (!x_hat || ld_out < 3) throw std::invalid_argument("bad output buffer");
+        checked<DMCIProxy>(h)->prepare(Hp, Wp, static_cast<f16*>(x_hat), ld_out);
+    });
+}
+
+int dcvc_dmci_compress(void* h, const void* x, int ldx, int H, int W, int qp, int pad_b, int pad_r, void* stream,
+                       const uint8_t** bits, long* nbytes, int* ec_parallel) {
+    return guarded([&] {
+        if (!x || !bits || !nbytes || !ec_parallel) throw std::invalid_argument("null argument");
+        *ec_parallel = checked<DMCIProxy>(h)->compress(static_cast<const f16*>(x), ldx, H, W, qp, pad_b, pad_r,
+                                                       S(stream), bits, nbytes);
+    });
+}
+
+int dcvc_dmci_decompress(void* h, const uint8_t* bits, long nbytes, int qp, int height, int width, int ec_parallel,
+                         void* stream) {
+    return guarded([&] {
+        if (!bits && nbytes > 0) throw std::invalid_argument("null bitstream");
+        checked<DMCIProxy>(h)->decompress(bits, nbytes, qp, height, width, ec_parallel, S(stream));
+    });
+}
+
+int dcvc_dmci_set_use_graphs(void* h, int on) {
+    return guarded([&] { checked<DMCIProxy>(h)->set_use_graphs(on != 0); });
+}
+
+int dcvc_dmci_debug_copy(void* h, int what, void* host, long bytes) {
+    return guarded([&] {
+        if (!host || bytes < 0) throw std::invalid_argument("bad host buffer");
+        checked<DMCIProxy>(h)->copy_debug(what, host, bytes);
+    });
+}
+
+long dcvc_dmci_step_symbol_count(void* h, int step) {
+    if (!h || step < 0 || step > 3) return -1;
+    return static_cast<DMCIProxy*>(h)->step_count(step);
+}
+
+void* dcvc_hts_create(void) { return create<HTSProxy>(); }
+void dcvc_hts_destroy(void* h) { delete static_cast<HTSProxy*>(h); }
+
+int dcvc_hts_set_param(void* h, int n, const char** names, const float** data, const int64_t* shapes,
+                       const int* ndims, float skip_thres) {
+    return guarded([&] { checked<HTSProxy>(h)->set_param(make_dict(n, names, data, shapes, ndims), skip_thres); });
+}
+
+int dcvc_hts_share_weights(void* h, void* src) {
+    return guarded([&] {
+        auto w = checked<HTSProxy>(src)->weights();
+        if (!w) throw std::logic_error("source proxy has no parameters");
+        checked<HTSProxy>(h)->share_weights(w);
+    });
+}
+
+int dcvc_hts_prepare(void* h, int Hp, int Wp, void* const* frames, int ld_out) {
+    return guarded([&] {
+        if (!frames || ld_out < 3) throw std::invalid_argument("bad output buffers");
+        f16* f[HTSProxy::kFrames];
+        for (int i = 0; i < HTSProxy::kFrames; ++i) {
+            if (!frames[i]) throw std::invalid_argument("null output frame");
+            f[i] = static_cast<f16*>(frames[i]);
+        }
+        checked<HTSProxy>(h)->prepare(Hp, Wp, f, ld_out);
+    });
+}
+
+int dcvc_hts_add_ref_feature_from_frame(void* h, const void* frame, int ld, int Hp, int Wp, int apply_adaptor,
+                                        void* stream) {
+    return guarded([&] {
+        if (!frame || ld < 3) throw std::invalid_argument("bad frame");
+        checked<HTSProxy>(h)->add_ref_feature_from_frame(static_cast<const f16*>(frame), ld, Hp, Wp, apply_adaptor != 0,
+                                                         S(stream));
+    });
+}
+
+int dcvc_hts_compress(void* h, const void* x, int ldx, int H, int W, int qp, int reset_feature_memory, int pad_b,
+                      int pad_r, void* stream, const uint8_t** bits, long* nbytes, int* ec_parallel) {
+    return guarded([&] {
+        if (!x || !bits || !nbytes || !ec_parallel) throw std::invalid_argument("null argument");
+        *ec_parallel = checked<HTSProxy>(h)->compress(static_cast<const f16*>(x), ldx, H, W, qp,
+                                                      reset_feature_memory != 0, pad_b, pad_r, S(stream), bits, nbytes);
+    });
+}
+
+int dcvc_hts_decompress(void* h, const uint8_t* bits, long nbytes, int qp, int height, int width, int ec_parallel,
+                        int reset_feature_memory, void* stream) {
+    return guarded([&] {
+        if (!bits && nbytes > 0) throw std::invalid_argument("null bitstream");
+        checked<HTSProxy>(h)->decompress(bits, nbytes, qp, height, width, ec_parallel, reset_feature_memory != 0,
+                                         S(stream));
+    });
+}
+
+int dcvc_hts_set_use_graphs(void* h, int on) {
+    return guarded([&] { checked<HTSProxy>(h)->set_use_graphs(on != 0); });
+}
+
+int dcvc_hts_debug_copy(void* h, int what, void* host, long bytes) {
+    return guarded([&] {
+        if (!host || bytes < 0) throw std::invalid_argument("bad host buffer");
+        checked<HTSProxy>(h)->copy_debug(what, host, bytes);
+    });
+}
+
+long dcvc_hts_last_symbol_count(void* h) { return h ? static_cast<HTSProxy*>(h)->last_symbol_count() : -1; }
+
+int dcvc_trace_begin(void) {
+    trace_set(true);
+    return DCVC_OK;
+}
+
+int dcvc_trace_end(long long* records, int max_records) {
+    trace_set(false);
+    auto& t = trace_buffer();
+    const int n = (int)t.size();
+    if (records)
+        for (int i = 0; i < n && i < max_records; ++i) {
+            const TraceRecord& r = t[i];
+            long long* o = records + 7LL * i;
+            o[0] = r.kind;
+            o[1] = r.epi;
+            o[2] = r.M;
+            o[3] = r.N;
+            o[4] = r.K;
+            o[5] = r.flops;
+            o[6] = r.bytes;
+        }
+    return n;
+}
+
+}  // extern "C"

@@ -1,0 +1,370 @@
+// Layer building blocks of the codec proxies (see runtime.h).
+#include "runtime.h"
+
+#include <cstring>
+
+namespace dcvc {
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw StatusError(DCVC_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+const HostTensor& need(const HostDict& d, const std::string& key) {
+    auto it = d.find(key);
+    if (it == d.end()) throw std::invalid_argument("state dict is missing '" + key + "'");
+    return it->second;
+}
+
+bool has(const HostDict& d, const std::string& key) { return d.find(key) != d.end(); }
+
+DevMem::~DevMem() { free_all(); }
+
+void* DevMem::alloc(size_t bytes) {
+    void* p = nullptr;
+    hip_check(hipMalloc(&p, bytes ? bytes : 16), "hipMalloc");
+    m_blocks.push_back(p);
+    m_bytes += bytes;
+    return p;
+}
+
+void DevMem::free_all() {
+    for (void* p : m_blocks) hipFree(p);
+    m_blocks.clear();
+    m_bytes = 0;
+}
+
+f16* upload_f16(DevMem& mem, const std::vector<float>& v) {
+    std::vector<f16> h(v.size());
+    for (size_t i = 0; i < v.size(); ++i) h[i] = (f16)v[i];
+    f16* d = mem.alloc_n<f16>(v.size());
+    hip_check(hipMemcpy(d, h.data(), h.size() * sizeof(f16), hipMemcpyHostToDevice), "upload");
+    return d;
+}
+
+float* upload_f32(DevMem& mem, const std::vector<float>& v) {
+    float* d = mem.alloc_n<float>(v.size());
+    hip_check(hipMemcpy(d, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice), "upload");
+    return d;
+}
+
+// weights are rounded to fp16 once, as the reference's model.half() does
+static inline float to_half_value(float x) { return (float)(f16)x; }
+
+static void trace_gemm(int kind, int epi, long M, int N_out, int K, bool quad, int n_shortcut);
+
+// ---------------------------------------------------------------- Conv1x1
+void Conv1x1::load_raw(DevMem& mem, const std::vector<float>& wt, const std::vector<float>& bias, int N_, int K_) {
+    N = N_;
+    K = K_;
+    w = upload_f16(mem, wt);
+    b = upload_f32(mem, bias);
+}
+
+void Conv1x1::load(DevMem& mem, const HostDict& d, const std::string& prefix, bool quad_) {
+    const HostTensor& wt = need(d, prefix + "weight");
+    if (wt.shape.size() != 4 || wt.shape[2] != 1 || wt.shape[3] != 1) throw std::invalid_argument(prefix + ": not 1x1");
+    quad = quad_;
+    const int rows = (int)wt.shape[0];
+    K = (int)wt.shape[1];
+    N = quad ? rows / 4 : rows;
+    std::vector<float> bias(rows, 0.f);
+    if (has(d, prefix + "bias")) {
+        const HostTensor& bt = need(d, prefix + "bias");
+        for (int i = 0; i < rows; ++i) bias[i] = to_half_value(bt.data[i]);
+    }
+    w = upload_f16(mem, wt.data);
+    b = upload_f32(mem, bias);
+}
+
+void Conv1x1::run(const Exec& e, const Act& a, const Act& dst, int epi, const Act* s1, const Act* s2,
+                  const QuantSel* q) const {
+    GemmArgs g;
+    g.A = a.p;
+    g.lda = a.ld;
+    g.M = (int)a.pixels();
+    g.K = K;
+    g.B = w;
+    g.N = N;
+    g.bias = b;
+    g.D = dst.p;
+    g.ldd = dst.ld;
+    g.outH = a.H;
+    g.outW = a.W;
+    if (s1) {
+        g.S1 = s1->p;
+        g.lds1 = s1->ld;
+    }
+    if (s2) {
+        g.S2 = s2->p;
+        g.lds2 = s2->ld;
+    }
+    if (q) {
+        g.q = q->table;
+        g.qstride = q->stride;
+        g.qp = e.d_qp;
+    }
+    if (a.C != K) throw std::invalid_argument("conv1x1: input channels do not match the weight");
+    const int v = gemm_validate(g, epi, A_PLAIN, D_PLAIN);
+    if (v != DCVC_OK) throw StatusError(v, "conv1x1: invalid GEMM arguments");
+    trace_gemm(0, epi, g.M, N, K, epi == EPI_QUAD,
+               (epi == EPI_SHORTCUT || epi == EPI_SHORTCUT_QUANT) ? 1 : (epi == EPI_SHORTCUT2 ? 2 : 0));
+    hip_check(gemm_launch(g, epi, A_PLAIN, D_PLAIN, e.stream), "gemm launch");
+}
+
+// ---------------------------------------------------------------- DW3x3
+void DW3x3::load(DevMem& mem, const HostDict& d, const std::string& key) {
+    const HostTensor& wt = need(d, key);  // [C, 1, 3, 3]
+    C = (int)wt.shape[0];
+    std::vector<float> t(9 * (size_t)C);
+    for (int c = 0; c < C; ++c)
+        for (int k = 0; k < 9; ++k) t[(size_t)k * C + c] = wt.data[(size_t)c * 9 + k];
+    w = upload_f16(mem, t);
+}
+
+void DW3x3::run(const Exec& e, const Act& in, const Act& out) const {
+    DwArgs a;
+    a.x = in.p;
+    a.ldx = in.ld;
+    a.wt = w;
+    a.y = out.p;
+    a.ldy = out.ld;
+    a.n_img = in.n_img;
+    a.H = in.H;
+    a.W = in.W;
+    a.C = C;
+    const int v = dw3x3_validate(a);
+    if (v != DCVC_OK) throw StatusError(v, "d3x3: invalid arguments");
+    if (trace_enabled()) {
+        const long long M = in.pixels();
+        trace_record(TraceRecord{3, 0, (int)M, C, 9, 18LL * M * C, 2LL * (2 * M * C + 9LL * C)});
+    }
+    hip_check(dw3x3_launch(a, e.stream), "d3x3 launch");
+}
+
+// ---------------------------------------------------------------- DCB
+void DCB::load(DevMem& mem, const HostDict& d, const std::string& p, bool shortcut_) {
+    shortcut = shortcut_;
+    has_adaptor = has(d, p + "adaptor.weight");
+    if (has_adaptor) ad.load(mem, d, p + "adaptor.");
+    dc0.load(mem, d, p + "dc.0.");
+    dw.load(mem, d, p + "dc.2.weight");
+    // dc.3 bias' = dc.3.bias + W3 . dc.2.bias (layers_proxy.cpp:175-178), in double
+    {
+        const HostTensor& w3 = need(d, p + "dc.3.weight");
+        const int n = (int)w3.shape[0], k = (int)w3.shape[1];
+        std::vector<float> b(n, 0.f);
+        const HostTensor* b3 = has(d, p + "dc.3.bias") ? &need(d, p + "dc.3.bias") : nullptr;
+        const HostTensor* bd = has(d, p + "dc.2.bias") ? &need(d, p + "dc.2.bias") : nullptr;
+        for (int o = 0; o < n; ++o) {
+            double s = b3 ? (double)to_half_value(b3->data[o]) : 0.0;
+            if (bd)
+                for (int c = 0; c < k; ++c)
+                    s += (double)to_half_value(w3.data[(size_t)o * k + c]) * (double)to_half_value(bd->data[c]);
+            b[o] = (float)s;
+        }
+        dc3.load_raw(mem, w3.data, b, n, k);
+    }
+    ffn0.load(mem, d, p + "ffn.0.", true);
+    ffn2.load(mem, d, p + "ffn.2.");
+    cout = dc0.K;
+    cmid = dc0.N;
+    cin = has_adaptor ? ad.K : cout;
+}
+
+void DCB::forward(const Exec& e, const Act& in, const Act& out, const QuantSel* q) const {
+    if (in.C != cin || out.C != cout) throw std::invalid_argument("DCB: channel mismatch");
+    Scratch& sc = *e.sc;
+    const size_t need_mid = (size_t)in.pixels() * cmid;
+    if (need_mid > sc.halves || (size_t)in.pixels() * cout > sc.halves) throw std::logic_error("DCB: scratch too small");
+    Act x = in;
+    if (has_adaptor) {
+        ad.run(e, in, out, EPI_BIAS);
+        x = out;
+    }
+    const Act t1 = make_act(sc.t1, in.n_img, in.H, in.W, cmid, cmid);
+    const Act t2 = make_act(sc.t2, in.n_img, in.H, in.W, cmid, cmid);
+    dc0.run(e, x, t1, EPI_WSILU);
+    dw.run(e, t1, t2);
+    const Act o = shortcut ? make_act(sc.so, in.n_img, in.H, in.W, cout, cout) : out;
+    dc3.run(e, t2, o, EPI_SHORTCUT, &x);
+    ffn0.run(e, o, t1, EPI_QUAD);
+    if (shortcut) {
+        ffn2.run(e, t1, out, EPI_SHORTCUT2, &o, &x);
+        if (q) throw std::logic_error("DCB: quant tail with block shortcut is not used by the models");
+    } else if (q) {
+        ffn2.run(e, t1, out, EPI_SHORTCUT_QUANT, &o, nullptr, q);
+    } else {
+        ffn2.run(e, t1, out, EPI_SHORTCUT, &o);
+    }
+}
+
+void DCBSeq::load(DevMem& mem, const HostDict& d, const std::string& prefix, int n, int first) {
+    blocks.resize(n - first);
+    for (int i = first; i < n; ++i) blocks[i - first].load(mem, d, prefix + std::to_string(i) + ".");
+}
+
+void DCBSeq::forward(const Exec& e, const Act& in, const Act& out, const Act& tmp, const QuantSel* q) const {
+    const int n = (int)blocks.size();
+    if (n == 1) {
+        blocks[0].forward(e, in, out, q);
+        return;
+    }
+    Act t = channels(tmp, 0, blocks[0].cout);
+    blocks[0].forward(e, in, t);
+    for (int i = 1; i + 1 < n; ++i) blocks[i].forward(e, t, t);
+    blocks[n - 1].forward(e, t, out, q);
+}
+
+// ---------------------------------------------------------------- ConvS2
+void ConvS2::load(DevMem& mem, const HostDict& d, const std::string& prefix, int k_) {
+    k = k_;
+    const HostTensor& wt = need(d, prefix + "weight");
+    N = (int)wt.shape[0];
+    std::vector<float> t;
+    if (k == 2) {
+        // down.weight [N, 4*Cin, 1, 1], channel c*4 + dy*2 + dx of pixel_unshuffle(x, 2)
+        Cin = (int)wt.shape[1] / 4;
+        t.resize((size_t)N * 4 * Cin);
+        for (int o = 0; o < N; ++o)
+            for (int c = 0; c < Cin; ++c)
+                for (int tap = 0; tap < 4; ++tap)
+                    t[((size_t)o * 4 + tap) * Cin + c] = wt.data[((size_t)o * Cin + c) * 4 + tap];
+    } else {
+        // Conv2d weight [N, Cin, 3, 3] -> OHWI
+        Cin = (int)wt.shape[1];
+        t.resize((size_t)N * 9 * Cin);
+        for (int o = 0; o < N; ++o)
+            for (int c = 0; c < Cin; ++c)
+                for (int tap = 0; tap < 9; ++tap)
+                    t[((size_t)o * 9 + tap) * Cin + c] = wt.data[((size_t)o * Cin + c) * 9 + tap];
+    }
+    w = upload_f16(mem, t);
+    std::vector<float> bias(N, 0.f);
+    if (has(d, prefix + "bias"))
+        for (int i = 0; i < N; ++i) bias[i] = to_half_value(need(d, prefix + "bias").data[i]);
+    b = upload_f32(mem, bias);
+}
+
+void ConvS2::run(const Exec& e, const Act& in, const Act& out) const {
+    GemmArgs g;
+    g.A = in.p;
+    g.lda = in.ld;
+    g.inH = in.H;
+    g.inW = in.W;
+    g.outH = out.H;
+    g.outW = out.W;
+    g.cin = Cin;
+    g.M = (int)out.pixels();
+    g.K = k * k * Cin;
+    g.B = w;
+    g.N = N;
+    g.bias = b;
+    g.D = out.p;
+    g.ldd = out.ld;
+    const int amode = (k == 2) ? A_S2K2 : A_S2K3;
+    const int v = gemm_validate(g, EPI_BIAS, amode, D_PLAIN);
+    if (v != DCVC_OK || in.C != Cin || out.C != N) throw StatusError(DCVC_ERR_ARG, "strided conv: invalid arguments");
+    if (trace_enabled()) {
+        // input read once + weights + output
+        const long long M = out.pixels();
+        trace_record(TraceRecord{1, EPI_BIAS, (int)M, N, g.K, 2LL * M * N * g.K,
+                                 2LL * (in.pixels() * (long long)Cin + (long long)N * g.K + M * N)});
+    }
+    hip_check(gemm_launch(g, EPI_BIAS, amode, D_PLAIN, e.stream), "strided conv launch");
+}
+
+// ---------------------------------------------------------------- Subpel
+void Subpel::load(DevMem& mem, const HostDict& d, const std::string& prefix) {
+    const HostTensor& wt = need(d, prefix + "conv.0.weight");  // [4*Cout, Cin, 1, 1]
+    Cout = (int)wt.shape[0] / 4;
+    Cin = (int)wt.shape[1];
+    std::vector<float> t((size_t)4 * Cout * Cin);
+    std::vector<float> bias((size_t)4 * Cout, 0.f);
+    const HostTensor* bt = has(d, prefix + "conv.0.bias") ? &need(d, prefix + "conv.0.bias") : nullptr;
+    for (int c = 0; c < Cout; ++c)
+        for (int sub = 0; sub < 4; ++sub) {
+            std::memcpy(&t[((size_t)sub * Cout + c) * Cin], &wt.data[((size_t)c * 4 + sub) * Cin], Cin * sizeof(float));
+            if (bt) bias[(size_t)sub * Cout + c] = to_half_value(bt->data[c * 4 + sub]);
+        }
+    w = upload_f16(mem, t);
+    b = upload_f32(mem, bias);
+}
+
+void Subpel::run(const Exec& e, const Act& in, const Act& out) const {
+    GemmArgs g;
+    g.A = in.p;
+    g.lda = in.ld;
+    g.outH = in.H;
+    g.outW = in.W;
+    g.M = (int)in.pixels();
+    g.K = Cin;
+    g.B = w;
+    g.N = 4 * Cout;
+    g.bias = b;
+    g.D = out.p;
+    g.ldd = out.ld;
+    const int v = gemm_validate(g, EPI_BIAS, A_PLAIN, D_SUBPEL);
+    if (v != DCVC_OK || in.C != Cin || out.C != Cout || out.H != 2 * in.H || out.W != 2 * in.W)
+        throw StatusError(DCVC_ERR_ARG, "subpel conv: invalid arguments");
+    trace_gemm(2, EPI_BIAS, g.M, 4 * Cout, Cin, false, 0);
+    hip_check(gemm_launch(g, EPI_BIAS, A_PLAIN, D_SUBPEL, e.stream), "subpel launch");
+}
+
+void RBS2::load(DevMem& mem, const HostDict& d, const std::string& p, bool shortcut) {
+    down.load(mem, d, p + "down.", 2);
+    conv.load(mem, d, p + "conv.", shortcut);
+}
+
+void RBS2::forward(const Exec& e, const Act& in, const Act& out) const {
+    down.run(e, in, out);
+    conv.forward(e, out, out);
+}
+
+void RBU::load(DevMem& mem, const HostDict& d, const std::string& p, bool shortcut) {
+    up.load(mem, d, p + "up.");
+    conv.load(mem, d, p + "conv.", shortcut);
+}
+
+void RBU::forward(const Exec& e, const Act& in, const Act& out) const {
+    up.run(e, in, out);
+    conv.forward(e, out, out);
+}
+
+// ---------------------------------------------------------------- trace
+static std::vector<TraceRecord> g_trace;
+static bool g_trace_on = false;
+bool trace_enabled() { return g_trace_on; }
+void trace_record(const TraceRecord& r) {
+    if (g_trace_on) g_trace.push_back(r);
+}
+
+static void trace_gemm(int kind, int epi, long M, int N_out, int K, bool quad, int n_shortcut) {
+    if (!g_trace_on) return;
+    const long n_eff = quad ? 4L * N_out : N_out;
+    TraceRecord r{kind, epi, (int)M, N_out, K, 2LL * M * n_eff * K,
+                  2LL * (M * (long long)K + n_eff * (long long)K + M * (long long)N_out * (1 + n_shortcut))};
+    g_trace.push_back(r);
+}
+
+// ---------------------------------------------------------------- graphs
+void Graph::reset() {
+    if (m_exec) {
+        hipGraphExecDestroy(m_exec);
+        m_exec = nullptr;
+    }
+}
+
+__global__ void set_int_kernel(int* dst, int v) { *dst = v; }
+
+void set_device_int(int* dst, int v, hipStream_t s) {
+    hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, s, dst, v);
+    hip_check(hipGetLastError(), "set_device_int");
+}
+
+std::vector<TraceRecord>& trace_buffer() { return g_trace; }
+void trace_set(bool on) {
+    g_trace_on = on;
+    if (on) g_trace.clear();
+}
+
+}  // namespace dcvc

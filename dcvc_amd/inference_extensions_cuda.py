@@ -1,0 +1,233 @@
+"""Drop-in replacement of the reference's ``inference_extensions_cuda`` module
+(src/layers/extensions/inference/bind.cpp:11-39): the same four proxy
+classes with the same methods and argument meaning, backed by the MI355X
+C ABI in libdcvc_amd.so (include/dcvc_codec.h).
+
+Model code imports it as ``from inference_extensions_cuda import DMCHTSProxy``
+(image_model.py:197, video_model_ht.py:419-424) once this package directory
+is on ``sys.path`` under that name (``dcvc_amd.install_module_alias()``).
+
+Tensors are the caller's torch HIP tensors: pictures are fp16 [1, C, H, W]
+in channels_last memory (test_video.py:118-122).  Returned reconstructions
+are views of proxy-owned buffers, overwritten by the next call, as in the
+reference (dmc_hts_proxy.cpp:347, dmci_proxy.cpp:418).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+def _stream():
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _nhwc(t: torch.Tensor) -> torch.Tensor:
+    """fp16 [1, C, H, W] -> contiguous NHWC storage (a view when channels_last)."""
+    if t.dtype != torch.float16:
+        t = t.half()
+    if not t.is_cuda:
+        raise ValueError("inputs must be on the GPU")
+    if not t.is_contiguous(memory_format=torch.channels_last):
+        t = t.contiguous(memory_format=torch.channels_last)
+    return t
+
+
+class _StateDictArgs:
+    """Marshals a state dict into the (names, float32 data, shapes, ndims) arrays."""
+
+    def __init__(self, state_dict):
+        self.keep = []
+        names, ptrs, shapes, ndims = [], [], [], []
+        for k, v in state_dict.items():
+            a = torch.as_tensor(v).detach().to("cpu", torch.float32).contiguous().numpy()
+            if a.ndim > 4:
+                raise ValueError(f"{k}: more than 4 dimensions")
+            self.keep.append(a)
+            names.append(k.encode())
+            ptrs.append(a.ctypes.data)
+            shp = list(a.shape) + [1] * (4 - a.ndim)
+            shapes.extend(shp)
+            ndims.append(a.ndim)
+        n = len(names)
+        self.n = n
+        self.names = (C.c_char_p * n)(*names)
+        self.ptrs = (C.c_void_p * n)(*ptrs)
+        self.shapes = (C.c_int64 * (4 * n))(*shapes)
+        self.ndims = (C.c_int * n)(*ndims)
+
+    def args(self):
+        return (self.n, C.cast(self.names, C.c_void_p), C.cast(self.ptrs, C.c_void_p),
+                C.cast(self.shapes, C.c_void_p), C.cast(self.ndims, C.c_void_p))
+
+
+def _bits_to_numpy(p, n):
+    if n.value == 0:
+        return np.zeros(0, np.uint8)
+    return np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_uint8)), shape=(n.value,)).copy()
+
+
+class DMCIProxy:
+    """DMCIProxy (dmci_proxy.h:132-254)."""
+
+    def __init__(self):
+        self._lib = _lib.load()
+        self._h = self._lib.dcvc_dmci_create()
+        if not self._h:
+            raise _lib.DcvcError("dcvc_dmci_create", 2)
+        self._x_hat = None
+        self._size = None
+        self._device = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.dcvc_dmci_destroy(h)
+            self._h = None
+
+    def set_param(self, state_dict, skip_threshold):
+        dev = state_dict["q_scale_dec"].device if isinstance(state_dict["q_scale_dec"], torch.Tensor) else None
+        self._device = dev if dev is not None and dev.type == "cuda" else torch.device("cuda", torch.cuda.current_device())
+        a = _StateDictArgs(state_dict)
+        _lib.call("dcvc_dmci_set_param", self._h, *a.args(), float(skip_threshold))
+
+    def set_use_graphs(self, on):
+        _lib.call("dcvc_dmci_set_use_graphs", self._h, int(bool(on)))
+
+    def _prepare(self, height, width):
+        Hp, Wp = (height + 15) // 16 * 16, (width + 15) // 16 * 16
+        if self._size != (Hp, Wp):
+            self._x_hat = torch.empty((1, 3, Hp, Wp), dtype=torch.float16, device=self._device,
+                                      memory_format=torch.channels_last)
+            _lib.call("dcvc_dmci_prepare", self._h, Hp, Wp, self._x_hat.data_ptr(), 3)
+            self._size = (Hp, Wp)
+
+    def compress(self, x, qp, padding_b, padding_r):
+        """-> (np.uint8 bitstream, x_hat [1,3,Hp,Wp], ec_parallel)  (dmci_proxy.cpp:296-421)"""
+        x = _nhwc(x)
+        _, c, H, W = x.shape
+        if c != 3:
+            raise ValueError("DMCI input must have 3 channels")
+        self._prepare(H + padding_b, W + padding_r)
+        bits = C.c_void_p()
+        n = C.c_long()
+        ec = C.c_int()
+        _lib.call("dcvc_dmci_compress", self._h, x.data_ptr(), 3, H, W, int(qp), int(padding_b), int(padding_r),
+                  _stream(), C.byref(bits), C.byref(n), C.byref(ec))
+        return _bits_to_numpy(bits, n), self._x_hat, ec.value
+
+    def decompress(self, bit_stream, qp, height, width, entropy_coder_parallel):
+        """-> x_hat [1,3,Hp,Wp]  (dmci_proxy.cpp:423-602)"""
+        self._prepare(height, width)
+        b = np.ascontiguousarray(np.asarray(bit_stream, dtype=np.uint8))
+        _lib.call("dcvc_dmci_decompress", self._h, b.ctypes.data, b.size, int(qp), int(height), int(width),
+                  int(entropy_coder_parallel), _stream())
+        return self._x_hat
+
+    def step_symbol_count(self, step):
+        return int(self._lib.dcvc_dmci_step_symbol_count(self._h, step))
+
+    def debug_copy(self, what, nbytes, dtype):
+        out = np.zeros(nbytes // np.dtype(dtype).itemsize, dtype=dtype)
+        _lib.call("dcvc_dmci_debug_copy", self._h, int(what), out.ctypes.data, out.nbytes)
+        return out
+
+
+class DMCHTSProxy:
+    """DMCHTSProxy (dmc_hts_proxy.h:215-355)."""
+    FRAMES = 8
+
+    def __init__(self):
+        self._lib = _lib.load()
+        self._h = self._lib.dcvc_hts_create()
+        if not self._h:
+            raise _lib.DcvcError("dcvc_hts_create", 2)
+        self._frames = None
+        self._size = None
+        self._device = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.dcvc_hts_destroy(h)
+            self._h = None
+
+    def set_param(self, state_dict, skip_threshold):
+        q = state_dict["q_decoder"]
+        dev = q.device if isinstance(q, torch.Tensor) else None
+        self._device = dev if dev is not None and dev.type == "cuda" else torch.device("cuda", torch.cuda.current_device())
+        a = _StateDictArgs(state_dict)
+        _lib.call("dcvc_hts_set_param", self._h, *a.args(), float(skip_threshold))
+
+    def share_weights_from(self, other: "DMCHTSProxy"):
+        """Use `other`'s weights (one HBM copy for many stream contexts)."""
+        self._device = other._device
+        _lib.call("dcvc_hts_share_weights", self._h, other._h)
+
+    def set_use_graphs(self, on):
+        _lib.call("dcvc_hts_set_use_graphs", self._h, int(bool(on)))
+
+    def _prepare(self, Hp, Wp):
+        if self._size != (Hp, Wp):
+            self._frames = [torch.empty((1, 3, Hp, Wp), dtype=torch.float16, device=self._device,
+                                        memory_format=torch.channels_last) for _ in range(self.FRAMES)]
+            ptrs = (C.c_void_p * self.FRAMES)(*[f.data_ptr() for f in self._frames])
+            _lib.call("dcvc_hts_prepare", self._h, Hp, Wp, C.cast(ptrs, C.c_void_p), 3)
+            self._size = (Hp, Wp)
+
+    def add_ref_feature_from_frame(self, frame, apply_adaptor=True):
+        """(dmc_hts_proxy.cpp:492-502) frame: [1,3,Hp,Wp], a decoded I frame."""
+        frame = _nhwc(frame)
+        _, c, Hp, Wp = frame.shape
+        if c != 3 or Hp % 16 or Wp % 16:
+            raise ValueError("reference frame must be [1,3,H,W] with H, W multiples of 16")
+        self._prepare(Hp, Wp)
+        _lib.call("dcvc_hts_add_ref_feature_from_frame", self._h, frame.data_ptr(), 3, Hp, Wp,
+                  int(bool(apply_adaptor)), _stream())
+
+    def compress(self, x, qp, reset_feature_memory, padding_b, padding_r):
+        """-> (np.uint8 bitstream, ec_parallel)  (dmc_hts_proxy.cpp:504-585); x: [1,24,H,W]"""
+        x = _nhwc(x)
+        _, c, H, W = x.shape
+        if c != 24:
+            raise ValueError("HT-S chunk input must have 24 channels (8 frames)")
+        bits = C.c_void_p()
+        n = C.c_long()
+        ec = C.c_int()
+        _lib.call("dcvc_hts_compress", self._h, x.data_ptr(), 24, H, W, int(qp), int(bool(reset_feature_memory)),
+                  int(padding_b), int(padding_r), _stream(), C.byref(bits), C.byref(n), C.byref(ec))
+        return _bits_to_numpy(bits, n), ec.value
+
+    def decompress(self, bit_stream, qp, height, width, entropy_coder_parallel, reset_feature_memory):
+        """-> list of 8 tensors [1,3,Hp,Wp]  (dmc_hts_proxy.cpp:587-710)"""
+        b = np.ascontiguousarray(np.asarray(bit_stream, dtype=np.uint8))
+        _lib.call("dcvc_hts_decompress", self._h, b.ctypes.data, b.size, int(qp), int(height), int(width),
+                  int(entropy_coder_parallel), int(bool(reset_feature_memory)), _stream())
+        return list(self._frames)
+
+    # --- test hooks (not in the reference interface) ---
+    def debug_copy(self, what, nbytes, dtype):
+        out = np.zeros(nbytes // np.dtype(dtype).itemsize, dtype=dtype)
+        _lib.call("dcvc_hts_debug_copy", self._h, int(what), out.ctypes.data, out.nbytes)
+        return out
+
+    def last_symbol_count(self):
+        return int(self._lib.dcvc_hts_last_symbol_count(self._h))
+
+
+class DMCHTLProxy:
+    """DMCHTLProxy (dmc_htl_proxy.h): not built on MI355X yet."""
+
+    def __init__(self):
+        raise NotImplementedError("the HT-L model structure is not implemented by the MI355X build; use 'hts'")
+
+
+class DMCLDProxy:
+    """DMCLDProxy (dmc_ld_proxy.h): not built on MI355X yet."""
+
+    def __init__(self):
+        raise NotImplementedError("the LD model structure is not implemented by the MI355X build; use 'hts'")

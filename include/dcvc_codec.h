@@ -55,6 +55,73 @@ int dcvc_rans_decoder_set_stream(void* dec, const uint8_t* data, long size);
 int dcvc_rans_decoder_decode_z(void* dec, long n, int cdf_offset, int ch, int8_t* out);
 int dcvc_rans_decoder_decode_y(void* dec, const uint8_t* indexes, long n, int8_t* out);
 
+/* ---------------- codec proxies (inference_extensions_cuda) ----------------
+ * set_param (dmci_proxy.cpp:604-652, dmc_hts_proxy.cpp:712-762): the model's
+ * state dict as n named float32 host arrays (shape[i*4 .. i*4+ndim[i]-1]),
+ * including gaussian_encoder.{quantized_cdf,cdf_length} and
+ * bit_estimator_z.{quantized_cdf,cdf_length}; skip_thres as in the reference.
+ * Pictures are NHWC fp16 device buffers (pixel stride ld), values in
+ * [-0.5, 0.5].  `stream` is the caller's hipStream_t; the proxy orders its
+ * own stream after it and the caller's stream after the proxy's work. */
+
+/* DMCIProxy (dmci_proxy.h:132-254) */
+void* dcvc_dmci_create(void);
+void dcvc_dmci_destroy(void* h);
+int dcvc_dmci_set_param(void* h, int n, const char** names, const float** data, const int64_t* shapes,
+                        const int* ndims, float skip_thres);
+/* buffers for a Hp x Wp (multiple of 16) picture; x_hat = caller-owned
+ * [Hp, Wp, 3] output, the tensor compress/decompress return (a proxy-owned
+ * view in the reference, dmci_proxy.cpp:418, :601). */
+int dcvc_dmci_prepare(void* h, int Hp, int Wp, void* x_hat, int ld_out);
+/* DMCIProxy::compress (dmci_proxy.cpp:296-421): x is [H, W, 3] unpadded. */
+int dcvc_dmci_compress(void* h, const void* x, int ldx, int H, int W, int qp, int pad_b, int pad_r,
+                       void* stream, const uint8_t** bits, long* nbytes, int* ec_parallel);
+/* DMCIProxy::decompress (dmci_proxy.cpp:423-602) */
+int dcvc_dmci_decompress(void* h, const uint8_t* bits, long nbytes, int qp, int height, int width,
+                         int ec_parallel, void* stream);
+
+/* DMCHTSProxy (dmc_hts_proxy.h:215-355) */
+void* dcvc_hts_create(void);
+void dcvc_hts_destroy(void* h);
+int dcvc_hts_set_param(void* h, int n, const char** names, const float** data, const int64_t* shapes,
+                       const int* ndims, float skip_thres);
+/* Shares the weights of `src` (set_param done there) with `h`: one weight
+ * copy in HBM for many stream contexts.  No reference counterpart (the
+ * reference keeps one weight set per proxy). */
+int dcvc_hts_share_weights(void* h, void* src);
+/* 8 caller-owned [Hp, Wp, 3] reconstruction outputs (dmc_hts_proxy.cpp:347) */
+int dcvc_hts_prepare(void* h, int Hp, int Wp, void* const* frames, int ld_out);
+/* add_ref_feature_from_frame (dmc_hts_proxy.cpp:492-502): frame [Hp, Wp, 3] */
+int dcvc_hts_add_ref_feature_from_frame(void* h, const void* frame, int ld, int Hp, int Wp,
+                                        int apply_adaptor, void* stream);
+/* compress (dmc_hts_proxy.cpp:504-585): x is [H, W, 24], 8 frames on channels */
+int dcvc_hts_compress(void* h, const void* x, int ldx, int H, int W, int qp, int reset_feature_memory,
+                      int pad_b, int pad_r, void* stream, const uint8_t** bits, long* nbytes,
+                      int* ec_parallel);
+/* decompress (dmc_hts_proxy.cpp:587-710): writes the 8 prepared frames */
+int dcvc_hts_decompress(void* h, const uint8_t* bits, long nbytes, int qp, int height, int width,
+                        int ec_parallel, int reset_feature_memory, void* stream);
+/* Switch hipGraph capture on (default) or off (eager launches). */
+int dcvc_hts_set_use_graphs(void* h, int on);
+int dcvc_dmci_set_use_graphs(void* h, int on);
+/* Test hook: copies an internal buffer (0 y_hat|reduced, 1 y, 2 q_dec|scales|
+ * means, 3 memory|feature, 4 x|ctx, 5 dense symbols, 6 z) to host memory. */
+int dcvc_hts_debug_copy(void* h, int what, void* host, long bytes);
+/* DMCI test hook: 0-3 dense symbols of encoder step k, 4-7 their conditions,
+ * 8 the final y_hat. */
+int dcvc_dmci_debug_copy(void* h, int what, void* host, long bytes);
+/* Symbols handed to the entropy coder by the last call (per step for DMCI). */
+long dcvc_hts_last_symbol_count(void* h);
+/* Layer-table trace (no reference counterpart): between begin and end, every
+ * GEMM / conv / depthwise launch issued from the host is recorded; end
+ * returns the record count and fills up to max_records rows of 7 int64:
+ * kind (0 1x1, 1 stride-2 conv, 2 sub-pixel conv, 3 depthwise), epilogue,
+ * M, N, K, algorithmic FLOPs, algorithmic bytes.  Graph replays launch
+ * nothing from the host, so trace with graphs off. */
+int dcvc_trace_begin(void);
+int dcvc_trace_end(long long* records, int max_records);
+long dcvc_dmci_step_symbol_count(void* h, int step);
+
 #ifdef __cplusplus
 }
 #endif

@@ -322,6 +322,8 @@ class IntraResult:
     x_hat: torch.Tensor = None
     y: torch.Tensor = None
     scales_steps: list = field(default_factory=list)
+    steps_dense: list = field(default_factory=list)
+    steps_cond: list = field(default_factory=list)
 
 
 def dmci_compress(sd, x, qp, skip_thres=0.0):
@@ -357,6 +359,8 @@ def dmci_compress(sd, x, qp, skip_thres=0.0):
         s_w = s_hat[:, :q] + s_hat[:, q:2 * q] + s_hat[:, 2 * q:3 * q] + s_hat[:, 3 * q:]
         sym16, _, cond = build_index(y_q_w, s_w, skip_thres)
         res.steps_sym16.append(compact(sym16, cond))
+        res.steps_dense.append(sym16)
+        res.steps_cond.append(cond)
         res.scales_steps.append(s_w)
         y_hat_so_far = y_hat if y_hat_so_far is None else y_hat_so_far + y_hat
     y_hat = y_hat_so_far * qsel(sd, "q_scale_y_dec", qp).reshape(1, -1, 1, 1)
@@ -420,6 +424,8 @@ class HTSState:
 class HTSResult:
     sym16: np.ndarray = None
     z_int8: np.ndarray = None
+    sym16_dense: np.ndarray = None
+    cond_dense: np.ndarray = None
     y: torch.Tensor = None
     y_q: torch.Tensor = None
     y_hat: torch.Tensor = None
@@ -438,11 +444,35 @@ def hts_add_ref_feature_from_frame(sd, st: HTSState, frame, apply_adaptor=True):
     return st
 
 
-def _hts_prior(sd, y, z_hat, memory, qp):
-    temporal = rbs2(sd, "temporal_prior_encoder.conv.", memory * qsel(sd, "q_feature", qp).reshape(1, -1, 1, 1), False)
+def hts_encoder(sd, x_padded, ctx, qp):
+    """Encoder (video_model_ht.py:63-92) with the quant tail fused in the last block."""
+    feat = torch.cat([F.pixel_unshuffle(x_padded, 8), ctx], 1)
+    feat = seq(sd, "encoder.conv1.", 6, feat, quant_last=qsel(sd, "q_encoder", qp))
+    return conv(sd, "encoder.down.", feat, stride=2, padding=1)
+
+
+def hts_hyper_encoder(sd, y_pad):
+    """HyperEncoder, HT-S (shortcut=False; video_model_ht.py:184-198)."""
+    z = dcb(sd, "hyper_encoder.conv.0.", y_pad)
+    z = rbs2(sd, "hyper_encoder.conv.1.", z, False)
+    return rbs2(sd, "hyper_encoder.conv.2.", z, False)
+
+
+def hts_hyper_decoder(sd, z_hat):
+    """HyperDecoder, HT-S (shortcut=False; video_model_ht.py:167-181)."""
     hyper = dcb(sd, "hyper_decoder.conv.0.conv.", subpel(sd, "hyper_decoder.conv.0.up.", z_hat))
     hyper = dcb(sd, "hyper_decoder.conv.1.conv.", subpel(sd, "hyper_decoder.conv.1.up.", hyper))
-    hyper = dcb(sd, "hyper_decoder.conv.2.", hyper)
+    return dcb(sd, "hyper_decoder.conv.2.", hyper)
+
+
+def hts_temporal_prior(sd, memory, qp):
+    """TemporalPriorEncoder (video_model_ht.py:307-317)."""
+    return rbs2(sd, "temporal_prior_encoder.conv.", memory * qsel(sd, "q_feature", qp).reshape(1, -1, 1, 1), False)
+
+
+def _hts_prior(sd, y, z_hat, memory, qp):
+    temporal = hts_temporal_prior(sd, memory, qp)
+    hyper = hts_hyper_decoder(sd, z_hat)
     hyper = hyper[:, :, :temporal.shape[2], :temporal.shape[3]]
     x = torch.cat([hyper, temporal], 1)
     for i in range(3):
@@ -470,20 +500,14 @@ def _hts_apply_adaptor(sd, st: HTSState):
     st.ctx = hts_feature_extractor(sd, st.memory)
 
 
-def hts_compress(sd, st: HTSState, x, qp, reset_feature_memory, skip_thres=0.0):
+def hts_compress(sd, st: HTSState, x, qp, reset_feature_memory, skip_thres=0.0, recon=False):
     """DMCHTSProxy::compress (dmc_hts_proxy.cpp:504-585).
 
     ``x``: [1,24,H,W] (8 frames on channels).  The state must hold memory/ctx
     (adaptor applied).  Updates the state for the next chunk as the proxy does.
     """
-    xp = pad_to(x, 16)
-    feat = torch.cat([F.pixel_unshuffle(xp, 8), st.ctx], 1)
-    feat = seq(sd, "encoder.conv1.", 6, feat, quant_last=qsel(sd, "q_encoder", qp))
-    y = conv(sd, "encoder.down.", feat, stride=2, padding=1)
-    y_pad = pad_to(y, 4)
-    z = dcb(sd, "hyper_encoder.conv.0.", y_pad)
-    z = rbs2(sd, "hyper_encoder.conv.1.", z, False)
-    z = rbs2(sd, "hyper_encoder.conv.2.", z, False)
+    y = hts_encoder(sd, pad_to(x, 16), st.ctx, qp)
+    z = hts_hyper_encoder(sd, pad_to(y, 4))
     z_hat = round_z(z)
     params = _hts_prior(sd, y, z_hat, st.memory, qp)
     q_dec, scales, means = params.chunk(3, 1)
@@ -505,8 +529,12 @@ def hts_compress(sd, st: HTSState, x, qp, reset_feature_memory, skip_thres=0.0):
     res = HTSResult(sym16=compact(sym16, cond), y=y, y_q=y_q_tot, y_hat=y_hat, scales=scales,
                     common_params=params,
                     z_int8=z_hat.permute(0, 2, 3, 1).reshape(-1).numpy().astype(np.int8))
+    res.sym16_dense = sym16
+    res.cond_dense = cond
     # lambda_enc_1: decoder + context update (dmc_hts_proxy.cpp:571-574, 832-842)
     st.feature_p = _hts_decoder(sd, y_hat, st.ctx, qp)
+    if recon:
+        res.x_hat, _ = hts_recon_head(sd, st.feature_p)
     if reset_feature_memory:
         _, head = hts_recon_head(sd, st.feature_p, only_reset=True)
         st.memory = hts_feature_adaptor_i(sd, head)

