@@ -20,6 +20,7 @@ F = C.c_float
 SIGNATURES = {
     "dcvc_version": [],
     "dcvc_scale_index_thresholds": [P],
+    "dcvc_gemm_set_tile_policy": [I],
     "dcvc_conv1x1_bias": [P, I, I, I, P, I, P, P, I, P],
     "dcvc_conv1x1_bias_wsilu": [P, I, I, I, P, I, P, P, I, P],
     "dcvc_conv1x1_bias_shortcut": [P, I, I, I, P, I, P, P, I, P, I, P],

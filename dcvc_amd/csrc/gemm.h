@@ -45,9 +45,13 @@ struct GemmArgs {
     const int* qp = nullptr;    // device QP index (nullptr -> row 0)
     f16* D = nullptr;
     int ldd = 0;
+    // filled by gemm_launch: byte extents of A and B for the buffer range checks
+    uint32_t a_bytes = 0, b_bytes = 0;
 };
 
 int gemm_validate(const GemmArgs& g, int epi, int amode, int dmode);
 hipError_t gemm_launch(const GemmArgs& g, int epi, int amode, int dmode, hipStream_t stream);
+// 0: automatic tile choice; 1-3: force one tile configuration (tests, tuning)
+int gemm_set_tile_policy(int policy);
 
 }  // namespace dcvc

@@ -7,8 +7,32 @@
 // B: fp16 weights, row n = output channel, K contiguous (PyTorch conv layout
 //    [N, K, 1, 1]; the strided convs use tap-major K = tap * Cin + c).
 // Accumulation: fp32 on v_mfma_f32_32x32x16_f16.  Each output element is
-// accumulated by one lane in increasing k, 16 at a time, so the result is
-// independent of the tile shape, grid, and batch composition.
+// accumulated by one lane in increasing k, 16 at a time, whatever the tile
+// configuration, so results do not depend on the tile shape, the grid or the
+// batch composition (every configuration below is bit-identical).
+//
+// Structure (gfx950, wave64):
+//   * block tile BM x BN (B rows), 4 waves in a WM x WN grid, wave tile up to
+//     128 x 128 (16 MFMA accumulators of 32x32 in the accumulation registers),
+//     so every k16 step issues 8 LDS fragment reads for 16 MFMAs;
+//   * operands stream into a 4-stage LDS ring (K tile 32) through
+//     buffer_load_dwordx4 ... lds (direct global->LDS, no register staging),
+//     three K tiles in flight; out-of-range rows and conv padding taps read as
+//     zero through the buffer range check.  Each 64-byte LDS row holds its
+//     four 16-byte K chunks XOR-swizzled by bits 1-3 of the row, so the
+//     fragment reads of 8 (or 16) consecutive rows hit distinct banks;
+//   * one barrier per K tile, preceded by an explicit vmcnt wait for exactly
+//     the tile about to be read; fragment reads run one k16 step ahead of the
+//     MFMAs, across the barrier, so the matrix pipe stays fed while waves
+//     synchronise;
+//   * epilogue through LDS: bias / WSiLU / quad-sum in fp32 registers, the
+//     fp32 tile staged in LDS 32 rows per wave at a time, then each lane
+//     finishes 8 consecutive channels of one pixel (shortcut, second
+//     shortcut, per-channel quant) with 16-byte loads and stores, so every
+//     output row segment is written whole;
+//   * XCD-aware tile order: hardware hands block b to XCD b % 8, so
+//     consecutive tiles (the N tiles of one M tile, which share A) are
+//     assigned to blocks of the same XCD and meet in its L2.
 //
 // Reference operators replaced (def_cutlass.h:10-39):
 //   conv1x1_bias, conv1x1_bias_wsilu, conv1x1_bias_shortcut(2),
@@ -22,134 +46,210 @@
 
 namespace dcvc {
 
-template <int WM, int WN, int SM, int SN, int BK, int EPI, int AMODE, int DMODE>
-struct GemmCfg {
-    static constexpr int kWaves = WM * WN;
-    static constexpr int kThreads = kWaves * 64;
-    static constexpr int kBM = WM * SM * 32;
-    static constexpr int kBNE = WN * SN * 32;                      // B rows per block
-    static constexpr bool kQuad = (EPI == EPI_QUAD);
-    static constexpr int kBNO = kQuad ? WN * 32 : kBNE;            // output columns per block
-    static constexpr int kLdsRow = BK + 8;                         // halves, padded
-    static constexpr int kChunksPerRow = BK / 8;                   // 16-byte chunks
-    static constexpr int kAChunks = kBM * kChunksPerRow / kThreads;
-    static constexpr int kBChunks = kBNE * kChunksPerRow / kThreads;
-    static constexpr int kLdsHalves = (kBM + kBNE) * kLdsRow;     // one stage
-    static_assert(!kQuad || SN == 4, "quad GEMM needs 4 B slabs per wave");
-    static_assert(kBM * kChunksPerRow % kThreads == 0, "A tile split");
-    static_assert(kBNE * kChunksPerRow % kThreads == 0, "B tile split");
-};
+namespace {
 
-// Global address of the 16-byte A chunk for tile row `row` (output pixel m)
-// and K offset k (multiple of 8).  Returns nullptr for a zero (padding) chunk.
-template <int AMODE>
-DCVC_DEV const f16* a_chunk_ptr(const GemmArgs& g, int m, int k) {
-    if (m >= g.M) return nullptr;
-    if constexpr (AMODE == A_PLAIN) {
-        return g.A + (size_t)m * g.lda + k;
-    } else {
-        const int hw = g.outH * g.outW;
-        const int img = m / hw;
-        const int r = m - img * hw;
-        const int ho = r / g.outW;
-        const int wo = r - ho * g.outW;
-        const int cin = g.cin;
-        const int tap = k / cin;
-        const int c = k - tap * cin;
-        int hi, wi;
-        if constexpr (AMODE == A_S2K2) {
-            hi = 2 * ho + (tap >> 1);
-            wi = 2 * wo + (tap & 1);
-        } else if constexpr (AMODE == A_S2K3) {
-            hi = 2 * ho - 1 + tap / 3;
-            wi = 2 * wo - 1 + tap % 3;
-        } else {  // A_S1K3
-            hi = ho - 1 + tap / 3;
-            wi = wo - 1 + tap % 3;
-        }
-        if (hi < 0 || hi >= g.inH || wi < 0 || wi >= g.inW) return nullptr;
-        return g.A + ((size_t)(img * g.inH + hi) * g.inW + wi) * g.lda + c;
+constexpr int kBK = 32;                 // K tile (halves): one 64-byte LDS row
+constexpr int kRowBytes = kBK * 2;
+constexpr int kStages = 5;              // LDS ring depth
+constexpr int kAhead = 3;               // K tiles in flight ahead of the one being read
+constexpr uint32_t kOOB = 0x80000000u;  // buffer offset beyond every range: reads zero
+constexpr int kRsrcWord3 = 0x00020000;  // gfx9 raw buffer: 32-bit data format
+
+DCVC_DEV __amdgpu_buffer_rsrc_t make_rsrc(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, kRsrcWord3);
+}
+
+typedef __attribute__((address_space(3))) void* lds_ptr_t;
+
+// 16 bytes per lane from buffer offset `off` straight into LDS at
+// dst + 16 * lane (dst wave-uniform).
+DCVC_DEV void dma16(__amdgpu_buffer_rsrc_t r, char* dst, uint32_t off) {
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (lds_ptr_t)dst, 16, off, 0, 0, 0);
+}
+
+// XOR swizzle of the 16-byte K chunk within a 64-byte LDS row.
+DCVC_DEV int swz(int row) { return ((row >> 2) ^ (row >> 1)) & 3; }
+
+// s_waitcnt vmcnt(VM) (gfx9 encoding; expcnt / lgkmcnt not waited for), then
+// a workgroup barrier.  The compiler fences keep LDS accesses on their side of
+// the barrier; no memory-model fence is used because it would also wait for
+// the K tiles still in flight.
+template <int VM>
+DCVC_DEV void wait_tile_and_barrier() {
+    static_assert(VM >= 0 && VM < 64, "vmcnt range");
+    constexpr int imm = (VM & 15) | ((VM >> 4) << 14) | (7 << 4) | (15 << 8);
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(imm);
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+// Scheduling pattern for one k16 step: NDS fragment reads, NMF MFMAs and NVM
+// DMA issues spread evenly, so the matrix pipe never waits behind a burst of
+// memory instructions.
+template <int NDS, int NMF, int NVM, int R = 0>
+DCVC_DEV void interleave() {
+    if constexpr (R < NDS) {
+        constexpr int vm = (NVM * (R + 1)) / NDS - (NVM * R) / NDS;
+        constexpr int mf = (NMF * (R + 1)) / NDS - (NMF * R) / NDS;
+        if constexpr (vm > 0) __builtin_amdgcn_sched_group_barrier(0x020, vm, 0);  // VMEM read
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);                        // DS read
+        if constexpr (mf > 0) __builtin_amdgcn_sched_group_barrier(0x008, mf, 0);  // MFMA
+        interleave<NDS, NMF, NVM, R + 1>();
     }
 }
 
-template <class Cfg, int EPI, int AMODE, int DMODE, int BK, int WM, int WN, int SM, int SN>
-__global__ __launch_bounds__(Cfg::kThreads) void gemm_kernel(GemmArgs g) {
-    extern __shared__ __attribute__((aligned(16))) f16 lds[];
-    constexpr int BM = Cfg::kBM, BNE = Cfg::kBNE, LR = Cfg::kLdsRow, CPR = Cfg::kChunksPerRow;
+int g_tile_policy = 0;  // 0 automatic, otherwise a forced configuration id
+
+}  // namespace
+
+template <int BM_, int BN_, int WM_, int WN_, int EPI_, int AMODE_, int DMODE_>
+struct Tile {
+    static constexpr int BM = BM_, BN = BN_, WM = WM_, WN = WN_;
+    static constexpr int EPI = EPI_, AMODE = AMODE_, DMODE = DMODE_;
+    static constexpr int kWaves = WM * WN;
+    static constexpr int kThreads = kWaves * 64;
+    static constexpr int TM = BM / WM, TN = BN / WN;
+    static constexpr int SM = TM / 32, SN = TN / 32;
+    static constexpr bool kQuad = (EPI == EPI_QUAD);
+    static constexpr int BNO = kQuad ? BN / 4 : BN;  // output columns per block
+    static constexpr int TNO = kQuad ? TN / 4 : TN;  // output columns per wave
+    static constexpr int ARW = BM / kWaves;           // A rows each wave loads per K tile
+    static constexpr int BRW = BN / kWaves;
+    static constexpr int AI = ARW / 16;               // DMA instructions (16 rows each)
+    static constexpr int BI = BRW / 16;
+    static constexpr int kDma = AI + BI;              // per lane per K tile
+    static constexpr int kStageBytes = (BM + BN) * kRowBytes;
+    static constexpr int kEPitch = BNO + 4;           // epilogue fp32 row pitch
+    static constexpr int kERows = WM * 32;            // rows staged per epilogue pass
+    static constexpr int kEChunks = kERows * BNO / 8;
+    static constexpr size_t kLdsBytes = (kStages * kStageBytes > kERows * kEPitch * 4)
+                                            ? (size_t)kStages * kStageBytes
+                                            : (size_t)kERows * kEPitch * 4;
+    static_assert(TM % 32 == 0 && TN % 32 == 0, "wave tile");
+    static_assert(!kQuad || SN % 4 == 0, "quad GEMM needs the 4 B slabs of an output in one wave");
+    static_assert(ARW % 16 == 0 && BRW % 16 == 0, "DMA split");
+    static_assert(kEChunks % kThreads == 0, "epilogue split");
+    static_assert(kAhead * kDma < 64, "vmcnt range");
+    static_assert(kLdsBytes <= 160 * 1024, "LDS");
+};
+
+template <class T>
+__global__ __launch_bounds__(T::kThreads) void gemm_kernel(GemmArgs g, int tiles_n) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    constexpr int BM = T::BM, BN = T::BN, SM = T::SM, SN = T::SN, NT = T::kThreads;
+    constexpr int AMODE = T::AMODE, EPI = T::EPI;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int wave = tid >> 6;
-    const int wm = wave % WM;
-    const int wn = wave / WM;
-    const int m0 = blockIdx.x * BM;
-    const int nb = blockIdx.y;  // output-column block
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = wave % T::WM;
+    const int wn = wave / T::WM;
 
-    // ---- per-thread global chunk coordinates (fixed across the K loop) ----
-    const f16* a_rowptr[Cfg::kAChunks];
-    int a_lds_off[Cfg::kAChunks];
-    int a_kk[Cfg::kAChunks];
-    int a_m[Cfg::kAChunks];
+    // ---- XCD-aware tile order: XCD x gets a contiguous run of tiles ----
+    const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+    const int xq = nb >> 3, xr = nb & 7, xcd = b & 7;
+    const int tile = xcd * xq + (xcd < xr ? xcd : xr) + (b >> 3);
+    const int mt = tile / tiles_n;
+    const int nt = tile - mt * tiles_n;
+    const int m0 = mt * BM;
+
+    const __amdgpu_buffer_rsrc_t rA = make_rsrc(g.A, g.a_bytes);
+    const __amdgpu_buffer_rsrc_t rB = make_rsrc(g.B, g.b_bytes);
+    // DMA lane geometry: a 16-row group is 1 KB of LDS; lane -> row dr,
+    // 16-byte position dp, holding K chunk kc (row bases are multiples of 16,
+    // so the swizzle of the row equals that of dr)
+    const int dr = lane >> 2;
+    const int kc = (lane & 3) ^ swz(dr);
+
+    // ---- A rows of this lane's DMAs (fixed over the K loop) ----
+    uint32_t a_off[T::AI];
+    int a_pix[T::AI], a_h[T::AI], a_w[T::AI];
 #pragma unroll
-    for (int i = 0; i < Cfg::kAChunks; ++i) {
-        const int chunk = tid + i * Cfg::kThreads;
-        const int row = chunk / CPR;
-        const int kk = (chunk % CPR) * 8;
-        a_m[i] = m0 + row;
-        a_kk[i] = kk;
-        a_lds_off[i] = row * LR + kk;
-        a_rowptr[i] = nullptr;
+    for (int q = 0; q < T::AI; ++q) {
+        const int m = m0 + wave * T::ARW + q * 16 + dr;
+        if constexpr (AMODE == A_PLAIN) {
+            a_off[q] = (m < g.M) ? (uint32_t)(((size_t)m * g.lda + kc * 8) * 2) : kOOB;
+        } else {
+            const int hw = g.outH * g.outW;
+            const int img = m / hw;
+            const int rr = m - img * hw;
+            const int ho = rr / g.outW;
+            const int wo = rr - ho * g.outW;
+            a_pix[q] = img * g.inH * g.inW;
+            if constexpr (AMODE == A_S2K2) {
+                a_h[q] = 2 * ho;
+                a_w[q] = 2 * wo;
+            } else if constexpr (AMODE == A_S2K3) {
+                a_h[q] = 2 * ho - 1;
+                a_w[q] = 2 * wo - 1;
+            } else {
+                a_h[q] = ho - 1;
+                a_w[q] = wo - 1;
+            }
+            if (m >= g.M) a_h[q] = -(1 << 20);  // every tap out of range
+            a_off[q] = 0;
+        }
     }
-    const f16* b_rowptr[Cfg::kBChunks];
-    int b_lds_off[Cfg::kBChunks];
+    // ---- B rows of this lane's DMAs ----
+    uint32_t b_off[T::BI];
 #pragma unroll
-    for (int i = 0; i < Cfg::kBChunks; ++i) {
-        const int chunk = tid + i * Cfg::kThreads;
-        const int row = chunk / CPR;
-        const int kk = (chunk % CPR) * 8;
+    for (int q = 0; q < T::BI; ++q) {
+        const int row = wave * T::BRW + q * 16 + dr;
         int grow;
         bool valid;
-        if constexpr (Cfg::kQuad) {
-            // LDS row -> (wave column, slab j, r): B row 4 * n + j (reference ffn.0 order)
-            const int w_n = row / (SN * 32);
-            const int j = (row / 32) % SN;
-            const int r = row % 32;
-            const int n = nb * Cfg::kBNO + w_n * 32 + r;
+        if constexpr (T::kQuad) {
+            // LDS row -> wave column, sub-tile s (output group s / 4, slab s % 4), row r:
+            // B row 4 * n + slab (the reference ffn.0 channel order, layers.py:114-125)
+            const int w_n = row / T::TN;
+            const int s = (row % T::TN) / 32;
+            const int n = nt * T::BNO + w_n * T::TNO + (s >> 2) * 32 + (row & 31);
             valid = n < g.N;
-            grow = 4 * n + j;
+            grow = 4 * n + (s & 3);
         } else {
-            grow = nb * BNE + row;
+            grow = nt * T::BN + row;
             valid = grow < g.N;
         }
-        b_rowptr[i] = valid ? g.B + (size_t)grow * g.K + kk : nullptr;
-        b_lds_off[i] = (BM + row) * LR + kk;
+        b_off[q] = valid ? (uint32_t)(((size_t)grow * g.K + kc * 8) * 2) : kOOB;
     }
 
-    f16x8 ra[Cfg::kAChunks], rb[Cfg::kBChunks];
-    const f16x8 zero8 = {0, 0, 0, 0, 0, 0, 0, 0};
-
-    auto gload = [&](int k0) {
+    // issue the DMAs of K tile kt into ring stage st; tiles past the end read
+    // zeros into a slot that is never read (uniform instruction counts keep
+    // the vmcnt bookkeeping static)
+    const int KT = g.K / kBK;
+    auto dma_tile = [&](int kt, int st) {
+        const uint32_t past = (kt < KT) ? 0u : kOOB;
+        const int k0 = (kt < KT) ? kt * kBK : 0;
+        char* sa = smem + st * T::kStageBytes + wave * T::ARW * kRowBytes;
+        char* sb = smem + st * T::kStageBytes + (BM + wave * T::BRW) * kRowBytes;
+        if constexpr (AMODE == A_PLAIN) {
 #pragma unroll
-        for (int i = 0; i < Cfg::kAChunks; ++i) {
-            const f16* p;
-            if constexpr (AMODE == A_PLAIN) {
-                p = (a_m[i] < g.M) ? g.A + (size_t)a_m[i] * g.lda + k0 + a_kk[i] : nullptr;
+            for (int q = 0; q < T::AI; ++q)
+                dma16(rA, sa + q * 16 * kRowBytes, (a_off[q] + (uint32_t)k0 * 2) | past);
+        } else {
+            // cin % 64 == 0, so one K tile never straddles two taps
+            const int tap = k0 / g.cin;
+            const int c0 = k0 - tap * g.cin;
+            int dy, dx;
+            if constexpr (AMODE == A_S2K2) {
+                dy = tap >> 1;
+                dx = tap & 1;
             } else {
-                p = a_chunk_ptr<AMODE>(g, a_m[i], k0 + a_kk[i]);
+                dy = tap / 3;
+                dx = tap - dy * 3;
             }
-            ra[i] = p ? *reinterpret_cast<const f16x8*>(p) : zero8;
+#pragma unroll
+            for (int q = 0; q < T::AI; ++q) {
+                const int hi = a_h[q] + dy, wi = a_w[q] + dx;
+                const bool ok = (unsigned)hi < (unsigned)g.inH && (unsigned)wi < (unsigned)g.inW;
+                const uint32_t off =
+                    ok ? (uint32_t)(((size_t)(a_pix[q] + hi * g.inW + wi) * g.lda + c0 + kc * 8) * 2) : kOOB;
+                dma16(rA, sa + q * 16 * kRowBytes, off | past);
+            }
         }
 #pragma unroll
-        for (int i = 0; i < Cfg::kBChunks; ++i)
-            rb[i] = b_rowptr[i] ? *reinterpret_cast<const f16x8*>(b_rowptr[i] + k0) : zero8;
+        for (int q = 0; q < T::BI; ++q)
+            dma16(rB, sb + q * 16 * kRowBytes, (b_off[q] + (uint32_t)k0 * 2) | past);
     };
-    auto lstore = [&](int stage) {
-        f16* base = lds + stage * Cfg::kLdsHalves;
-#pragma unroll
-        for (int i = 0; i < Cfg::kAChunks; ++i) *reinterpret_cast<f16x8*>(base + a_lds_off[i]) = ra[i];
-#pragma unroll
-        for (int i = 0; i < Cfg::kBChunks; ++i) *reinterpret_cast<f16x8*>(base + b_lds_off[i]) = rb[i];
-    };
-    (void)a_rowptr;
 
     f32x16 acc[SM][SN];
 #pragma unroll
@@ -159,116 +259,221 @@ __global__ __launch_bounds__(Cfg::kThreads) void gemm_kernel(GemmArgs g) {
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
 
-    const int KT = g.K / BK;
-    gload(0);
-    lstore(0);
-    __syncthreads();
+    // fragment addresses: lane reads row lr, K chunk 2 * ks + (lane >> 5)
     const int lr = lane & 31;
-    const int lk = (lane >> 5) * 8;
-    for (int kt = 0; kt < KT; ++kt) {
-        if (kt + 1 < KT) gload((kt + 1) * BK);
-        const f16* base = lds + (kt & 1) * Cfg::kLdsHalves;
+    const int fa_row = (wm * T::TM + lr) * kRowBytes;
+    const int fb_row = (BM + wn * T::TN + lr) * kRowBytes;
+    const int pos0 = ((0 + (lane >> 5)) ^ swz(lr)) * 16;
+    const int pos1 = ((2 + (lane >> 5)) ^ swz(lr)) * 16;
+    f16x8 f0a[SM], f0b[SN], f1a[SM], f1b[SN];
+    auto read_frags = [&](int st, int pos, f16x8* fa, f16x8* fb) {
+        const char* base = smem + st * T::kStageBytes + pos;
 #pragma unroll
-        for (int ks = 0; ks < BK / 16; ++ks) {
-            f16x8 fa[SM], fb[SN];
+        for (int i = 0; i < SM; ++i)
+            fa[i] = *reinterpret_cast<const f16x8*>(base + fa_row + i * 32 * kRowBytes);
 #pragma unroll
-            for (int i = 0; i < SM; ++i)
-                fa[i] = *reinterpret_cast<const f16x8*>(base + ((wm * SM + i) * 32 + lr) * LR + ks * 16 + lk);
+        for (int j = 0; j < SN; ++j)
+            fb[j] = *reinterpret_cast<const f16x8*>(base + fb_row + j * 32 * kRowBytes);
+    };
+    auto mfma_all = [&](const f16x8* fa, const f16x8* fb) {
+#pragma unroll
+        for (int i = 0; i < SM; ++i)
 #pragma unroll
             for (int j = 0; j < SN; ++j)
-                fb[j] = *reinterpret_cast<const f16x8*>(base + (BM + (wn * SN + j) * 32 + lr) * LR + ks * 16 + lk);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+    };
+
+    // K loop.  Tile kt lives in ring slot kt % 5; at iteration kt the DMAs of
+    // tiles kt + 1 .. kt + 3 are in flight and tile kt + 3 goes to the slot of
+    // tile kt - 2, whose fragment reads every wave consumed (so completed)
+    // before reaching this iteration's barrier.
+    constexpr int NDS = SM + SN, NMF = SM * SN;
 #pragma unroll
-            for (int i = 0; i < SM; ++i)
-#pragma unroll
-                for (int j = 0; j < SN; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fa[i], fb[j], acc[i][j], 0, 0, 0);
-        }
-        if (kt + 1 < KT) lstore((kt + 1) & 1);
-        __syncthreads();
+    for (int st = 0; st < kAhead; ++st) dma_tile(st, st);
+    wait_tile_and_barrier<(kAhead - 1) * T::kDma>();
+    dma_tile(kAhead, kAhead);
+    read_frags(0, pos0, f0a, f0b);
+    read_frags(0, pos1, f1a, f1b);
+    mfma_all(f0a, f0b);
+    for (int kt = 1; kt < KT; ++kt) {
+        // this wave's DMAs of tile kt have landed; after the barrier, every wave's
+        wait_tile_and_barrier<(kAhead - 1) * T::kDma>();
+        const int st = kt % kStages;
+        dma_tile(kt + kAhead, (kt + kAhead) % kStages);
+        read_frags(st, pos0, f0a, f0b);
+        mfma_all(f1a, f1b);  // second k16 step of tile kt - 1
+        interleave<NDS, NMF, T::kDma>();
+        read_frags(st, pos1, f1a, f1b);
+        mfma_all(f0a, f0b);
+        interleave<NDS, NMF, 0>();
     }
+    mfma_all(f1a, f1b);
+    __syncthreads();  // every wave is done with the ring before the epilogue reuses it
 
     // ---- epilogue ----
+    // bias (and activation) in registers; accumulator lane layout of a 32x32
+    // tile: column lane & 31, row (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3).
+    constexpr int NG = T::kQuad ? SN / 4 : SN;
+    float bias_r[SN];
+#pragma unroll
+    for (int s = 0; s < SN; ++s) {
+        if constexpr (T::kQuad) {
+            const int n = nt * T::BNO + wn * T::TNO + (s >> 2) * 32 + lr;
+            bias_r[s] = (n < g.N) ? g.bias[4 * n + (s & 3)] : 0.f;
+        } else {
+            const int n = nt * BN + wn * T::TN + s * 32 + lr;
+            bias_r[s] = (n < g.N) ? g.bias[n] : 0.f;
+        }
+    }
     const f16* q = nullptr;
     if constexpr (EPI == EPI_SHORTCUT_QUANT || EPI == EPI_QUANT) {
         const int qp = g.qp ? *g.qp : 0;
         q = g.q + (size_t)qp * g.qstride;
     }
+    float* E = reinterpret_cast<float*>(smem);
+    constexpr int EP = T::kEPitch;
     const int hwo = g.outH * g.outW;
 #pragma unroll
     for (int i = 0; i < SM; ++i) {
+        if (i > 0) __syncthreads();  // previous pass fully read back
+        // -- stage rows wm * 32 + r of pass i (tile rows wm * TM + i * 32 + r) --
 #pragma unroll
-        for (int v = 0; v < 16; ++v) {
-            const int m = m0 + (wm * SM + i) * 32 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
-            if (m >= g.M) continue;
-            if constexpr (Cfg::kQuad) {
-                const int n = nb * Cfg::kBNO + wn * 32 + lr;
-                if (n >= g.N) continue;
-                float s = 0.f;
+        for (int gI = 0; gI < NG; ++gI) {
+            const int col = wn * T::TNO + gI * 32 + lr;
 #pragma unroll
-                for (int j = 0; j < SN; ++j) s += wsilu(acc[i][j][v] + g.bias[4 * n + j]);
-                g.D[(size_t)m * g.ldd + n] = (f16)s;
-            } else {
+            for (int v = 0; v < 16; ++v) {
+                const int row = wm * 32 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+                float x;
+                if constexpr (T::kQuad) {
+                    x = 0.f;
 #pragma unroll
-                for (int j = 0; j < SN; ++j) {
-                    const int n = nb * BNE + (wn * SN + j) * 32 + lr;
-                    if (n >= g.N) continue;
-                    float x = acc[i][j][v] + g.bias[n];
+                    for (int j = 0; j < 4; ++j) x += wsilu(acc[i][4 * gI + j][v] + bias_r[4 * gI + j]);
+                } else {
+                    x = acc[i][gI][v] + bias_r[gI];
                     if constexpr (EPI == EPI_WSILU) x = wsilu(x);
-                    if constexpr (EPI == EPI_SHORTCUT || EPI == EPI_SHORTCUT2 || EPI == EPI_SHORTCUT_QUANT)
-                        x += (float)g.S1[(size_t)m * g.lds1 + n];
-                    if constexpr (EPI == EPI_SHORTCUT2) x += (float)g.S2[(size_t)m * g.lds2 + n];
-                    if constexpr (EPI == EPI_SHORTCUT_QUANT || EPI == EPI_QUANT) x *= (float)q[n];
-                    size_t off;
-                    if constexpr (DMODE == D_SUBPEL) {
-                        // n = sub * Cout + c ; pixel (2h + dy, 2w + dx) of the 2x grid
-                        const int cout = g.N >> 2;
-                        const int sub = n / cout;
-                        const int c = n - sub * cout;
-                        const int img = m / hwo;
-                        const int r = m - img * hwo;
-                        const int h = r / g.outW;
-                        const int w = r - h * g.outW;
-                        const int H2 = 2 * g.outH, W2 = 2 * g.outW;
-                        off = ((size_t)(img * H2 + 2 * h + (sub >> 1)) * W2 + 2 * w + (sub & 1)) * g.ldd + c;
-                    } else {
-                        off = (size_t)m * g.ldd + n;
-                    }
-                    g.D[off] = (f16)x;
                 }
+                E[row * EP + col] = x;
             }
+        }
+        __syncthreads();
+        // -- finish 8 consecutive channels of one pixel per lane --
+#pragma unroll
+        for (int t = 0; t < T::kEChunks / NT; ++t) {
+            const int c = tid + t * NT;
+            const int rl = c / (T::BNO / 8);
+            const int cc = (c - rl * (T::BNO / 8)) * 8;
+            const int m = m0 + (rl >> 5) * T::TM + i * 32 + (rl & 31);
+            const int n = nt * T::BNO + cc;
+            if (m >= g.M || n >= g.N) continue;
+            const f32x4 lo = *reinterpret_cast<const f32x4*>(E + rl * EP + cc);
+            const f32x4 hi = *reinterpret_cast<const f32x4*>(E + rl * EP + cc + 4);
+            float x[8] = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+            if constexpr (EPI == EPI_SHORTCUT || EPI == EPI_SHORTCUT2 || EPI == EPI_SHORTCUT_QUANT) {
+                const f16x8 s = *reinterpret_cast<const f16x8*>(g.S1 + (size_t)m * g.lds1 + n);
+#pragma unroll
+                for (int e = 0; e < 8; ++e) x[e] += (float)s[e];
+            }
+            if constexpr (EPI == EPI_SHORTCUT2) {
+                const f16x8 s = *reinterpret_cast<const f16x8*>(g.S2 + (size_t)m * g.lds2 + n);
+#pragma unroll
+                for (int e = 0; e < 8; ++e) x[e] += (float)s[e];
+            }
+            if constexpr (EPI == EPI_SHORTCUT_QUANT || EPI == EPI_QUANT) {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) x[e] *= (float)q[n + e];
+            }
+            f16x8 o;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) o[e] = (f16)x[e];
+            size_t off;
+            if constexpr (T::DMODE == D_SUBPEL) {
+                // n = sub * Cout + c ; pixel (2h + sub / 2, 2w + sub % 2) of the 2x grid
+                const int cout = g.N >> 2;
+                const int sub = n / cout;
+                const int ch = n - sub * cout;
+                const int img = m / hwo;
+                const int r = m - img * hwo;
+                const int h = r / g.outW;
+                const int w = r - h * g.outW;
+                off = ((size_t)(img * 2 * g.outH + 2 * h + (sub >> 1)) * (2 * g.outW) + 2 * w + (sub & 1)) * g.ldd + ch;
+            } else {
+                off = (size_t)m * g.ldd + n;
+            }
+            *reinterpret_cast<f16x8*>(g.D + off) = o;
         }
     }
 }
 
-template <int WM, int WN, int SM, int SN, int BK, int EPI, int AMODE, int DMODE>
-static hipError_t launch_cfg(const GemmArgs& g, hipStream_t stream) {
-    using Cfg = GemmCfg<WM, WN, SM, SN, BK, EPI, AMODE, DMODE>;
-    const int n_out = (EPI == EPI_QUAD) ? g.N : g.N;  // output columns covered by grid.y
-    dim3 grid(ceil_div(g.M, Cfg::kBM), ceil_div(n_out, Cfg::kBNO));
-    const size_t lds = 2 * Cfg::kLdsHalves * sizeof(f16);
-    auto kern = gemm_kernel<Cfg, EPI, AMODE, DMODE, BK, WM, WN, SM, SN>;
-    static bool attr_set = false;
+// ---------------------------------------------------------------------------
+// configurations and dispatch
+// ---------------------------------------------------------------------------
+
+// Tile configurations (ids for dcvc_gemm_set_tile_policy):
+//   1: 256 x 256, wave 128 x 128     2: 256 x 128, wave 128 x 64
+//   3: 128 x 128, wave 64 x 64
+//   quad (B rows): 1: 256 x 256 (64 outputs), 2: 128 x 256 (64 outputs),
+//   3: 128 x 128, waves 4 x 1 (32 outputs)
+template <int BM, int BN, int WM, int WN, int EPI, int AMODE, int DMODE>
+static hipError_t launch_tile(GemmArgs g, hipStream_t stream) {
+    using T = Tile<BM, BN, WM, WN, EPI, AMODE, DMODE>;
+    const int tiles_m = ceil_div(g.M, BM);
+    const int tiles_n = ceil_div(g.N, T::BNO);
+    auto kern = gemm_kernel<T>;
+    static bool attr_set = false;  // set once per instantiation (host-side, before first launch)
     if (!attr_set) {
-        hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)T::kLdsBytes);
+        if (e != hipSuccess) return e;
         attr_set = true;
     }
-    hipLaunchKernelGGL(kern, grid, dim3(Cfg::kThreads), lds, stream, g);
+    hipLaunchKernelGGL(kern, dim3(tiles_m * tiles_n), dim3(T::kThreads), T::kLdsBytes, stream, g, tiles_n);
     return hipGetLastError();
+}
+
+// Automatic choice: the largest tile that still gives about one block per CU
+// (256 CUs); smaller tiles for the small 1/16- and 1/64-resolution layers.
+static int pick_tile(const GemmArgs& g, bool quad) {
+    if (g_tile_policy >= 1 && g_tile_policy <= 3) return g_tile_policy;
+    const long long m256 = ceil_div(g.M, 256), m128 = ceil_div(g.M, 128);
+    if (quad) {
+        const long long n64 = ceil_div(g.N, 64);
+        if (m256 * n64 >= 240) return 1;
+        if (m128 * n64 >= 240) return 2;
+        return 3;
+    }
+    if (m256 * ceil_div(g.N, 256) >= 240) return 1;
+    if (m256 * ceil_div(g.N, 128) >= 240) return 2;
+    return 3;
 }
 
 template <int EPI, int AMODE, int DMODE>
 static hipError_t dispatch_tiles(const GemmArgs& g, hipStream_t stream) {
+    const int t = pick_tile(g, EPI == EPI_QUAD);
     if constexpr (EPI == EPI_QUAD) {
-        // one 32-column output slab per wave column, 4 B slabs per wave
-        return launch_cfg<4, 1, 1, 4, 64, EPI, AMODE, DMODE>(g, stream);
+        if (t == 1) return launch_tile<256, 256, 2, 2, EPI, AMODE, DMODE>(g, stream);
+        if (t == 2) return launch_tile<128, 256, 2, 2, EPI, AMODE, DMODE>(g, stream);
+        return launch_tile<128, 128, 4, 1, EPI, AMODE, DMODE>(g, stream);
     } else {
-        if (g.N <= 64 || g.M <= 2048)
-            return launch_cfg<4, 1, 1, 2, 64, EPI, AMODE, DMODE>(g, stream);  // 128 x 64
-        return launch_cfg<2, 2, 2, 2, 64, EPI, AMODE, DMODE>(g, stream);      // 128 x 128
+        if (t == 1) return launch_tile<256, 256, 2, 2, EPI, AMODE, DMODE>(g, stream);
+        if (t == 2) return launch_tile<256, 128, 2, 2, EPI, AMODE, DMODE>(g, stream);
+        return launch_tile<128, 128, 2, 2, EPI, AMODE, DMODE>(g, stream);
     }
 }
 
-hipError_t gemm_launch(const GemmArgs& g, int epi, int amode, int dmode, hipStream_t stream) {
+hipError_t gemm_launch(const GemmArgs& g_in, int epi, int amode, int dmode, hipStream_t stream) {
+    GemmArgs g = g_in;
+    // byte extents of the operands for the buffer range checks
+    size_t a_rows;
+    int a_last;
+    if (amode == A_PLAIN) {
+        a_rows = (size_t)g.M;
+        a_last = g.K;
+    } else {
+        a_rows = (size_t)(g.M / (g.outH * g.outW)) * g.inH * g.inW;
+        a_last = g.cin;
+    }
+    g.a_bytes = (uint32_t)(((a_rows - 1) * g.lda + a_last) * 2);
+    g.b_bytes = (uint32_t)((size_t)(epi == EPI_QUAD ? 4 * g.N : g.N) * g.K * 2);
 #define DCVC_EPI_CASE(E)                                                                        \
     case E:                                                                                     \
         if (dmode == D_SUBPEL) return dispatch_tiles<E, A_PLAIN, D_SUBPEL>(g, stream);          \
@@ -295,13 +500,16 @@ hipError_t gemm_launch(const GemmArgs& g, int epi, int amode, int dmode, hipStre
 #undef DCVC_EPI_CASE
 }
 
+static bool aligned16(const void* p) { return ((uintptr_t)p) % 16 == 0; }
+
 // Host-side validation shared by every GEMM entry point.
 int gemm_validate(const GemmArgs& g, int epi, int amode, int dmode) {
     if (!g.A || !g.B || !g.D || !g.bias) return DCVC_ERR_ARG;
     if (g.M <= 0 || g.N <= 0 || g.K <= 0) return DCVC_ERR_ARG;
-    if (g.K % 64 != 0) return DCVC_ERR_ARG;
+    if (g.K % 64 != 0 || g.N % 8 != 0) return DCVC_ERR_ARG;
     if (g.lda % 8 != 0 || g.lda < (amode == A_PLAIN ? g.K : g.cin)) return DCVC_ERR_ARG;
-    if (((uintptr_t)g.A) % 16 != 0 || ((uintptr_t)g.B) % 16 != 0) return DCVC_ERR_ARG;
+    if (!aligned16(g.A) || !aligned16(g.B) || !aligned16(g.D) || g.ldd % 8 != 0) return DCVC_ERR_ARG;
+    size_t a_rows = (size_t)g.M;
     if (amode != A_PLAIN) {
         const int taps = (amode == A_S2K2) ? 4 : 9;
         if (g.cin <= 0 || g.cin % 64 != 0 || g.cin * taps != g.K) return DCVC_ERR_ARG;
@@ -310,17 +518,28 @@ int gemm_validate(const GemmArgs& g, int epi, int amode, int dmode) {
         if (amode == A_S2K2 && (g.inH != 2 * g.outH || g.inW != 2 * g.outW)) return DCVC_ERR_ARG;
         if (amode == A_S2K3 && (g.outH != (g.inH + 1) / 2 || g.outW != (g.inW + 1) / 2)) return DCVC_ERR_ARG;
         if (amode == A_S1K3 && (g.outH != g.inH || g.outW != g.inW)) return DCVC_ERR_ARG;
+        a_rows = (size_t)(g.M / (g.outH * g.outW)) * g.inH * g.inW;
     }
+    // buffer addressing uses 32-bit byte offsets below 2^31
+    if (a_rows * (size_t)g.lda * 2 >= kOOB) return DCVC_ERR_ARG;
+    if ((size_t)(epi == EPI_QUAD ? 4 * g.N : g.N) * g.K * 2 >= kOOB) return DCVC_ERR_ARG;
     if (dmode == D_SUBPEL) {
-        if (g.N % 4 != 0 || g.outH <= 0 || g.outW <= 0 || g.M % (g.outH * g.outW) != 0) return DCVC_ERR_ARG;
+        if (g.N % 32 != 0 || g.outH <= 0 || g.outW <= 0 || g.M % (g.outH * g.outW) != 0) return DCVC_ERR_ARG;
         if (g.ldd < g.N / 4) return DCVC_ERR_ARG;
     } else if (g.ldd < g.N) {
         return DCVC_ERR_ARG;
     }
-    if ((epi == EPI_SHORTCUT || epi == EPI_SHORTCUT2 || epi == EPI_SHORTCUT_QUANT) && (!g.S1 || g.lds1 < g.N))
-        return DCVC_ERR_ARG;
-    if (epi == EPI_SHORTCUT2 && (!g.S2 || g.lds2 < g.N)) return DCVC_ERR_ARG;
+    if (epi == EPI_SHORTCUT || epi == EPI_SHORTCUT2 || epi == EPI_SHORTCUT_QUANT) {
+        if (!g.S1 || g.lds1 < g.N || g.lds1 % 8 != 0 || !aligned16(g.S1)) return DCVC_ERR_ARG;
+    }
+    if (epi == EPI_SHORTCUT2 && (!g.S2 || g.lds2 < g.N || g.lds2 % 8 != 0 || !aligned16(g.S2))) return DCVC_ERR_ARG;
     if ((epi == EPI_SHORTCUT_QUANT || epi == EPI_QUANT) && !g.q) return DCVC_ERR_ARG;
+    return DCVC_OK;
+}
+
+int gemm_set_tile_policy(int policy) {
+    if (policy < 0 || policy > 3) return DCVC_ERR_ARG;
+    g_tile_policy = policy;
     return DCVC_OK;
 }
 

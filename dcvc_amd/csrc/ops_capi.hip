@@ -46,6 +46,8 @@ int dcvc_scale_index_thresholds(float* th128) {
     return DCVC_OK;
 }
 
+int dcvc_gemm_set_tile_policy(int policy) { return gemm_set_tile_policy(policy); }
+
 int dcvc_conv1x1_bias(const void* A, int lda, int M, int K, const void* W, int N, const float* bias, void* D,
                       int ldd, void* stream) {
     GemmArgs g = plain_args(A, lda, M, K, W, N, bias, D, ldd);

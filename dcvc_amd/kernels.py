@@ -92,7 +92,6 @@ def conv1x1(a, w, bias, epi="bias", s1=None, s2=None, q=None, qp=None, out=None)
 
 def conv_bias(x, n_img, H, W, wt_ohwi, bias, kernel, stride, out=None):
     """x [n_img*H*W, Cin]; wt_ohwi [N, k*k*Cin] fp16 tap-major."""
-    Cin = x.shape[1] if x.stride(0) == x.shape[1] else None
     Cin = wt_ohwi.shape[1] // (kernel * kernel)
     N = wt_ohwi.shape[0]
     Ho = (H + stride - 1) // stride if kernel == 3 else H // 2

@@ -30,9 +30,17 @@ int dcvc_version(void);
  * (def_const.h:6-11, elementwise/stream.cu:77-87).  Host-only. */
 int dcvc_scale_index_thresholds(float* th128);
 
+/* GEMM tile configuration: 0 automatic (default), 1-3 force one of the
+ * three tile shapes for every GEMM launched afterwards (process-wide).
+ * No counterpart in the reference: used by tests (all shapes must give
+ * bit-identical results) and by tile tuning.  Host-only. */
+int dcvc_gemm_set_tile_policy(int policy);
+
 /* ---------------- 1x1-conv GEMMs (MFMA, fp32 accumulate) ----------------
  * A [M x K] fp16 (row stride lda), W [N x K] fp16 (PyTorch [N,K,1,1]),
- * bias [N] fp32, D [M x N] fp16 (row stride ldd).  K % 64 == 0.
+ * bias [N] fp32, D [M x N] fp16 (row stride ldd).  K % 64 == 0,
+ * N % 8 == 0, lda / ldd / shortcut strides % 8 == 0, A / W / D / shortcut
+ * pointers 16-byte aligned.
  * D may alias S1 (in-place shortcut), as the reference's out_buf does.
  * q: per-QP table of fp16 rows (row stride qstride); qp: device int* row
  * index, or NULL for row 0. */

@@ -134,6 +134,50 @@ def test_gemm_batch_invariance(gpu):
     assert torch.equal(full[:100], small)
 
 
+@pytest.mark.parametrize("epi", ["bias", "wsilu", "shortcut2", "shortcut_quant", "quad", "s2k3", "s1k3", "subpel"])
+def test_gemm_tile_configs_bit_identical(gpu, epi):
+    """Every tile configuration gives the same bits (same per-element accumulation
+    order), including ragged M / N tails, implicit convs and the sub-pixel store."""
+    from dcvc_amd import _lib, kernels
+    lib = _lib.load()
+    r = rng(21)
+    M, K, N = 2999, 512, 200
+    a = f16(r.standard_normal((M, K)), gpu)
+    s1 = f16(r.standard_normal((M, N)), gpu)
+    s2 = f16(r.standard_normal((M, N)), gpu)
+    qtab = f16(1 + 0.5 * r.random((3, N)), gpu)
+    qp = torch.tensor([1], dtype=torch.int32, device=gpu)
+    H, W, cin = 37, 53, 128
+
+    def run():
+        if epi == "quad":
+            w = f16(r0.standard_normal((4 * N, K)) / 20, gpu)
+            return kernels.conv1x1(a, w, torch.zeros(4 * N, device=gpu), epi="quad")
+        if epi in ("s2k3", "s1k3"):
+            x = f16(r0.standard_normal((H * W, cin)), gpu)
+            w = f16(r0.standard_normal((N, 9 * cin)) / 30, gpu)
+            return kernels.conv_bias(x, 1, H, W, w, torch.ones(N, device=gpu), 3, 2 if epi == "s2k3" else 1)
+        if epi == "subpel":
+            x = f16(r0.standard_normal((H * W, cin)), gpu)
+            w = f16(r0.standard_normal((4 * 64, cin)) / 10, gpu)
+            return kernels.transposed_conv(x, 1, H, W, w, 64, torch.ones(4 * 64, device=gpu))
+        w = f16(r0.standard_normal((N, K)) / 20, gpu)
+        return kernels.conv1x1(a, w, torch.full((N,), 0.1, device=gpu), epi=epi, s1=s1, s2=s2, q=qtab, qp=qp)
+
+    outs = []
+    try:
+        for policy in (1, 2, 3):
+            assert lib.dcvc_gemm_set_tile_policy(policy) == 0
+            r0 = rng(5)
+            outs.append(run())
+    finally:
+        lib.dcvc_gemm_set_tile_policy(0)
+    torch.cuda.synchronize()
+    assert torch.isfinite(outs[0].float()).all()
+    for o in outs[1:]:
+        assert torch.equal(o.view(torch.int16), outs[0].view(torch.int16))
+
+
 # ------------------------------------------------------------------ convs
 
 @pytest.mark.parametrize("kernel,stride,H,W,cin,n,n_img", [
