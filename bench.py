@@ -8,9 +8,15 @@ weights.  A step is one chunk compress (encoder proxy) followed by its
 decompress (decoder proxy), including the CPU entropy coder.  `value` counts
 the 4 real frames per step.
 
-Also reported: p50/p99 step latency, the native 8-frame rate, the layer-table
-work per chunk, the dominant kernel's roofline (timed with HIP events on its
-launch stream) and the FP32 CPU oracle timed on this host.
+Also reported: p50/p99 step latency over the timed steps, the native
+8-frame rate, the per-launch layer table of one encode + decode (algorithmic
+FLOPs / bytes, each launch timed with HIP events on its own stream), the
+roofline of the kernel class that takes the most time, and the FP32 CPU
+oracle timed on this host.
+
+--profile-step runs the warm-up and one step, then a 256 MiB device copy
+(the calibration kernel for the FETCH_SIZE / WRITE_SIZE counters), and
+prints nothing else: the program to put under rocprofv3.
 """
 from __future__ import annotations
 
@@ -28,68 +34,91 @@ sys.path.insert(0, ROOT)
 METRIC = "1080p encode+decode frames/s on 1 MI355X, BF16/FP8 vs FP16 plain; p99 latency"
 PEAK_FP16_TFLOPS = 2500.0   # MI355X dense FP16/BF16 MFMA
 PEAK_HBM_GBS = 8000.0
+EPI_NAMES = {0: "bias", 1: "wsilu", 2: "shortcut", 3: "shortcut2", 4: "shortcut_quant", 5: "quant", 6: "quad"}
+KIND_NAMES = {0: "conv1x1", 1: "conv_s2", 2: "subpel", 3: "d3x3"}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def layer_table(enc, dec, x, qp, pad_b, pad_r, H, W):
-    """One encode and one decode with graphs off and the launch trace on;
-    returns the (n, 7) record arrays of both sides."""
+def layer_trace(enc, dec, x, qp, pad_b, pad_r, H, W):
+    """One encode and one decode with graphs off and the timed launch trace
+    on; returns the (n, 8) record array of both sides."""
     import ctypes as C
     import torch
     from dcvc_amd import _lib
     lib = _lib.load()
-    buf = np.zeros((50000, 7), np.int64)
+    buf = np.zeros((50000, 8), np.int64)
     ptr = buf.ctypes.data_as(C.c_void_p)
     enc.proxy.set_use_graphs(False)
     dec.proxy.set_use_graphs(False)
-    lib.dcvc_trace_begin()
-    bits, ec = enc.proxy.compress(x, qp, False, pad_b, pad_r)
-    torch.cuda.synchronize()
-    rec_enc = buf[:lib.dcvc_trace_end(ptr, len(buf))].copy()
-    lib.dcvc_trace_begin()
-    dec.proxy.decompress(bits, qp, H, W, ec, False)
-    torch.cuda.synchronize()
-    rec_dec = buf[:lib.dcvc_trace_end(ptr, len(buf))].copy()
-    enc.proxy.set_use_graphs(True)
-    dec.proxy.set_use_graphs(True)
+    try:
+        _lib.call("dcvc_trace_begin", 1)
+        bits, ec = enc.proxy.compress(x, qp, False, pad_b, pad_r)
+        torch.cuda.synchronize()
+        n = lib.dcvc_trace_end(ptr, len(buf))
+        rec_enc = buf[:n].copy()
+        _lib.call("dcvc_trace_begin", 1)
+        dec.proxy.decompress(bits, qp, H, W, ec, False)
+        torch.cuda.synchronize()
+        n = lib.dcvc_trace_end(ptr, len(buf))
+        rec_dec = buf[:n].copy()
+    finally:
+        enc.proxy.set_use_graphs(True)
+        dec.proxy.set_use_graphs(True)
     return rec_enc, rec_dec
 
 
-def time_quad_kernel(records, dev, reps=5):
-    """Replays every quad-GEMM launch of one chunk at its real shape on the
-    current stream; returns (avg ms per launch, flops per launch)."""
-    import torch
-    from dcvc_amd import kernels
-    quad = records[(records[:, 0] == 0) & (records[:, 1] == 6)]
-    bufs = []
-    for M, N, K in quad[:, 2:5]:
-        a = torch.randn((int(M), int(K)), device=dev, dtype=torch.float16)
-        w = torch.randn((4 * int(N), int(K)), device=dev, dtype=torch.float16) / np.sqrt(K)
-        b = torch.zeros(4 * int(N), device=dev)
-        o = torch.empty((int(M), int(N)), device=dev, dtype=torch.float16)
-        bufs.append((a, w, b, o))
-    for a, w, b, o in bufs:
-        kernels.conv1x1(a, w, b, epi="quad", out=o)
-    torch.cuda.synchronize()
-    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    s.record()
-    for _ in range(reps):
-        for a, w, b, o in bufs:
-            kernels.conv1x1(a, w, b, epi="quad", out=o)
-    e.record()
-    torch.cuda.synchronize()
-    n = reps * len(bufs)
-    return s.elapsed_time(e) / n, float(quad[:, 5].sum()) / len(quad), len(quad)
+def kernel_classes(recs):
+    """Aggregate trace records per kernel class (kind, epilogue)."""
+    out = {}
+    for kind in np.unique(recs[:, 0]):
+        for epi in np.unique(recs[recs[:, 0] == kind][:, 1]):
+            r = recs[(recs[:, 0] == kind) & (recs[:, 1] == epi)]
+            name = KIND_NAMES[int(kind)] + ("" if kind == 3 else "_" + EPI_NAMES[int(epi)])
+            t = float(r[:, 7].sum()) * 1e-9
+            out[name] = {"launches": int(len(r)), "time_ms": round(t * 1e3, 4),
+                         "tflop": round(float(r[:, 5].sum()) / 1e12, 5), "gbyte": round(float(r[:, 6].sum()) / 1e9, 5),
+                         "tflops": round(float(r[:, 5].sum()) / t / 1e12, 2) if t > 0 else None,
+                         "gbs": round(float(r[:, 6].sum()) / t / 1e9, 1) if t > 0 else None,
+                         "flops_per_launch": int(r[:, 5].sum() // len(r)),
+                         "bytes_per_launch": int(r[:, 6].sum() // len(r)),
+                         "avg_launch_ms": round(t * 1e3 / len(r), 5)}
+    return dict(sorted(out.items(), key=lambda kv: -kv[1]["time_ms"]))
 
 
-def cpu_baseline(budget_s=12.0):
-    """FP32 CPU oracle: HT-S encode (+context update) and decode of a cropped
-    sample of the same synthetic chunk; frames/s scaled to 1088x1920."""
+def roofline_of(classes):
+    """Roofline object for the class with the largest total time: MFMA-bound
+    when its algorithmic intensity is above the FP16 ridge, HBM-bound below."""
+    name, c = next(iter(classes.items()))
+    ridge = PEAK_FP16_TFLOPS * 1e12 / (PEAK_HBM_GBS * 1e9)
+    intensity = c["flops_per_launch"] / max(c["bytes_per_launch"], 1)
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        if t.get("kernel_class") == name:
+            traffic = t.get("bytes_per_launch")
+    if intensity >= ridge and c["tflops"] is not None:
+        return {"bound": "mfma", "kernel_class": name, "achieved": c["tflops"], "peak": PEAK_FP16_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(c["tflops"] / PEAK_FP16_TFLOPS, 4), "traffic": traffic,
+                "avg_launch_ms": c["avg_launch_ms"], "launches_per_step": c["launches"],
+                "flops_per_launch": c["flops_per_launch"], "bytes_per_launch": c["bytes_per_launch"]}
+    return {"bound": "hbm", "kernel_class": name, "achieved": c["gbs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": round(c["gbs"] / PEAK_HBM_GBS, 4) if c["gbs"] else None, "traffic": traffic,
+            "avg_launch_ms": c["avg_launch_ms"], "launches_per_step": c["launches"],
+            "flops_per_launch": c["flops_per_launch"], "bytes_per_launch": c["bytes_per_launch"]}
+
+
+def cpu_baseline(budget_s=20.0):
+    """FP32 CPU oracle on this host: HT-S encode (+context update) and decode
+    of a cropped sample of the same synthetic chunk, frames/s scaled by area
+    to 1088x1920 (the headline), plus the 256x256 intra frame at all threads
+    and at 1 thread, and the C++ rANS coder in symbols/s."""
     import torch
-    from dcvc_amd import synthetic, weights
+    from dcvc_amd import entropy, synthetic, weights
     from oracle import dcvc_oracle as O
     threads = torch.get_num_threads()
     Hs, Ws = 256, 448
@@ -110,27 +139,72 @@ def cpu_baseline(budget_s=12.0):
             O.hts_decompress(sdp, st_d, r.z_int8, lambda idx: (sym[r.cond_dense] >> 8).astype(np.int8), 32, Hs, Ws,
                              False, skip_thres=0.15)
             times.append(time.perf_counter() - t0)
-            if time.perf_counter() - t_all > budget_s or len(times) >= 5:
+            if time.perf_counter() - t_all > budget_s / 2 or len(times) >= 5:
                 break
     t = float(np.median(times))
     scale = (1088 * 1920) / (Hs * Ws)
-    return {"value": round(4.0 / (t * scale), 4), "unit": "frames/s", "cores": threads, "kind": "port",
-            "sample": f"FP32 torch oracle, HT-S encode+decode of a {Hs}x{Ws} crop of the same 4-frame chunk "
-                      f"(padded to 8), {len(times)} run(s), median {t:.2f} s, scaled by area x{scale:.2f} "
-                      f"to 1088x1920"}
+    out = {"value": round(4.0 / (t * scale), 4), "unit": "frames/s", "cores": threads, "kind": "port",
+           "sample": f"FP32 torch oracle, HT-S encode+decode of a {Hs}x{Ws} crop of the same 4-frame chunk "
+                     f"(padded to 8), {len(times)} run(s), median {t:.2f} s, scaled by area x{scale:.2f} "
+                     f"to 1088x1920"}
+    # 256x256 intra frame (encode incl. reconstruction), all threads and 1 thread
+    sdi = {k: v.float() for k, v in weights.to_torch(weights.round_to_half(weights.init_state_dict("dmci", 0))).items()}
+    xi = torch.from_numpy(synthetic.frame_to_input(synthetic.texture_yuv420(256, 256, seed=0)))[None]
+    intra = {}
+    with torch.inference_mode():
+        for n_thr in (threads, 1):
+            torch.set_num_threads(n_thr)
+            O.dmci_compress(sdi, xi, 32)
+            t0 = time.perf_counter()
+            O.dmci_compress(sdi, xi, 32)
+            intra[f"threads_{n_thr}"] = round(1.0 / (time.perf_counter() - t0), 3)
+    torch.set_num_threads(threads)
+    out["intra_256x256_frames_per_s"] = intra
+    # C++ rANS coder: symbols/s at ec_parallel 1 and 8 on the Gaussian tables
+    g_cdf, g_len = entropy.gaussian_cdf_table()
+    rng = np.random.Generator(np.random.PCG64(0))
+    n = 2_000_000
+    idx = rng.integers(0, 128, n)
+    sym = np.clip(np.round(rng.standard_normal(n) * np.exp(np.linspace(np.log(0.11), np.log(16), 128))[idx] / 4),
+                  -8, 8).astype(np.int64)
+    sym16 = ((sym << 8) + idx).astype(np.int16)
+    rans = {}
+    for ec in (1, 8):
+        e = entropy.RansEncoder()
+        e.set_cdf(g_cdf, g_len, 1)
+        e.set_entropy_coder_parallel(ec)
+        e.reset()
+        t0 = time.perf_counter()
+        e.encode_y(sym16)
+        e.flush()
+        t_enc = time.perf_counter() - t0
+        stream = e.get_encoded_stream()
+        d = entropy.RansDecoder()
+        d.set_cdf(g_cdf, g_len, 1)
+        d.set_entropy_coder_parallel(ec)
+        d.set_stream(stream)
+        t0 = time.perf_counter()
+        d.decode_y(idx.astype(np.uint8))
+        d.get_decoded_tensor()
+        t_dec = time.perf_counter() - t0
+        rans[f"ec_parallel_{ec}"] = {"encode_msym_per_s": round(n / t_enc / 1e6, 2),
+                                     "decode_msym_per_s": round(n / t_dec / 1e6, 2)}
+    out["rans"] = rans
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--qp", type=int, default=32)
     ap.add_argument("--skip-thres", type=float, default=0.15)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--profile-step", action="store_true")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -173,6 +247,14 @@ def main():
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    if args.profile_step:
+        step()
+        a = torch.empty(64 << 20, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)  # 256 MiB read + 256 MiB write: counter calibration
+        torch.cuda.synchronize()
+        log("profile step done")
+        return
     log(f"timing {args.steps} steps")
     lat, enc_t, dec_t, nbytes = [], [], [], []
     t_start = time.perf_counter()
@@ -207,7 +289,7 @@ def main():
     }
 
     if not args.no_roofline:
-        rec_e, rec_d = layer_table(p_enc, p_dec, xc, args.qp, pad_b, pad_r, H, W)
+        rec_e, rec_d = layer_trace(p_enc, p_dec, xc, args.qp, pad_b, pad_r, H, W)
         recs = np.concatenate([rec_e, rec_d])
         gemm = recs[recs[:, 0] <= 2]
         dw = recs[recs[:, 0] == 3]
@@ -216,14 +298,11 @@ def main():
             "decode_gemm_tflop": round(float(rec_d[rec_d[:, 0] <= 2][:, 5].sum()) / 1e12, 4),
             "depthwise_gb": round(float(dw[:, 6].sum()) / 1e9, 4),
             "gemm_launches": int(len(gemm)), "depthwise_launches": int(len(dw)),
+            "traced_kernel_ms": round(float(recs[:, 7].sum()) * 1e-6, 3),
         }
-        avg_ms, flops, n_quad = time_quad_kernel(recs, dev)
-        achieved = flops / (avg_ms * 1e-3) / 1e12
-        out["roofline"] = {"bound": "mfma", "kernel": "gemm_kernel<EPI_QUAD> (ffn.0 quad GEMM)",
-                           "achieved": round(achieved, 2), "peak": PEAK_FP16_TFLOPS, "unit": "TFLOP/s",
-                           "frac": round(achieved / PEAK_FP16_TFLOPS, 4), "traffic": None,
-                           "avg_launch_ms": round(avg_ms, 5), "launches_per_step": n_quad,
-                           "flops_per_launch": int(flops)}
+        classes = kernel_classes(recs)
+        out["kernel_classes"] = classes
+        out["roofline"] = roofline_of(classes)
     if not args.no_cpu_baseline:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline()

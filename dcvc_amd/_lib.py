@@ -87,7 +87,7 @@ SIGNATURES = {
     "dcvc_hts_set_use_graphs": [P, I],
     "dcvc_hts_debug_copy": [P, I, P, L],
     "dcvc_hts_last_symbol_count": [P],
-    "dcvc_trace_begin": [],
+    "dcvc_trace_begin": [I],
     "dcvc_trace_end": [P, I],
 }
 _RESTYPE = {"dcvc_last_error": C.c_char_p, "dcvc_rans_encoder_create": C.c_void_p,

@@ -151,28 +151,30 @@ int dcvc_hts_debug_copy(void* h, int what, void* host, long bytes) {
 
 long dcvc_hts_last_symbol_count(void* h) { return h ? static_cast<HTSProxy*>(h)->last_symbol_count() : -1; }
 
-int dcvc_trace_begin(void) {
-    trace_set(true);
-    return DCVC_OK;
+int dcvc_trace_begin(int timed) {
+    return guarded([&] { trace_set(true, timed != 0); });
 }
 
 int dcvc_trace_end(long long* records, int max_records) {
-    trace_set(false);
-    auto& t = trace_buffer();
-    const int n = (int)t.size();
-    if (records)
-        for (int i = 0; i < n && i < max_records; ++i) {
-            const TraceRecord& r = t[i];
-            long long* o = records + 7LL * i;
-            o[0] = r.kind;
-            o[1] = r.epi;
-            o[2] = r.M;
-            o[3] = r.N;
-            o[4] = r.K;
-            o[5] = r.flops;
-            o[6] = r.bytes;
-        }
-    return n;
+    int n = -1;
+    const int st = guarded([&] {
+        const std::vector<TraceRecord> t = trace_collect();
+        n = (int)t.size();
+        if (records)
+            for (int i = 0; i < n && i < max_records; ++i) {
+                const TraceRecord& r = t[i];
+                long long* o = records + 8LL * i;
+                o[0] = r.kind;
+                o[1] = r.epi;
+                o[2] = r.M;
+                o[3] = r.N;
+                o[4] = r.K;
+                o[5] = r.flops;
+                o[6] = r.bytes;
+                o[7] = r.ns;
+            }
+    });
+    return st == DCVC_OK ? n : -st;
 }
 
 }  // extern "C"

@@ -41,9 +41,11 @@ DCVC_DEV float round_half_away(float x) {
 
 // x * sigmoid(4x) with a deterministic exp (layers.py:106-111).
 DCVC_DEV float wsilu(float x) {
-    // sigmoid(4x) = 1 / (1 + 2^(-4x*log2(e)))
+    // sigmoid(4x) = 1 / (1 + 2^(-4x*log2(e))); v_exp_f32 and v_rcp_f32 (1 ulp)
+    // instead of an IEEE divide, whose scale/fmas/fixup sequence costs ~4x
+    // the VALU time in the GEMM epilogues.  x * (1/inf) = 0 for large -x.
     const float e = __builtin_amdgcn_exp2f(-5.770780163555854f * x);
-    return x / (1.0f + e);
+    return x * __builtin_amdgcn_rcpf(1.0f + e);
 }
 
 // Micro-mask of the 4-step checkerboard (common_model.py:174-195,

@@ -288,6 +288,51 @@ __global__ __launch_bounds__(T::kThreads) void gemm_kernel(GemmArgs g, int tiles
     // tile kt - 2, whose fragment reads every wave consumed (so completed)
     // before reaching this iteration's barrier.
     constexpr int NDS = SM + SN, NMF = SM * SN;
+
+    // Epilogue operands that do not depend on the product (bias, shortcut
+    // tile) are loaded before the first DMA, so their latency hides under
+    // the K loop.  Epilogue chunk (pass i, t): lane finishes channels
+    // n .. n + 7 of pixel m.
+    constexpr int NG = T::kQuad ? SN / 4 : SN;
+    constexpr int EC = T::kEChunks / NT;
+    auto chunk_mn = [&](int i, int t, int& rl, int& cc, int& m, int& n) {
+        const int c = tid + t * NT;
+        rl = c / (T::BNO / 8);
+        cc = (c - rl * (T::BNO / 8)) * 8;
+        m = m0 + (rl >> 5) * T::TM + i * 32 + (rl & 31);
+        n = nt * T::BNO + cc;
+    };
+    float bias_r[SN];
+#pragma unroll
+    for (int s = 0; s < SN; ++s) {
+        if constexpr (T::kQuad) {
+            const int n = nt * T::BNO + wn * T::TNO + (s >> 2) * 32 + lr;
+            bias_r[s] = (n < g.N) ? g.bias[4 * n + (s & 3)] : 0.f;
+        } else {
+            const int n = nt * BN + wn * T::TN + s * 32 + lr;
+            bias_r[s] = (n < g.N) ? g.bias[n] : 0.f;
+        }
+    }
+    // shortcut passes held in registers through the K loop: all of them
+    // when they fit in 64 VGPRs, else only the first; each epilogue pass then
+    // loads the pass kS1Early ahead of it
+    constexpr bool kS1 = (EPI == EPI_SHORTCUT || EPI == EPI_SHORTCUT2 || EPI == EPI_SHORTCUT_QUANT);
+    constexpr int kS1Early = (SM * EC * 4 <= 64) ? SM : 1;
+    f16x8 s1r[kS1 ? SM : 1][kS1 ? EC : 1];
+    auto load_s1 = [&](int i0, int i1) {
+#pragma unroll
+        for (int i = i0; i < i1; ++i)
+#pragma unroll
+            for (int t = 0; t < EC; ++t) {
+                int rl, cc, m, n;
+                chunk_mn(i, t, rl, cc, m, n);
+                const f16x8 zero8 = {0, 0, 0, 0, 0, 0, 0, 0};
+                s1r[i][t] = (m < g.M && n < g.N) ? *reinterpret_cast<const f16x8*>(g.S1 + (size_t)m * g.lds1 + n)
+                                                  : zero8;
+            }
+    };
+    if constexpr (kS1) load_s1(0, kS1Early);
+
 #pragma unroll
     for (int st = 0; st < kAhead; ++st) dma_tile(st, st);
     wait_tile_and_barrier<(kAhead - 1) * T::kDma>();
@@ -313,18 +358,6 @@ __global__ __launch_bounds__(T::kThreads) void gemm_kernel(GemmArgs g, int tiles
     // ---- epilogue ----
     // bias (and activation) in registers; accumulator lane layout of a 32x32
     // tile: column lane & 31, row (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3).
-    constexpr int NG = T::kQuad ? SN / 4 : SN;
-    float bias_r[SN];
-#pragma unroll
-    for (int s = 0; s < SN; ++s) {
-        if constexpr (T::kQuad) {
-            const int n = nt * T::BNO + wn * T::TNO + (s >> 2) * 32 + lr;
-            bias_r[s] = (n < g.N) ? g.bias[4 * n + (s & 3)] : 0.f;
-        } else {
-            const int n = nt * BN + wn * T::TN + s * 32 + lr;
-            bias_r[s] = (n < g.N) ? g.bias[n] : 0.f;
-        }
-    }
     const f16* q = nullptr;
     if constexpr (EPI == EPI_SHORTCUT_QUANT || EPI == EPI_QUANT) {
         const int qp = g.qp ? *g.qp : 0;
@@ -336,6 +369,8 @@ __global__ __launch_bounds__(T::kThreads) void gemm_kernel(GemmArgs g, int tiles
 #pragma unroll
     for (int i = 0; i < SM; ++i) {
         if (i > 0) __syncthreads();  // previous pass fully read back
+        if constexpr (kS1 && kS1Early < SM)
+            if (i + kS1Early < SM) load_s1(i + kS1Early, i + kS1Early + 1);
         // -- stage rows wm * 32 + r of pass i (tile rows wm * TM + i * 32 + r) --
 #pragma unroll
         for (int gI = 0; gI < NG; ++gI) {
@@ -358,20 +393,16 @@ __global__ __launch_bounds__(T::kThreads) void gemm_kernel(GemmArgs g, int tiles
         __syncthreads();
         // -- finish 8 consecutive channels of one pixel per lane --
 #pragma unroll
-        for (int t = 0; t < T::kEChunks / NT; ++t) {
-            const int c = tid + t * NT;
-            const int rl = c / (T::BNO / 8);
-            const int cc = (c - rl * (T::BNO / 8)) * 8;
-            const int m = m0 + (rl >> 5) * T::TM + i * 32 + (rl & 31);
-            const int n = nt * T::BNO + cc;
+        for (int t = 0; t < EC; ++t) {
+            int rl, cc, m, n;
+            chunk_mn(i, t, rl, cc, m, n);
             if (m >= g.M || n >= g.N) continue;
             const f32x4 lo = *reinterpret_cast<const f32x4*>(E + rl * EP + cc);
             const f32x4 hi = *reinterpret_cast<const f32x4*>(E + rl * EP + cc + 4);
             float x[8] = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
-            if constexpr (EPI == EPI_SHORTCUT || EPI == EPI_SHORTCUT2 || EPI == EPI_SHORTCUT_QUANT) {
-                const f16x8 s = *reinterpret_cast<const f16x8*>(g.S1 + (size_t)m * g.lds1 + n);
+            if constexpr (kS1) {
 #pragma unroll
-                for (int e = 0; e < 8; ++e) x[e] += (float)s[e];
+                for (int e = 0; e < 8; ++e) x[e] += (float)s1r[i][t][e];
             }
             if constexpr (EPI == EPI_SHORTCUT2) {
                 const f16x8 s = *reinterpret_cast<const f16x8*>(g.S2 + (size_t)m * g.lds2 + n);

@@ -214,16 +214,20 @@ void set_device_int(int* dst, int v, hipStream_t s);
 
 // Launch trace: one record per GEMM / conv / depthwise launch issued by the
 // layer classes while tracing is on (used to build the per-chunk layer table).
+// A timed trace brackets each launch with HIP events on its stream.
 struct TraceRecord {
     int kind;   // 0 1x1 GEMM, 1 stride-2 conv, 2 sub-pixel conv, 3 depthwise 3x3
     int epi;    // GemmEpi for kinds 0-2
     int M, N, K;            // N = output channels (quad: 4N weight rows)
     long long flops;        // algorithmic multiply-adds x 2
     long long bytes;        // algorithmic HBM bytes: inputs + weights + outputs once
+    long long ns;           // launch duration (timed traces), else -1
 };
-void trace_record(const TraceRecord& r);
+void trace_record(const TraceRecord& r, hipStream_t s);
+void trace_launched(hipStream_t s);
 bool trace_enabled();
-void trace_set(bool on);
-std::vector<TraceRecord>& trace_buffer();
+void trace_set(bool on, bool timed);
+// Waits for the timed events, returns the records and ends the trace.
+std::vector<TraceRecord> trace_collect();
 
 }  // namespace dcvc

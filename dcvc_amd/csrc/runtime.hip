@@ -50,7 +50,7 @@ float* upload_f32(DevMem& mem, const std::vector<float>& v) {
 // weights are rounded to fp16 once, as the reference's model.half() does
 static inline float to_half_value(float x) { return (float)(f16)x; }
 
-static void trace_gemm(int kind, int epi, long M, int N_out, int K, bool quad, int n_shortcut);
+static void trace_gemm(int kind, int epi, long M, int N_out, int K, bool quad, int n_shortcut, hipStream_t s);
 
 // ---------------------------------------------------------------- Conv1x1
 void Conv1x1::load_raw(DevMem& mem, const std::vector<float>& wt, const std::vector<float>& bias, int N_, int K_) {
@@ -107,8 +107,9 @@ void Conv1x1::run(const Exec& e, const Act& a, const Act& dst, int epi, const Ac
     const int v = gemm_validate(g, epi, A_PLAIN, D_PLAIN);
     if (v != DCVC_OK) throw StatusError(v, "conv1x1: invalid GEMM arguments");
     trace_gemm(0, epi, g.M, N, K, epi == EPI_QUAD,
-               (epi == EPI_SHORTCUT || epi == EPI_SHORTCUT_QUANT) ? 1 : (epi == EPI_SHORTCUT2 ? 2 : 0));
+               (epi == EPI_SHORTCUT || epi == EPI_SHORTCUT_QUANT) ? 1 : (epi == EPI_SHORTCUT2 ? 2 : 0), e.stream);
     hip_check(gemm_launch(g, epi, A_PLAIN, D_PLAIN, e.stream), "gemm launch");
+    trace_launched(e.stream);
 }
 
 // ---------------------------------------------------------------- DW3x3
@@ -136,9 +137,10 @@ void DW3x3::run(const Exec& e, const Act& in, const Act& out) const {
     if (v != DCVC_OK) throw StatusError(v, "d3x3: invalid arguments");
     if (trace_enabled()) {
         const long long M = in.pixels();
-        trace_record(TraceRecord{3, 0, (int)M, C, 9, 18LL * M * C, 2LL * (2 * M * C + 9LL * C)});
+        trace_record(TraceRecord{3, 0, (int)M, C, 9, 18LL * M * C, 2LL * (2 * M * C + 9LL * C), -1}, e.stream);
     }
     hip_check(dw3x3_launch(a, e.stream), "d3x3 launch");
+    trace_launched(e.stream);
 }
 
 // ---------------------------------------------------------------- DCB
@@ -268,9 +270,11 @@ void ConvS2::run(const Exec& e, const Act& in, const Act& out) const {
         // input read once + weights + output
         const long long M = out.pixels();
         trace_record(TraceRecord{1, EPI_BIAS, (int)M, N, g.K, 2LL * M * N * g.K,
-                                 2LL * (in.pixels() * (long long)Cin + (long long)N * g.K + M * N)});
+                                 2LL * (in.pixels() * (long long)Cin + (long long)N * g.K + M * N), -1},
+                     e.stream);
     }
     hip_check(gemm_launch(g, EPI_BIAS, amode, D_PLAIN, e.stream), "strided conv launch");
+    trace_launched(e.stream);
 }
 
 // ---------------------------------------------------------------- Subpel
@@ -306,8 +310,9 @@ void Subpel::run(const Exec& e, const Act& in, const Act& out) const {
     const int v = gemm_validate(g, EPI_BIAS, A_PLAIN, D_SUBPEL);
     if (v != DCVC_OK || in.C != Cin || out.C != Cout || out.H != 2 * in.H || out.W != 2 * in.W)
         throw StatusError(DCVC_ERR_ARG, "subpel conv: invalid arguments");
-    trace_gemm(2, EPI_BIAS, g.M, 4 * Cout, Cin, false, 0);
+    trace_gemm(2, EPI_BIAS, g.M, 4 * Cout, Cin, false, 0, e.stream);
     hip_check(gemm_launch(g, EPI_BIAS, A_PLAIN, D_SUBPEL, e.stream), "subpel launch");
+    trace_launched(e.stream);
 }
 
 void RBS2::load(DevMem& mem, const HostDict& d, const std::string& p, bool shortcut) {
@@ -331,19 +336,68 @@ void RBU::forward(const Exec& e, const Act& in, const Act& out) const {
 }
 
 // ---------------------------------------------------------------- trace
-static std::vector<TraceRecord> g_trace;
-static bool g_trace_on = false;
+namespace {
+struct TraceEntry {
+    TraceRecord r;
+    hipEvent_t e0 = nullptr, e1 = nullptr;  // timed traces: around the launch, on its stream
+};
+std::vector<TraceEntry> g_trace;
+bool g_trace_on = false;
+bool g_trace_timed = false;
+}  // namespace
+
 bool trace_enabled() { return g_trace_on; }
-void trace_record(const TraceRecord& r) {
-    if (g_trace_on) g_trace.push_back(r);
+
+void trace_record(const TraceRecord& r, hipStream_t s) {
+    if (!g_trace_on) return;
+    TraceEntry t;
+    t.r = r;
+    if (g_trace_timed) {
+        hip_check(hipEventCreate(&t.e0), "trace event");
+        hip_check(hipEventCreate(&t.e1), "trace event");
+        hip_check(hipEventRecord(t.e0, s), "trace event");
+    }
+    g_trace.push_back(t);
 }
 
-static void trace_gemm(int kind, int epi, long M, int N_out, int K, bool quad, int n_shortcut) {
+void trace_launched(hipStream_t s) {
+    if (!g_trace_on || !g_trace_timed || g_trace.empty()) return;
+    hip_check(hipEventRecord(g_trace.back().e1, s), "trace event");
+}
+
+static void trace_gemm(int kind, int epi, long M, int N_out, int K, bool quad, int n_shortcut, hipStream_t s) {
     if (!g_trace_on) return;
     const long n_eff = quad ? 4L * N_out : N_out;
     TraceRecord r{kind, epi, (int)M, N_out, K, 2LL * M * n_eff * K,
-                  2LL * (M * (long long)K + n_eff * (long long)K + M * (long long)N_out * (1 + n_shortcut))};
-    g_trace.push_back(r);
+                  2LL * (M * (long long)K + n_eff * (long long)K + M * (long long)N_out * (1 + n_shortcut)), -1};
+    trace_record(r, s);
+}
+
+void trace_set(bool on, bool timed) {
+    for (auto& t : g_trace) {
+        if (t.e0) hipEventDestroy(t.e0);
+        if (t.e1) hipEventDestroy(t.e1);
+    }
+    g_trace.clear();
+    g_trace_on = on;
+    g_trace_timed = timed;
+}
+
+std::vector<TraceRecord> trace_collect() {
+    std::vector<TraceRecord> out;
+    out.reserve(g_trace.size());
+    for (auto& t : g_trace) {
+        TraceRecord r = t.r;
+        if (t.e0 && t.e1) {
+            float ms = 0.f;
+            hip_check(hipEventSynchronize(t.e1), "trace event");
+            hip_check(hipEventElapsedTime(&ms, t.e0, t.e1), "trace event");
+            r.ns = (long long)(ms * 1e6);
+        }
+        out.push_back(r);
+    }
+    trace_set(false, false);
+    return out;
 }
 
 // ---------------------------------------------------------------- graphs
@@ -361,10 +415,5 @@ void set_device_int(int* dst, int v, hipStream_t s) {
     hip_check(hipGetLastError(), "set_device_int");
 }
 
-std::vector<TraceRecord>& trace_buffer() { return g_trace; }
-void trace_set(bool on) {
-    g_trace_on = on;
-    if (on) g_trace.clear();
-}
 
 }  // namespace dcvc

@@ -114,11 +114,13 @@ int dcvc_dmci_debug_copy(void* h, int what, void* host, long bytes);
 long dcvc_hts_last_symbol_count(void* h);
 /* Layer-table trace (no reference counterpart): between begin and end, every
  * GEMM / conv / depthwise launch issued from the host is recorded; end
- * returns the record count and fills up to max_records rows of 7 int64:
- * kind (0 1x1, 1 stride-2 conv, 2 sub-pixel conv, 3 depthwise), epilogue,
- * M, N, K, algorithmic FLOPs, algorithmic bytes.  Graph replays launch
- * nothing from the host, so trace with graphs off. */
-int dcvc_trace_begin(void);
+ * returns the record count (or -status) and fills up to max_records rows of
+ * 8 int64: kind (0 1x1, 1 stride-2 conv, 2 sub-pixel conv, 3 depthwise),
+ * epilogue, M, N, K, algorithmic FLOPs, algorithmic bytes, launch duration
+ * in ns (timed != 0: HIP events recorded around each launch on its own
+ * stream; else -1).  Graph replays launch nothing from the host, so trace
+ * with graphs off. */
+int dcvc_trace_begin(int timed);
 int dcvc_trace_end(long long* records, int max_records);
 long dcvc_dmci_step_symbol_count(void* h, int step);
 
