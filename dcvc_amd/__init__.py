@@ -10,9 +10,11 @@ __version__ = "0.1.0"
 
 
 def install_module_alias():
-    """Make ``import inference_extensions_cuda`` resolve to this package's
-    drop-in module, the name the reference model code imports
-    (image_model.py:197, video_model_ht.py:419-424)."""
-    from . import inference_extensions_cuda as m
+    """Make ``import inference_extensions_cuda`` and ``import
+    MLCodec_extensions_cpp`` resolve to this package's drop-in modules, the
+    names the reference model code imports (image_model.py:197,
+    video_model_ht.py:419-424, entropy_models.py:34-40)."""
+    from . import entropy, inference_extensions_cuda as m
     sys.modules.setdefault("inference_extensions_cuda", m)
+    sys.modules.setdefault("MLCodec_extensions_cpp", entropy)
     return m

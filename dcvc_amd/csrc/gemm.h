@@ -51,7 +51,7 @@ struct GemmArgs {
 
 int gemm_validate(const GemmArgs& g, int epi, int amode, int dmode);
 hipError_t gemm_launch(const GemmArgs& g, int epi, int amode, int dmode, hipStream_t stream);
-// 0: automatic tile choice; 1-3: force one tile configuration (tests, tuning)
+// 0: automatic tile choice; 1-5: force one tile configuration (tests, tuning)
 int gemm_set_tile_policy(int policy);
 
 }  // namespace dcvc

@@ -50,8 +50,6 @@ namespace {
 
 constexpr int kBK = 32;                 // K tile (halves): one 64-byte LDS row
 constexpr int kRowBytes = kBK * 2;
-constexpr int kStages = 5;              // LDS ring depth
-constexpr int kAhead = 3;               // K tiles in flight ahead of the one being read
 constexpr uint32_t kOOB = 0x80000000u;  // buffer offset beyond every range: reads zero
 constexpr int kRsrcWord3 = 0x00020000;  // gfx9 raw buffer: 32-bit data format
 
@@ -70,14 +68,14 @@ DCVC_DEV void dma16(__amdgpu_buffer_rsrc_t r, char* dst, uint32_t off) {
 // XOR swizzle of the 16-byte K chunk within a 64-byte LDS row.
 DCVC_DEV int swz(int row) { return ((row >> 2) ^ (row >> 1)) & 3; }
 
-// s_waitcnt vmcnt(VM) (gfx9 encoding; expcnt / lgkmcnt not waited for), then
-// a workgroup barrier.  The compiler fences keep LDS accesses on their side of
-// the barrier; no memory-model fence is used because it would also wait for
-// the K tiles still in flight.
-template <int VM>
+// s_waitcnt vmcnt(VM) [lgkmcnt(0) if LGKM] (gfx9 encoding), then a workgroup
+// barrier.  The compiler fences keep LDS accesses on their side of the
+// barrier; no memory-model fence is used because it would also wait for the
+// K tiles still in flight.
+template <int VM, bool LGKM>
 DCVC_DEV void wait_tile_and_barrier() {
     static_assert(VM >= 0 && VM < 64, "vmcnt range");
-    constexpr int imm = (VM & 15) | ((VM >> 4) << 14) | (7 << 4) | (15 << 8);
+    constexpr int imm = (VM & 15) | ((VM >> 4) << 14) | (7 << 4) | ((LGKM ? 0 : 15) << 8);
     asm volatile("" ::: "memory");
     __builtin_amdgcn_s_waitcnt(imm);
     __builtin_amdgcn_s_barrier();
@@ -103,9 +101,20 @@ int g_tile_policy = 0;  // 0 automatic, otherwise a forced configuration id
 
 }  // namespace
 
-template <int BM_, int BN_, int WM_, int WN_, int EPI_, int AMODE_, int DMODE_>
+template <int BM_, int BN_, int WM_, int WN_, int STAGES_, int EPI_, int AMODE_, int DMODE_>
 struct Tile {
     static constexpr int BM = BM_, BN = BN_, WM = WM_, WN = WN_;
+    // LDS ring of STAGES K tiles, STAGES - 2 in flight ahead of the one being
+    // read: the slot refilled at iteration kt is that of tile kt - 2, whose
+    // fragment reads every wave consumed before this iteration's barrier.
+    // STAGES == 3 keeps one tile ahead and refills the slot of tile kt - 1,
+    // so its barrier also waits for the wave's own fragment reads (lgkmcnt).
+    static constexpr int kStages = STAGES_;
+    static constexpr int kAhead = STAGES_ >= 5 ? STAGES_ - 2 : STAGES_ - 1;
+    static constexpr bool kWaitLds = (kAhead == STAGES_ - 1);
+    // configurations meant to run 2 blocks per CU keep within 256 registers
+    // per lane (architectural + accumulation) so two waves share each SIMD
+    static constexpr int kWavesPerSimd = (STAGES_ == 3 || (BM_ * BN_ <= 128 * 128)) ? 2 : 1;
     static constexpr int EPI = EPI_, AMODE = AMODE_, DMODE = DMODE_;
     static constexpr int kWaves = WM * WN;
     static constexpr int kThreads = kWaves * 64;
@@ -131,11 +140,12 @@ struct Tile {
     static_assert(ARW % 16 == 0 && BRW % 16 == 0, "DMA split");
     static_assert(kEChunks % kThreads == 0, "epilogue split");
     static_assert(kAhead * kDma < 64, "vmcnt range");
+    static_assert(kStages >= 3 && kAhead >= 2, "ring depth");
     static_assert(kLdsBytes <= 160 * 1024, "LDS");
 };
 
 template <class T>
-__global__ __launch_bounds__(T::kThreads) void gemm_kernel(GemmArgs g, int tiles_n) {
+__global__ __launch_bounds__(T::kThreads, T::kWavesPerSimd) void gemm_kernel(GemmArgs g, int tiles_n) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     constexpr int BM = T::BM, BN = T::BN, SM = T::SM, SN = T::SN, NT = T::kThreads;
     constexpr int AMODE = T::AMODE, EPI = T::EPI;
@@ -283,10 +293,8 @@ __global__ __launch_bounds__(T::kThreads) void gemm_kernel(GemmArgs g, int tiles
                 acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fa[i], fb[j], acc[i][j], 0, 0, 0);
     };
 
-    // K loop.  Tile kt lives in ring slot kt % 5; at iteration kt the DMAs of
-    // tiles kt + 1 .. kt + 3 are in flight and tile kt + 3 goes to the slot of
-    // tile kt - 2, whose fragment reads every wave consumed (so completed)
-    // before reaching this iteration's barrier.
+    // K loop.  Tile kt lives in ring slot kt % kStages; at iteration kt the
+    // DMAs of tiles kt + 1 .. kt + kAhead are in flight (see Tile).
     constexpr int NDS = SM + SN, NMF = SM * SN;
 
     // Epilogue operands that do not depend on the product (bias, shortcut
@@ -317,7 +325,7 @@ __global__ __launch_bounds__(T::kThreads) void gemm_kernel(GemmArgs g, int tiles
     // when they fit in 64 VGPRs, else only the first; each epilogue pass then
     // loads the pass kS1Early ahead of it
     constexpr bool kS1 = (EPI == EPI_SHORTCUT || EPI == EPI_SHORTCUT2 || EPI == EPI_SHORTCUT_QUANT);
-    constexpr int kS1Early = (SM * EC * 4 <= 64) ? SM : 1;
+    constexpr int kS1Early = (SM * EC * 4 <= 64 && T::kWavesPerSimd == 1) ? SM : 1;
     f16x8 s1r[kS1 ? SM : 1][kS1 ? EC : 1];
     auto load_s1 = [&](int i0, int i1) {
 #pragma unroll
@@ -334,17 +342,17 @@ __global__ __launch_bounds__(T::kThreads) void gemm_kernel(GemmArgs g, int tiles
     if constexpr (kS1) load_s1(0, kS1Early);
 
 #pragma unroll
-    for (int st = 0; st < kAhead; ++st) dma_tile(st, st);
-    wait_tile_and_barrier<(kAhead - 1) * T::kDma>();
-    dma_tile(kAhead, kAhead);
+    for (int st = 0; st < T::kAhead; ++st) dma_tile(st, st);
+    wait_tile_and_barrier<(T::kAhead - 1) * T::kDma, T::kWaitLds>();
+    dma_tile(T::kAhead, T::kAhead);
     read_frags(0, pos0, f0a, f0b);
     read_frags(0, pos1, f1a, f1b);
     mfma_all(f0a, f0b);
     for (int kt = 1; kt < KT; ++kt) {
         // this wave's DMAs of tile kt have landed; after the barrier, every wave's
-        wait_tile_and_barrier<(kAhead - 1) * T::kDma>();
-        const int st = kt % kStages;
-        dma_tile(kt + kAhead, (kt + kAhead) % kStages);
+        wait_tile_and_barrier<(T::kAhead - 1) * T::kDma, T::kWaitLds>();
+        const int st = kt % T::kStages;
+        dma_tile(kt + T::kAhead, (kt + T::kAhead) % T::kStages);
         read_frags(st, pos0, f0a, f0b);
         mfma_all(f1a, f1b);  // second k16 step of tile kt - 1
         interleave<NDS, NMF, T::kDma>();
@@ -440,13 +448,18 @@ __global__ __launch_bounds__(T::kThreads) void gemm_kernel(GemmArgs g, int tiles
 // ---------------------------------------------------------------------------
 
 // Tile configurations (ids for dcvc_gemm_set_tile_policy):
-//   1: 256 x 256, wave 128 x 128     2: 256 x 128, wave 128 x 64
-//   3: 128 x 128, wave 64 x 64
-//   quad (B rows): 1: 256 x 256 (64 outputs), 2: 128 x 256 (64 outputs),
-//   3: 128 x 128, waves 4 x 1 (32 outputs)
-template <int BM, int BN, int WM, int WN, int EPI, int AMODE, int DMODE>
+//   1: 256 x 256, wave 128 x 128, 5 stages (1 block / CU)
+//   2: 256 x 128, wave 128 x 64, 5 stages (1 block / CU)
+//   3: 128 x 128, wave 64 x 64, 5 stages (2 blocks / CU)
+//   4: 256 x 128, wave 128 x 64, 3 stages (2 blocks / CU)
+//   5: 128 x 256, wave 64 x 128, 3 stages (2 blocks / CU)
+//   quad (B rows; 4 B rows per output): 1: 256 x 256 5 st, 2: 128 x 256 5 st,
+//   3: 128 x 128 waves 4 x 1, 4: 128 x 256 3 st, 5: 256 x 256 4 st
+constexpr int kPolicies = 5;
+
+template <int BM, int BN, int WM, int WN, int ST, int EPI, int AMODE, int DMODE>
 static hipError_t launch_tile(GemmArgs g, hipStream_t stream) {
-    using T = Tile<BM, BN, WM, WN, EPI, AMODE, DMODE>;
+    using T = Tile<BM, BN, WM, WN, ST, EPI, AMODE, DMODE>;
     const int tiles_m = ceil_div(g.M, BM);
     const int tiles_n = ceil_div(g.N, T::BNO);
     auto kern = gemm_kernel<T>;
@@ -461,19 +474,17 @@ static hipError_t launch_tile(GemmArgs g, hipStream_t stream) {
     return hipGetLastError();
 }
 
-// Automatic choice: the largest tile that still gives about one block per CU
-// (256 CUs); smaller tiles for the small 1/16- and 1/64-resolution layers.
+// Automatic choice, from a sweep of the 1080p HT-S layer shapes on MI355X:
+// large-M layers run 2 blocks per CU (one block's epilogue overlaps the
+// other's K loop), except long-K layers (K >= 1024), which keep the 256 x 256
+// tile; the 1/16- and 1/64-resolution layers take 128 x 128 tiles so the grid
+// still covers the 256 CUs.
 static int pick_tile(const GemmArgs& g, bool quad) {
-    if (g_tile_policy >= 1 && g_tile_policy <= 3) return g_tile_policy;
+    if (g_tile_policy >= 1 && g_tile_policy <= kPolicies) return g_tile_policy;
     const long long m256 = ceil_div(g.M, 256), m128 = ceil_div(g.M, 128);
-    if (quad) {
-        const long long n64 = ceil_div(g.N, 64);
-        if (m256 * n64 >= 240) return 1;
-        if (m128 * n64 >= 240) return 2;
-        return 3;
-    }
-    if (m256 * ceil_div(g.N, 256) >= 240) return 1;
-    if (m256 * ceil_div(g.N, 128) >= 240) return 2;
+    if (quad) return (m128 * ceil_div(g.N, 64) >= 480) ? 4 : 3;
+    if (g.K >= 1024 && m256 * ceil_div(g.N, 256) >= 240) return 1;
+    if (m256 * ceil_div(g.N, 128) >= 480) return 4;
     return 3;
 }
 
@@ -481,13 +492,21 @@ template <int EPI, int AMODE, int DMODE>
 static hipError_t dispatch_tiles(const GemmArgs& g, hipStream_t stream) {
     const int t = pick_tile(g, EPI == EPI_QUAD);
     if constexpr (EPI == EPI_QUAD) {
-        if (t == 1) return launch_tile<256, 256, 2, 2, EPI, AMODE, DMODE>(g, stream);
-        if (t == 2) return launch_tile<128, 256, 2, 2, EPI, AMODE, DMODE>(g, stream);
-        return launch_tile<128, 128, 4, 1, EPI, AMODE, DMODE>(g, stream);
+        switch (t) {
+        case 1: return launch_tile<256, 256, 2, 2, 5, EPI, AMODE, DMODE>(g, stream);
+        case 2: return launch_tile<128, 256, 2, 2, 5, EPI, AMODE, DMODE>(g, stream);
+        case 4: return launch_tile<128, 256, 2, 2, 3, EPI, AMODE, DMODE>(g, stream);
+        case 5: return launch_tile<256, 256, 2, 2, 4, EPI, AMODE, DMODE>(g, stream);
+        default: return launch_tile<128, 128, 4, 1, 5, EPI, AMODE, DMODE>(g, stream);
+        }
     } else {
-        if (t == 1) return launch_tile<256, 256, 2, 2, EPI, AMODE, DMODE>(g, stream);
-        if (t == 2) return launch_tile<256, 128, 2, 2, EPI, AMODE, DMODE>(g, stream);
-        return launch_tile<128, 128, 2, 2, EPI, AMODE, DMODE>(g, stream);
+        switch (t) {
+        case 1: return launch_tile<256, 256, 2, 2, 5, EPI, AMODE, DMODE>(g, stream);
+        case 2: return launch_tile<256, 128, 2, 2, 5, EPI, AMODE, DMODE>(g, stream);
+        case 4: return launch_tile<256, 128, 2, 2, 3, EPI, AMODE, DMODE>(g, stream);
+        case 5: return launch_tile<128, 256, 2, 2, 3, EPI, AMODE, DMODE>(g, stream);
+        default: return launch_tile<128, 128, 2, 2, 5, EPI, AMODE, DMODE>(g, stream);
+        }
     }
 }
 
@@ -505,16 +524,20 @@ hipError_t gemm_launch(const GemmArgs& g_in, int epi, int amode, int dmode, hipS
     }
     g.a_bytes = (uint32_t)(((a_rows - 1) * g.lda + a_last) * 2);
     g.b_bytes = (uint32_t)((size_t)(epi == EPI_QUAD ? 4 * g.N : g.N) * g.K * 2);
-#define DCVC_EPI_CASE(E)                                                                        \
-    case E:                                                                                     \
-        if (dmode == D_SUBPEL) return dispatch_tiles<E, A_PLAIN, D_SUBPEL>(g, stream);          \
-        switch (amode) {                                                                        \
-        case A_PLAIN: return dispatch_tiles<E, A_PLAIN, D_PLAIN>(g, stream);                    \
-        case A_S2K2: return dispatch_tiles<E, A_S2K2, D_PLAIN>(g, stream);                      \
-        case A_S2K3: return dispatch_tiles<E, A_S2K3, D_PLAIN>(g, stream);                      \
-        case A_S1K3: return dispatch_tiles<E, A_S1K3, D_PLAIN>(g, stream);                      \
-        default: return hipErrorInvalidValue;                                                   \
+    // implicit-conv A modes and the sub-pixel store come with the plain bias
+    // epilogue only (conv_bias, transposed_conv)
+    if (amode != A_PLAIN || dmode != D_PLAIN) {
+        if (epi != EPI_BIAS) return hipErrorInvalidValue;
+        if (dmode == D_SUBPEL) return dispatch_tiles<EPI_BIAS, A_PLAIN, D_SUBPEL>(g, stream);
+        switch (amode) {
+        case A_S2K2: return dispatch_tiles<EPI_BIAS, A_S2K2, D_PLAIN>(g, stream);
+        case A_S2K3: return dispatch_tiles<EPI_BIAS, A_S2K3, D_PLAIN>(g, stream);
+        case A_S1K3: return dispatch_tiles<EPI_BIAS, A_S1K3, D_PLAIN>(g, stream);
+        default: return hipErrorInvalidValue;
         }
+    }
+#define DCVC_EPI_CASE(E) \
+    case E: return dispatch_tiles<E, A_PLAIN, D_PLAIN>(g, stream);
     switch (epi) {
         DCVC_EPI_CASE(EPI_BIAS)
         DCVC_EPI_CASE(EPI_WSILU)
@@ -522,9 +545,7 @@ hipError_t gemm_launch(const GemmArgs& g_in, int epi, int amode, int dmode, hipS
         DCVC_EPI_CASE(EPI_SHORTCUT2)
         DCVC_EPI_CASE(EPI_SHORTCUT_QUANT)
         DCVC_EPI_CASE(EPI_QUANT)
-    case EPI_QUAD:
-        if (amode != A_PLAIN || dmode != D_PLAIN) return hipErrorInvalidValue;
-        return dispatch_tiles<EPI_QUAD, A_PLAIN, D_PLAIN>(g, stream);
+        DCVC_EPI_CASE(EPI_QUAD)
     default:
         return hipErrorInvalidValue;
     }
@@ -536,6 +557,7 @@ static bool aligned16(const void* p) { return ((uintptr_t)p) % 16 == 0; }
 // Host-side validation shared by every GEMM entry point.
 int gemm_validate(const GemmArgs& g, int epi, int amode, int dmode) {
     if (!g.A || !g.B || !g.D || !g.bias) return DCVC_ERR_ARG;
+    if ((amode != A_PLAIN || dmode != D_PLAIN) && epi != EPI_BIAS) return DCVC_ERR_ARG;
     if (g.M <= 0 || g.N <= 0 || g.K <= 0) return DCVC_ERR_ARG;
     if (g.K % 64 != 0 || g.N % 8 != 0) return DCVC_ERR_ARG;
     if (g.lda % 8 != 0 || g.lda < (amode == A_PLAIN ? g.K : g.cin)) return DCVC_ERR_ARG;
@@ -569,7 +591,7 @@ int gemm_validate(const GemmArgs& g, int epi, int amode, int dmode) {
 }
 
 int gemm_set_tile_policy(int policy) {
-    if (policy < 0 || policy > 3) return DCVC_ERR_ARG;
+    if (policy < 0 || policy > kPolicies) return DCVC_ERR_ARG;
     g_tile_policy = policy;
     return DCVC_OK;
 }

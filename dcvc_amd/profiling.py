@@ -18,7 +18,8 @@ import sys
 from collections import defaultdict
 
 EPI_NAMES = {0: "bias", 1: "wsilu", 2: "shortcut", 3: "shortcut2", 4: "shortcut_quant", 5: "quant", 6: "quad"}
-_TILE = re.compile(r"Tile<(\d+), (\d+), (\d+), (\d+), (\d+), (\d+), (\d+)>")
+# Tile<BM, BN, WM, WN, STAGES, EPI, AMODE, DMODE>
+_TILE = re.compile(r"Tile<(\d+), (\d+), (\d+), (\d+), (\d+), (\d+), (\d+), (\d+)>")
 
 
 def gemm_class(kernel_name: str) -> str | None:
@@ -26,7 +27,7 @@ def gemm_class(kernel_name: str) -> str | None:
     m = _TILE.search(kernel_name)
     if not m:
         return None
-    epi, amode, dmode = int(m.group(5)), int(m.group(6)), int(m.group(7))
+    epi, amode, dmode = int(m.group(6)), int(m.group(7)), int(m.group(8))
     kind = "conv_s2" if amode in (1, 2) else ("subpel" if dmode == 1 else "conv1x1")
     return f"{kind}_{EPI_NAMES[epi]}"
 
@@ -64,19 +65,21 @@ def _per_dispatch(pmc_csv: str, counter: str):
 
 def traffic_per_launch(fetch_csv: str, write_csv: str, kernel_class: str, copy_bytes: int) -> dict:
     """FETCH_SIZE / WRITE_SIZE (KiB per dispatch) averaged over the dispatches of
-    `kernel_class`, scaled so the largest copy kernel's readings equal
-    copy_bytes read and copy_bytes written."""
+    `kernel_class`, scaled so the calibration copy (the program's last
+    dispatch, a device copy of copy_bytes) reads copy_bytes and writes
+    copy_bytes."""
     out = {"kernel_class": kernel_class}
     for counter, path, key in (("FETCH_SIZE", fetch_csv, "fetch"), ("WRITE_SIZE", write_csv, "write")):
         vals, names = _per_dispatch(path, counter)
-        copies = [v for d, v in vals.items() if "copy" in names[d].lower() or "elementwise" in names[d].lower()]
-        ref = max(copies) * 1024.0 if copies else float("nan")
+        last = max(vals)
+        ref = vals[last] * 1024.0
         factor = copy_bytes / ref
         mine = [v for d, v in vals.items() if gemm_class(names[d]) == kernel_class]
         out[f"{key}_kib_raw_avg"] = sum(mine) / len(mine) if mine else None
         out[f"{key}_scale"] = factor
         out[f"{key}_bytes_per_launch"] = (sum(mine) / len(mine)) * 1024.0 * factor if mine else None
         out["dispatches"] = len(mine)
+        out["calibration_kernel"] = names[last]
     if out["fetch_bytes_per_launch"] is not None and out["write_bytes_per_launch"] is not None:
         out["bytes_per_launch"] = int(out["fetch_bytes_per_launch"] + out["write_bytes_per_launch"])
     return out

@@ -30,7 +30,7 @@ int dcvc_version(void);
  * (def_const.h:6-11, elementwise/stream.cu:77-87).  Host-only. */
 int dcvc_scale_index_thresholds(float* th128);
 
-/* GEMM tile configuration: 0 automatic (default), 1-3 force one of the
+/* GEMM tile configuration: 0 automatic (default), 1-5 force one of the
  * three tile shapes for every GEMM launched afterwards (process-wide).
  * No counterpart in the reference: used by tests (all shapes must give
  * bit-identical results) and by tile tuning.  Host-only. */

@@ -166,7 +166,7 @@ def test_gemm_tile_configs_bit_identical(gpu, epi):
 
     outs = []
     try:
-        for policy in (1, 2, 3):
+        for policy in (1, 2, 3, 4, 5):
             assert lib.dcvc_gemm_set_tile_policy(policy) == 0
             r0 = rng(5)
             outs.append(run())
