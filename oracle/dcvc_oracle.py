@@ -18,8 +18,33 @@ forward (``elementwise/stream.cu``):
   evaluated in FP32 semantics through a 126-entry threshold table computed in
   float64 (``scale_index_thresholds``), so host and device agree bit-for-bit.
 
-Parity status of each function is stated in its docstring ("pinned by": the
-check in tests/ that would fail on a plausible mistake).
+What pins each function (the check in tests/ that fails on a plausible
+mistake; none of them retypes the oracle's formula):
+
+* scale_index_thresholds / scale_to_index: float64 formula on every finite
+  fp16 scale, random float32 and edges, the entropy model's scale table, and
+  the C++ table (test_oracle_boundary.py); special cases incl. the top-end
+  126 of the truncated constants (test_oracle_pins.py).
+* round_half_away, round_z, quantize_masked, build_index: hand-derived
+  special cases of the stream.cu semantics (ties, clamps, skip threshold,
+  NHWC order, symbol packing) (test_oracle_boundary.py, test_oracle_pins.py).
+* compact / recover: stability and inverse invariants incl. empty and full
+  inputs (test_oracle_pins.py).  compute_ec_parallel: closed form.
+* mask_4x: the reference's get_mask_4x output (golden vectors,
+  test_oracle_golden.py) and the partition / permutation pattern.
+* conv / wsilu / dcb / rbs2 / subpel / rbu and every DMCI / HT-S submodule
+  (encoder, hyper codec, priors, spatial priors, decoder, recon head,
+  feature adaptors / extractor): the reference modules' own forward on the
+  same seeded weights (golden vectors, test_oracle_golden.py).
+* dmci_compress / hts_compress / hts_decompress (step order, y_hat
+  accumulation, context update): composed of the pinned parts; the HT-S
+  decoder rebuilds the encoder's y_hat and reconstruction exactly from its
+  symbols (test_oracle_pins.py).  The DMCI composition is pinned only
+  through its parts and the GPU encoder/decoder agreement.
+* hts/intra *_f32 step restatements: agree with the torch masked steps on
+  symbols, conditions and y_hat (test_oracle_pins.py, HT-S form).
+* ref_gemm_epilogue / ref_dw3x3: torch float64 conv2d / sigmoid forms
+  (test_oracle_pins.py).
 """
 from __future__ import annotations
 
@@ -200,7 +225,23 @@ def mask_4x(C, H, W):
 # layers (layers.py), FP32 on NCHW tensors
 # ---------------------------------------------------------------------------
 
+def _nchw(x):
+    """x with standard NCHW strides.  Tensors with size-1 dims count as
+    contiguous under several stride patterns, and the CPU convolution picks
+    its summation order from the strides; normalising them makes the result
+    depend on the values only (the encoder and decoder sides then agree bit
+    for bit)."""
+    std, acc = [], 1
+    for d in reversed(x.shape):
+        std.append(acc)
+        acc *= d
+    if tuple(reversed(std)) == tuple(x.stride()):
+        return x
+    return x.clone(memory_format=torch.contiguous_format).as_strided(x.shape, tuple(reversed(std)))
+
+
 def conv(sd, prefix, x, stride=1, padding=0, groups=1):
+    x = _nchw(x)
     w = sd[prefix + "weight"]
     b = sd.get(prefix + "bias")
     return F.conv2d(x, w.to(x.dtype), None if b is None else b.to(x.dtype),
@@ -555,8 +596,10 @@ def hts_decompress(sd, st: HTSState, z_int8, decode_y, qp, height, width, reset_
     _hts_apply_adaptor(sd, st)
     H16, W16 = (height + 15) // 16, (width + 15) // 16
     H64, W64 = (H16 + 3) // 4, (W16 + 3) // 4
+    # contiguous NCHW, as on the encoder side: a channels-last view would send
+    # the CPU convolutions down another summation order
     z_hat = torch.from_numpy(z_int8.astype(np.float32)).reshape(1, H64, W64, -1).permute(0, 3, 1, 2)
-    z_hat = z_hat.to(st.memory.dtype)
+    z_hat = z_hat.to(st.memory.dtype).contiguous()
     params = _hts_prior(sd, None, z_hat, st.memory, qp)
     q_dec, scales, means = params.chunk(3, 1)
     q_dec = torch.clamp_min(q_dec, 0.5)
@@ -564,7 +607,8 @@ def hts_decompress(sd, st: HTSState, z_int8, decode_y, qp, height, width, reset_
     decoded = decode_y(compact(idx, cond))
     C = scales.shape[1]
     y_q = recover(decoded, cond, idx.size)
-    y_q = torch.from_numpy(y_q.astype(np.float32)).reshape(1, H16, W16, C).permute(0, 3, 1, 2).to(params.dtype)
+    y_q = torch.from_numpy(y_q.astype(np.float32)).reshape(1, H16, W16, C).permute(0, 3, 1, 2)
+    y_q = y_q.to(params.dtype).contiguous()
     reduced = conv(sd, "y_spatial_prior_reduction.", params)
     masks = mask_4x(C, H16, W16)
     y_hat = torch.zeros_like(y_q)
