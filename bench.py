@@ -114,6 +114,87 @@ def roofline_of(classes):
             "flops_per_launch": c["flops_per_launch"], "bytes_per_launch": c["bytes_per_launch"]}
 
 
+def precision_forms(sdp, p_enc_ref, x_hat_i, chunks, H, W, pad_b, pad_r, skip, dev, steps=30):
+    """BF16 / FP8 1x1-conv products against the FP16 path: frames/s at qp 32,
+    then per qp in {0, 21, 42, 63} the 8-chunk stream coded in each precision:
+    symbol-match rate of the encoder's dense symbols against FP16, PSNR of the
+    4 real frames of each chunk, bpp; and per-layer error of representative
+    layers against float64 (real weights, N(0,1) activations)."""
+    import torch
+    from dcvc_amd import codec, kernels
+    from oracle import dcvc_oracle as O
+    out = {"layers": {}, "fps": {}, "per_qp": {}}
+    n_chunks = len(chunks)
+    P16 = ((H + 15) // 16) * ((W + 15) // 16)
+
+    def make(prec):
+        e = codec.VideoCodecHTS(share_from=p_enc_ref)
+        d = codec.VideoCodecHTS(share_from=p_enc_ref)
+        e.proxy.set_precision(prec)
+        d.proxy.set_precision(prec)
+        e.add_ref_feature_from_frame(x_hat_i)
+        d.add_ref_feature_from_frame(x_hat_i, apply_feature_adaptor=False)
+        return e, d
+
+    for prec in ("fp16", "bf16", "fp8"):
+        e, d = make(prec)
+        for i in range(3):
+            bits, ec = e.proxy.compress(chunks[i % n_chunks], 32, False, pad_b, pad_r)
+            d.proxy.decompress(bits, 32, H, W, ec, False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            bits, ec = e.proxy.compress(chunks[i % n_chunks], 32, False, pad_b, pad_r)
+            d.proxy.decompress(bits, 32, H, W, ec, False)
+        torch.cuda.synchronize()
+        out["fps"][prec] = round(4.0 * steps / (time.perf_counter() - t0), 2)
+    for qp in (0, 21, 42, 63):
+        res = {}
+        ref_syms = None
+        for prec in ("fp16", "bf16", "fp8"):
+            e, d = make(prec)
+            nbytes, psnrs, syms = 0, [], []
+            for c in range(n_chunks):
+                bits, ec = e.proxy.compress(chunks[c], qp, False, pad_b, pad_r)
+                syms.append(e.proxy.debug_copy(5, P16 * 256 * 2, np.int16))
+                rec = d.proxy.decompress(bits, qp, H, W, ec, False)
+                torch.cuda.synchronize()
+                nbytes += len(bits)
+                src = chunks[c].float()[:, :, :H, :W]
+                for f in range(4):
+                    x = rec[f].float()[:, :, :H, :W]
+                    mse = float(((x - src[:, 3 * f:3 * f + 3]) ** 2).mean())
+                    psnrs.append(10 * np.log10(1.0 / max(mse, 1e-12)))
+            if prec == "fp16":
+                ref_syms = syms
+            match = float(np.mean([np.mean(a == b) for a, b in zip(syms, ref_syms)]))
+            res[prec] = {"bpp": round(8.0 * nbytes / (4 * n_chunks * H * W), 5),
+                         "psnr_db": round(float(np.mean(psnrs)), 4), "symbol_match_vs_fp16": round(match, 5)}
+        for prec in ("bf16", "fp8"):
+            res[prec]["psnr_delta_db"] = round(res[prec]["psnr_db"] - res["fp16"]["psnr_db"], 4)
+            res[prec]["bpp_delta_pct"] = round(100 * (res[prec]["bpp"] / res["fp16"]["bpp"] - 1), 3)
+        out["per_qp"][str(qp)] = res
+    # per-layer error against float64 on real weights
+    r = np.random.Generator(np.random.PCG64(0))
+    M = 8160
+    for name, epi in (("encoder.conv1.1.dc.0", "wsilu"), ("encoder.conv1.1.ffn.0", "quad"),
+                      ("encoder.conv1.1.ffn.2", "bias"), ("encoder.conv1.0.adaptor", "bias")):
+        w = sdp[name + ".weight"].float()[:, :, 0, 0].numpy().astype(np.float16)
+        b = sdp[name + ".bias"].float().numpy().astype(np.float32)
+        a = r.standard_normal((M, w.shape[1])).astype(np.float16)
+        ref = O.ref_gemm_epilogue(a, w, b, epi)
+        row = {}
+        for prec in ("fp16", "bf16", "fp8"):
+            wt = torch.from_numpy(w).to(dev)
+            wq, sc = (wt, None) if prec == "fp16" else kernels.weights_to_lowp(wt, prec)
+            got = kernels.gemm_lowp(torch.from_numpy(a).to(dev), wq, sc, prec, torch.from_numpy(b).to(dev),
+                                    epi=epi).float().cpu().numpy().astype(np.float64)
+            err = np.abs(got - ref)
+            row[prec] = {"max_rel": float(err.max() / np.abs(ref).max()), "mean_rel": float(err.mean() / np.abs(ref).mean())}
+        out["layers"][name] = row
+    return out
+
+
 def cpu_baseline(budget_s=20.0):
     """FP32 CPU oracle on this host: HT-S encode (+context update) and decode
     of a cropped sample of the same synthetic chunk, frames/s scaled by area
@@ -207,6 +288,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--profile-step", action="store_true")
+    ap.add_argument("--no-precision-forms", action="store_true")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -312,6 +394,10 @@ def main():
         classes = kernel_classes(recs)
         out["kernel_classes"] = classes
         out["roofline"] = roofline_of(classes)
+    if not args.no_precision_forms:
+        log("precision forms")
+        out["precision_forms"] = precision_forms(sdp, p_enc, enc_i["x_hat"], chunks, H, W, pad_b, pad_r,
+                                                 args.skip_thres, dev)
     if not args.no_cpu_baseline:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline()

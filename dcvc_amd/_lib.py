@@ -29,6 +29,8 @@ SIGNATURES = {
     "dcvc_conv1x1_bias_with_quant": [P, I, I, I, P, I, P, P, I, P, P, I, P],
     "dcvc_conv1x1_bias_wsilu_chunk_add": [P, I, I, I, P, I, P, P, I, P],
     "dcvc_conv_bias": [P, I, I, I, I, I, P, I, P, I, I, P, I, P],
+    "dcvc_weights_to_lowp": [P, I, I, I, P, P, P],
+    "dcvc_gemm_lowp": [P, I, I, I, P, I, P, I, I, P, P, I, P, I, P, I, P, P, I, P],
     "dcvc_transposed_conv": [P, I, I, I, I, I, P, I, P, P, I, P],
     "dcvc_d3x3": [P, I, I, I, I, I, P, P, I, P],
     "dcvc_pad_and_unshuffle": [P, I, I, I, I, I, I, I, I, P, I, P],
@@ -74,6 +76,7 @@ SIGNATURES = {
     "dcvc_dmci_compress": [P, P, I, I, I, I, I, I, P, P, P, P],
     "dcvc_dmci_decompress": [P, P, L, I, I, I, I, P],
     "dcvc_dmci_set_use_graphs": [P, I],
+    "dcvc_dmci_set_precision": [P, I],
     "dcvc_dmci_step_symbol_count": [P, I],
     "dcvc_dmci_debug_copy": [P, I, P, L],
     "dcvc_hts_create": [],
@@ -85,6 +88,7 @@ SIGNATURES = {
     "dcvc_hts_compress": [P, P, I, I, I, I, I, I, I, P, P, P, P],
     "dcvc_hts_decompress": [P, P, L, I, I, I, I, I, P],
     "dcvc_hts_set_use_graphs": [P, I],
+    "dcvc_hts_set_precision": [P, I],
     "dcvc_hts_debug_copy": [P, I, P, L],
     "dcvc_hts_last_symbol_count": [P],
     "dcvc_trace_begin": [I],

@@ -71,6 +71,10 @@ int dcvc_dmci_set_use_graphs(void* h, int on) {
     return guarded([&] { checked<DMCIProxy>(h)->set_use_graphs(on != 0); });
 }
 
+int dcvc_dmci_set_precision(void* h, int prec) {
+    return guarded([&] { checked<DMCIProxy>(h)->set_precision(prec); });
+}
+
 int dcvc_dmci_debug_copy(void* h, int what, void* host, long bytes) {
     return guarded([&] {
         if (!host || bytes < 0) throw std::invalid_argument("bad host buffer");
@@ -140,6 +144,10 @@ int dcvc_hts_decompress(void* h, const uint8_t* bits, long nbytes, int qp, int h
 
 int dcvc_hts_set_use_graphs(void* h, int on) {
     return guarded([&] { checked<HTSProxy>(h)->set_use_graphs(on != 0); });
+}
+
+int dcvc_hts_set_precision(void* h, int prec) {
+    return guarded([&] { checked<HTSProxy>(h)->set_precision(prec); });
 }
 
 int dcvc_hts_debug_copy(void* h, int what, void* host, long bytes) {

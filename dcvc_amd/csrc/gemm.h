@@ -22,6 +22,15 @@ enum GemmAMode {
     A_S1K3 = 3,   // 3x3 stride-1 pad-1 conv, K = 9 * cin, tap-major
 };
 
+// Operand precision of the product (activations stay fp16 in memory and are
+// converted as fragments leave LDS; weights are stored in the precision).
+enum GemmPrec {
+    PREC_F16 = 0,   // fp16 x fp16, v_mfma_f32_32x32x16_f16
+    PREC_BF16 = 1,  // bf16 x bf16, v_mfma_f32_32x32x16_bf16
+    PREC_FP8 = 2,   // OCP fp8 e4m3 x e4m3, v_mfma_scale_f32_32x32x64_f8f6f4; weights carry
+                    // one fp32 scale per row (max |w| / 448), activations are taken at scale 1
+};
+
 enum GemmDMode {
     D_PLAIN = 0,   // D[m, n] = D + m * ldd + n
     D_SUBPEL = 1,  // n = sub * (N/4) + c stored at pixel (2h + sub/2, 2w + sub%2) of the 2x grid
@@ -45,6 +54,8 @@ struct GemmArgs {
     const int* qp = nullptr;    // device QP index (nullptr -> row 0)
     f16* D = nullptr;
     int ldd = 0;
+    int prec = PREC_F16;             // GemmPrec; B is bf16 / fp8 bytes accordingly
+    const float* wscale = nullptr;   // PREC_FP8: per B row dequantisation scale
     // filled by gemm_launch: byte extents of A and B for the buffer range checks
     uint32_t a_bytes = 0, b_bytes = 0;
 };
@@ -53,5 +64,11 @@ int gemm_validate(const GemmArgs& g, int epi, int amode, int dmode);
 hipError_t gemm_launch(const GemmArgs& g, int epi, int amode, int dmode, hipStream_t stream);
 // 0: automatic tile choice; 1-5: force one tile configuration (tests, tuning)
 int gemm_set_tile_policy(int policy);
+
+// Weight conversion for the low-precision products, on the device: rows x K
+// fp16 -> bf16 (round to nearest even), or -> fp8 e4m3 with scale[r] =
+// max_k |w[r, k]| / 448 (1 for an all-zero row) and q = w / scale.
+hipError_t weights_to_bf16(const f16* w, long n, void* out, hipStream_t stream);
+hipError_t weights_to_fp8(const f16* w, int rows, int K, void* out, float* scale, hipStream_t stream);
 
 }  // namespace dcvc

@@ -99,22 +99,63 @@ DCVC_DEV void interleave() {
 
 int g_tile_policy = 0;  // 0 automatic, otherwise a forced configuration id
 
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef int i32x8 __attribute__((ext_vector_type(8)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+
+// 8 fp16 -> 8 bf16 (through fp32, round to nearest even)
+DCVC_DEV bf16x8 to_bf16x8(f16x8 v) {
+    bf16x8 o;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) o[e] = (__bf16)(float)v[e];
+    return o;
+}
+
+// 32 fp16 (k order) -> 32 OCP e4m3 bytes in the same order, saturated at
+// +-448 (the largest finite e4m3) before conversion
+DCVC_DEV i32x8 to_fp8x32(const f16x8 (&v)[4]) {
+    i32x8 o;
+    const f16x2 lim = {(f16)448.f, (f16)448.f};
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+        const f16x8& src = v[d >> 1];
+        const int e = (d & 1) * 4;
+        f16x2 lo = {src[e], src[e + 1]}, hi = {src[e + 2], src[e + 3]};
+        lo = __builtin_elementwise_max(__builtin_elementwise_min(lo, lim), -lim);
+        hi = __builtin_elementwise_max(__builtin_elementwise_min(hi, lim), -lim);
+        s16x2 t = {0, 0};
+        t = __builtin_amdgcn_cvt_scalef32_pk_fp8_f16(t, lo, 1.0f, false);
+        t = __builtin_amdgcn_cvt_scalef32_pk_fp8_f16(t, hi, 1.0f, true);
+        o[d] = __builtin_bit_cast(int, t);
+    }
+    return o;
+}
+
 }  // namespace
 
-template <int BM_, int BN_, int WM_, int WN_, int STAGES_, int EPI_, int AMODE_, int DMODE_>
+template <int BM_, int BN_, int WM_, int WN_, int STAGES_, int EPI_, int AMODE_, int DMODE_, int PREC_ = PREC_F16>
 struct Tile {
     static constexpr int BM = BM_, BN = BN_, WM = WM_, WN = WN_;
+    static constexpr int PREC = PREC_;
+    static constexpr bool kFp8 = (PREC_ == PREC_FP8);
+    // K per ring tile: 32 (one 64-byte fp16 row), or 64 for fp8, whose B row
+    // is 64 fp8 bytes and whose A tile is two fp16 half tiles
+    static constexpr int kKT = kFp8 ? 64 : 32;
+    static constexpr int kAH = kFp8 ? 2 : 1;
     // LDS ring of STAGES K tiles, STAGES - 2 in flight ahead of the one being
     // read: the slot refilled at iteration kt is that of tile kt - 2, whose
     // fragment reads every wave consumed before this iteration's barrier.
     // STAGES == 3 keeps one tile ahead and refills the slot of tile kt - 1,
     // so its barrier also waits for the wave's own fragment reads (lgkmcnt).
+    // (fp8 converts every fragment inside its own iteration, so its reads are
+    // complete before the next barrier and STAGES - 1 tiles can be in flight)
     static constexpr int kStages = STAGES_;
-    static constexpr int kAhead = STAGES_ >= 5 ? STAGES_ - 2 : STAGES_ - 1;
-    static constexpr bool kWaitLds = (kAhead == STAGES_ - 1);
+    static constexpr int kAhead = (STAGES_ >= 5 && !kFp8) ? STAGES_ - 2 : STAGES_ - 1;
+    static constexpr bool kWaitLds = !kFp8 && (kAhead == STAGES_ - 1);
     // configurations meant to run 2 blocks per CU keep within 256 registers
     // per lane (architectural + accumulation) so two waves share each SIMD
-    static constexpr int kWavesPerSimd = (STAGES_ == 3 || (BM_ * BN_ <= 128 * 128)) ? 2 : 1;
+    static constexpr int kWavesPerSimd = ((STAGES_ == 3 && PREC_ != PREC_FP8) || (BM_ * BN_ <= 128 * 128)) ? 2 : 1;
     static constexpr int EPI = EPI_, AMODE = AMODE_, DMODE = DMODE_;
     static constexpr int kWaves = WM * WN;
     static constexpr int kThreads = kWaves * 64;
@@ -127,8 +168,8 @@ struct Tile {
     static constexpr int BRW = BN / kWaves;
     static constexpr int AI = ARW / 16;               // DMA instructions (16 rows each)
     static constexpr int BI = BRW / 16;
-    static constexpr int kDma = AI + BI;              // per lane per K tile
-    static constexpr int kStageBytes = (BM + BN) * kRowBytes;
+    static constexpr int kDma = AI * kAH + BI;        // per lane per K tile
+    static constexpr int kStageBytes = (BM * kAH + BN) * kRowBytes;
     static constexpr int kEPitch = BNO + 4;           // epilogue fp32 row pitch
     static constexpr int kERows = WM * 32;            // rows staged per epilogue pass
     static constexpr int kEChunks = kERows * BNO / 8;
@@ -219,46 +260,53 @@ __global__ __launch_bounds__(T::kThreads, T::kWavesPerSimd) void gemm_kernel(Gem
             grow = nt * T::BN + row;
             valid = grow < g.N;
         }
-        b_off[q] = valid ? (uint32_t)(((size_t)grow * g.K + kc * 8) * 2) : kOOB;
+        if constexpr (T::kFp8)
+            b_off[q] = valid ? (uint32_t)((size_t)grow * g.K + kc * 16) : kOOB;
+        else
+            b_off[q] = valid ? (uint32_t)(((size_t)grow * g.K + kc * 8) * 2) : kOOB;
     }
 
     // issue the DMAs of K tile kt into ring stage st; tiles past the end read
     // zeros into a slot that is never read (uniform instruction counts keep
     // the vmcnt bookkeeping static)
-    const int KT = g.K / kBK;
+    const int KT = g.K / T::kKT;
     auto dma_tile = [&](int kt, int st) {
         const uint32_t past = (kt < KT) ? 0u : kOOB;
-        const int k0 = (kt < KT) ? kt * kBK : 0;
-        char* sa = smem + st * T::kStageBytes + wave * T::ARW * kRowBytes;
-        char* sb = smem + st * T::kStageBytes + (BM + wave * T::BRW) * kRowBytes;
-        if constexpr (AMODE == A_PLAIN) {
+        const int kbase = (kt < KT) ? kt * T::kKT : 0;
+        char* sb = smem + st * T::kStageBytes + (BM * T::kAH + wave * T::BRW) * kRowBytes;
 #pragma unroll
-            for (int q = 0; q < T::AI; ++q)
-                dma16(rA, sa + q * 16 * kRowBytes, (a_off[q] + (uint32_t)k0 * 2) | past);
-        } else {
-            // cin % 64 == 0, so one K tile never straddles two taps
-            const int tap = k0 / g.cin;
-            const int c0 = k0 - tap * g.cin;
-            int dy, dx;
-            if constexpr (AMODE == A_S2K2) {
-                dy = tap >> 1;
-                dx = tap & 1;
+        for (int h = 0; h < T::kAH; ++h) {
+            const int k0 = kbase + h * kBK;
+            char* sa = smem + st * T::kStageBytes + (h * BM + wave * T::ARW) * kRowBytes;
+            if constexpr (AMODE == A_PLAIN) {
+#pragma unroll
+                for (int q = 0; q < T::AI; ++q)
+                    dma16(rA, sa + q * 16 * kRowBytes, (a_off[q] + (uint32_t)k0 * 2) | past);
             } else {
-                dy = tap / 3;
-                dx = tap - dy * 3;
-            }
+                // cin % 64 == 0, so one K tile never straddles two taps
+                const int tap = k0 / g.cin;
+                const int c0 = k0 - tap * g.cin;
+                int dy, dx;
+                if constexpr (AMODE == A_S2K2) {
+                    dy = tap >> 1;
+                    dx = tap & 1;
+                } else {
+                    dy = tap / 3;
+                    dx = tap - dy * 3;
+                }
 #pragma unroll
-            for (int q = 0; q < T::AI; ++q) {
-                const int hi = a_h[q] + dy, wi = a_w[q] + dx;
-                const bool ok = (unsigned)hi < (unsigned)g.inH && (unsigned)wi < (unsigned)g.inW;
-                const uint32_t off =
-                    ok ? (uint32_t)(((size_t)(a_pix[q] + hi * g.inW + wi) * g.lda + c0 + kc * 8) * 2) : kOOB;
-                dma16(rA, sa + q * 16 * kRowBytes, off | past);
+                for (int q = 0; q < T::AI; ++q) {
+                    const int hi = a_h[q] + dy, wi = a_w[q] + dx;
+                    const bool ok = (unsigned)hi < (unsigned)g.inH && (unsigned)wi < (unsigned)g.inW;
+                    const uint32_t off =
+                        ok ? (uint32_t)(((size_t)(a_pix[q] + hi * g.inW + wi) * g.lda + c0 + kc * 8) * 2) : kOOB;
+                    dma16(rA, sa + q * 16 * kRowBytes, off | past);
+                }
             }
         }
+        const uint32_t kb = T::kFp8 ? (uint32_t)kbase : (uint32_t)kbase * 2;
 #pragma unroll
-        for (int q = 0; q < T::BI; ++q)
-            dma16(rB, sb + q * 16 * kRowBytes, (b_off[q] + (uint32_t)k0 * 2) | past);
+        for (int q = 0; q < T::BI; ++q) dma16(rB, sb + q * 16 * kRowBytes, (b_off[q] + kb) | past);
     };
 
     f32x16 acc[SM][SN];
@@ -272,7 +320,7 @@ __global__ __launch_bounds__(T::kThreads, T::kWavesPerSimd) void gemm_kernel(Gem
     // fragment addresses: lane reads row lr, K chunk 2 * ks + (lane >> 5)
     const int lr = lane & 31;
     const int fa_row = (wm * T::TM + lr) * kRowBytes;
-    const int fb_row = (BM + wn * T::TN + lr) * kRowBytes;
+    const int fb_row = (BM * T::kAH + wn * T::TN + lr) * kRowBytes;
     const int pos0 = ((0 + (lane >> 5)) ^ swz(lr)) * 16;
     const int pos1 = ((2 + (lane >> 5)) ^ swz(lr)) * 16;
     f16x8 f0a[SM], f0b[SN], f1a[SM], f1b[SN];
@@ -286,11 +334,55 @@ __global__ __launch_bounds__(T::kThreads, T::kWavesPerSimd) void gemm_kernel(Gem
             fb[j] = *reinterpret_cast<const f16x8*>(base + fb_row + j * 32 * kRowBytes);
     };
     auto mfma_all = [&](const f16x8* fa, const f16x8* fb) {
+        if constexpr (T::PREC == PREC_BF16) {
+            bf16x8 ca[SM];
+#pragma unroll
+            for (int i = 0; i < SM; ++i) ca[i] = to_bf16x8(fa[i]);
+#pragma unroll
+            for (int i = 0; i < SM; ++i)
+#pragma unroll
+                for (int j = 0; j < SN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ca[i], __builtin_bit_cast(bf16x8, fb[j]),
+                                                                        acc[i][j], 0, 0, 0);
+        } else {
+#pragma unroll
+            for (int i = 0; i < SM; ++i)
+#pragma unroll
+                for (int j = 0; j < SN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+        }
+    };
+    // fp8: one k64 step per ring tile.  A: the lane's 32 k of its row (half
+    // tile lane >> 5, all four chunks), converted; B: the matching 32 fp8
+    // bytes (chunks 2h, 2h + 1 of the 64-byte row).  Any k order works as
+    // long as A and B use the same one.
+    const int a8_row = ((lane >> 5) * BM + wm * T::TM + lr) * kRowBytes;
+    auto read_fp8 = [&](int st, i32x8* fa, i32x8* fb) {
+        const char* base = smem + st * T::kStageBytes;
+#pragma unroll
+        for (int i = 0; i < SM; ++i) {
+            f16x8 v[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                v[c] = *reinterpret_cast<const f16x8*>(base + a8_row + i * 32 * kRowBytes + ((c ^ swz(lr)) * 16));
+            fa[i] = to_fp8x32(v);
+        }
+#pragma unroll
+        for (int j = 0; j < SN; ++j) {
+            const char* rowp = base + fb_row + j * 32 * kRowBytes;
+            const int c0 = 2 * (lane >> 5);
+            const i32x4 lo = *reinterpret_cast<const i32x4*>(rowp + ((c0 ^ swz(lr)) * 16));
+            const i32x4 hi = *reinterpret_cast<const i32x4*>(rowp + (((c0 + 1) ^ swz(lr)) * 16));
+            fb[j] = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+        }
+    };
+    auto mfma_fp8 = [&](const i32x8* fa, const i32x8* fb) {
 #pragma unroll
         for (int i = 0; i < SM; ++i)
 #pragma unroll
             for (int j = 0; j < SN; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(fa[i], fb[j], acc[i][j], 0, 0, 0, 127,
+                                                                            0, 127);
     };
 
     // K loop.  Tile kt lives in ring slot kt % kStages; at iteration kt the
@@ -310,22 +402,27 @@ __global__ __launch_bounds__(T::kThreads, T::kWavesPerSimd) void gemm_kernel(Gem
         m = m0 + (rl >> 5) * T::TM + i * 32 + (rl & 31);
         n = nt * T::BNO + cc;
     };
-    float bias_r[SN];
+    float bias_r[SN], wsc_r[T::kFp8 ? SN : 1];
 #pragma unroll
     for (int s = 0; s < SN; ++s) {
+        int row;
+        bool ok;
         if constexpr (T::kQuad) {
             const int n = nt * T::BNO + wn * T::TNO + (s >> 2) * 32 + lr;
-            bias_r[s] = (n < g.N) ? g.bias[4 * n + (s & 3)] : 0.f;
+            row = 4 * n + (s & 3);
+            ok = n < g.N;
         } else {
-            const int n = nt * BN + wn * T::TN + s * 32 + lr;
-            bias_r[s] = (n < g.N) ? g.bias[n] : 0.f;
+            row = nt * BN + wn * T::TN + s * 32 + lr;
+            ok = row < g.N;
         }
+        bias_r[s] = ok ? g.bias[row] : 0.f;
+        if constexpr (T::kFp8) wsc_r[s] = ok ? g.wscale[row] : 0.f;
     }
     // shortcut passes held in registers through the K loop: all of them
     // when they fit in 64 VGPRs, else only the first; each epilogue pass then
     // loads the pass kS1Early ahead of it
     constexpr bool kS1 = (EPI == EPI_SHORTCUT || EPI == EPI_SHORTCUT2 || EPI == EPI_SHORTCUT_QUANT);
-    constexpr int kS1Early = (SM * EC * 4 <= 64 && T::kWavesPerSimd == 1) ? SM : 1;
+    constexpr int kS1Early = (T::kFp8 && SM * SN >= 16) ? 0 : (SM * EC * 4 <= 64 && T::kWavesPerSimd == 1) ? SM : 1;
     f16x8 s1r[kS1 ? SM : 1][kS1 ? EC : 1];
     auto load_s1 = [&](int i0, int i1) {
 #pragma unroll
@@ -339,28 +436,47 @@ __global__ __launch_bounds__(T::kThreads, T::kWavesPerSimd) void gemm_kernel(Gem
                                                   : zero8;
             }
     };
-    if constexpr (kS1) load_s1(0, kS1Early);
+    if constexpr (kS1 && kS1Early > 0) load_s1(0, kS1Early);
 
 #pragma unroll
     for (int st = 0; st < T::kAhead; ++st) dma_tile(st, st);
-    wait_tile_and_barrier<(T::kAhead - 1) * T::kDma, T::kWaitLds>();
-    dma_tile(T::kAhead, T::kAhead);
-    read_frags(0, pos0, f0a, f0b);
-    read_frags(0, pos1, f1a, f1b);
-    mfma_all(f0a, f0b);
-    for (int kt = 1; kt < KT; ++kt) {
-        // this wave's DMAs of tile kt have landed; after the barrier, every wave's
+    if constexpr (T::kFp8) {
+        i32x8 pa[SM], pb[SN];
+        wait_tile_and_barrier<(T::kAhead - 1) * T::kDma, false>();
+        dma_tile(T::kAhead, T::kAhead);
+        read_fp8(0, pa, pb);
+        for (int kt = 1; kt < KT; ++kt) {
+            wait_tile_and_barrier<(T::kAhead - 1) * T::kDma, false>();
+            dma_tile(kt + T::kAhead, (kt + T::kAhead) % T::kStages);
+            i32x8 na[SM], nb[SN];
+            read_fp8(kt % T::kStages, na, nb);
+            mfma_fp8(pa, pb);  // tile kt - 1
+#pragma unroll
+            for (int i = 0; i < SM; ++i) pa[i] = na[i];
+#pragma unroll
+            for (int j = 0; j < SN; ++j) pb[j] = nb[j];
+        }
+        mfma_fp8(pa, pb);
+    } else {
         wait_tile_and_barrier<(T::kAhead - 1) * T::kDma, T::kWaitLds>();
-        const int st = kt % T::kStages;
-        dma_tile(kt + T::kAhead, (kt + T::kAhead) % T::kStages);
-        read_frags(st, pos0, f0a, f0b);
-        mfma_all(f1a, f1b);  // second k16 step of tile kt - 1
-        interleave<NDS, NMF, T::kDma>();
-        read_frags(st, pos1, f1a, f1b);
+        dma_tile(T::kAhead, T::kAhead);
+        read_frags(0, pos0, f0a, f0b);
+        read_frags(0, pos1, f1a, f1b);
         mfma_all(f0a, f0b);
-        interleave<NDS, NMF, 0>();
+        for (int kt = 1; kt < KT; ++kt) {
+            // this wave's DMAs of tile kt have landed; after the barrier, every wave's
+            wait_tile_and_barrier<(T::kAhead - 1) * T::kDma, T::kWaitLds>();
+            const int st = kt % T::kStages;
+            dma_tile(kt + T::kAhead, (kt + T::kAhead) % T::kStages);
+            read_frags(st, pos0, f0a, f0b);
+            mfma_all(f1a, f1b);  // second k16 step of tile kt - 1
+            interleave<NDS, NMF, T::kDma>();
+            read_frags(st, pos1, f1a, f1b);
+            mfma_all(f0a, f0b);
+            interleave<NDS, NMF, 0>();
+        }
+        mfma_all(f1a, f1b);
     }
-    mfma_all(f1a, f1b);
     __syncthreads();  // every wave is done with the ring before the epilogue reuses it
 
     // ---- epilogue ----
@@ -390,9 +506,15 @@ __global__ __launch_bounds__(T::kThreads, T::kWavesPerSimd) void gemm_kernel(Gem
                 if constexpr (T::kQuad) {
                     x = 0.f;
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) x += wsilu(acc[i][4 * gI + j][v] + bias_r[4 * gI + j]);
+                    for (int j = 0; j < 4; ++j) {
+                        float a = acc[i][4 * gI + j][v];
+                        if constexpr (T::kFp8) a *= wsc_r[4 * gI + j];
+                        x += wsilu(a + bias_r[4 * gI + j]);
+                    }
                 } else {
-                    x = acc[i][gI][v] + bias_r[gI];
+                    float a = acc[i][gI][v];
+                    if constexpr (T::kFp8) a *= wsc_r[gI];
+                    x = a + bias_r[gI];
                     if constexpr (EPI == EPI_WSILU) x = wsilu(x);
                 }
                 E[row * EP + col] = x;
@@ -457,9 +579,9 @@ __global__ __launch_bounds__(T::kThreads, T::kWavesPerSimd) void gemm_kernel(Gem
 //   3: 128 x 128 waves 4 x 1, 4: 128 x 256 3 st, 5: 256 x 256 4 st
 constexpr int kPolicies = 5;
 
-template <int BM, int BN, int WM, int WN, int ST, int EPI, int AMODE, int DMODE>
+template <int BM, int BN, int WM, int WN, int ST, int EPI, int AMODE, int DMODE, int PREC = PREC_F16>
 static hipError_t launch_tile(GemmArgs g, hipStream_t stream) {
-    using T = Tile<BM, BN, WM, WN, ST, EPI, AMODE, DMODE>;
+    using T = Tile<BM, BN, WM, WN, ST, EPI, AMODE, DMODE, PREC>;
     const int tiles_m = ceil_div(g.M, BM);
     const int tiles_n = ceil_div(g.N, T::BNO);
     auto kern = gemm_kernel<T>;
@@ -488,25 +610,54 @@ static int pick_tile(const GemmArgs& g, bool quad) {
     return 3;
 }
 
-template <int EPI, int AMODE, int DMODE>
+template <int EPI, int AMODE, int DMODE, int PREC = PREC_F16>
 static hipError_t dispatch_tiles(const GemmArgs& g, hipStream_t stream) {
     const int t = pick_tile(g, EPI == EPI_QUAD);
-    if constexpr (EPI == EPI_QUAD) {
+    if constexpr (PREC == PREC_FP8) {
+        // fp8 K tiles are 48 KB (256 x 256) / 24 KB (128 x 128): 3 stages.  The
+        // shortcut epilogues (HBM-bound) stay on 128 x 128, whose registers
+        // also hold the shortcut tile.
+        constexpr bool kShort = (EPI == EPI_SHORTCUT || EPI == EPI_SHORTCUT2 || EPI == EPI_SHORTCUT_QUANT);
+        const bool big = !kShort && (t == 1 || t == 2 || t == 4);
+        if constexpr (EPI == EPI_QUAD) {
+            if (big) return launch_tile<256, 256, 2, 2, 3, EPI, AMODE, DMODE, PREC>(g, stream);
+            return launch_tile<128, 128, 4, 1, 3, EPI, AMODE, DMODE, PREC>(g, stream);
+        } else if constexpr (kShort) {
+            return launch_tile<128, 128, 2, 2, 3, EPI, AMODE, DMODE, PREC>(g, stream);
+        } else {
+            if (big) return launch_tile<256, 256, 2, 2, 3, EPI, AMODE, DMODE, PREC>(g, stream);
+            return launch_tile<128, 128, 2, 2, 3, EPI, AMODE, DMODE, PREC>(g, stream);
+        }
+    } else if constexpr (EPI == EPI_QUAD) {
         switch (t) {
-        case 1: return launch_tile<256, 256, 2, 2, 5, EPI, AMODE, DMODE>(g, stream);
-        case 2: return launch_tile<128, 256, 2, 2, 5, EPI, AMODE, DMODE>(g, stream);
-        case 4: return launch_tile<128, 256, 2, 2, 3, EPI, AMODE, DMODE>(g, stream);
-        case 5: return launch_tile<256, 256, 2, 2, 4, EPI, AMODE, DMODE>(g, stream);
-        default: return launch_tile<128, 128, 4, 1, 5, EPI, AMODE, DMODE>(g, stream);
+        case 1: return launch_tile<256, 256, 2, 2, 5, EPI, AMODE, DMODE, PREC>(g, stream);
+        case 2: return launch_tile<128, 256, 2, 2, 5, EPI, AMODE, DMODE, PREC>(g, stream);
+        case 4: return launch_tile<128, 256, 2, 2, 3, EPI, AMODE, DMODE, PREC>(g, stream);
+        case 5: return launch_tile<256, 256, 2, 2, 4, EPI, AMODE, DMODE, PREC>(g, stream);
+        default: return launch_tile<128, 128, 4, 1, 5, EPI, AMODE, DMODE, PREC>(g, stream);
         }
     } else {
         switch (t) {
-        case 1: return launch_tile<256, 256, 2, 2, 5, EPI, AMODE, DMODE>(g, stream);
-        case 2: return launch_tile<256, 128, 2, 2, 5, EPI, AMODE, DMODE>(g, stream);
-        case 4: return launch_tile<256, 128, 2, 2, 3, EPI, AMODE, DMODE>(g, stream);
-        case 5: return launch_tile<128, 256, 2, 2, 3, EPI, AMODE, DMODE>(g, stream);
-        default: return launch_tile<128, 128, 2, 2, 5, EPI, AMODE, DMODE>(g, stream);
+        case 1: return launch_tile<256, 256, 2, 2, 5, EPI, AMODE, DMODE, PREC>(g, stream);
+        case 2: return launch_tile<256, 128, 2, 2, 5, EPI, AMODE, DMODE, PREC>(g, stream);
+        case 4: return launch_tile<256, 128, 2, 2, 3, EPI, AMODE, DMODE, PREC>(g, stream);
+        case 5: return launch_tile<128, 256, 2, 2, 3, EPI, AMODE, DMODE, PREC>(g, stream);
+        default: return launch_tile<128, 128, 2, 2, 5, EPI, AMODE, DMODE, PREC>(g, stream);
         }
+    }
+}
+
+template <int PREC>
+static hipError_t dispatch_plain(const GemmArgs& g, int epi, hipStream_t stream) {
+    switch (epi) {
+    case EPI_BIAS: return dispatch_tiles<EPI_BIAS, A_PLAIN, D_PLAIN, PREC>(g, stream);
+    case EPI_WSILU: return dispatch_tiles<EPI_WSILU, A_PLAIN, D_PLAIN, PREC>(g, stream);
+    case EPI_SHORTCUT: return dispatch_tiles<EPI_SHORTCUT, A_PLAIN, D_PLAIN, PREC>(g, stream);
+    case EPI_SHORTCUT2: return dispatch_tiles<EPI_SHORTCUT2, A_PLAIN, D_PLAIN, PREC>(g, stream);
+    case EPI_SHORTCUT_QUANT: return dispatch_tiles<EPI_SHORTCUT_QUANT, A_PLAIN, D_PLAIN, PREC>(g, stream);
+    case EPI_QUANT: return dispatch_tiles<EPI_QUANT, A_PLAIN, D_PLAIN, PREC>(g, stream);
+    case EPI_QUAD: return dispatch_tiles<EPI_QUAD, A_PLAIN, D_PLAIN, PREC>(g, stream);
+    default: return hipErrorInvalidValue;
     }
 }
 
@@ -523,7 +674,7 @@ hipError_t gemm_launch(const GemmArgs& g_in, int epi, int amode, int dmode, hipS
         a_last = g.cin;
     }
     g.a_bytes = (uint32_t)(((a_rows - 1) * g.lda + a_last) * 2);
-    g.b_bytes = (uint32_t)((size_t)(epi == EPI_QUAD ? 4 * g.N : g.N) * g.K * 2);
+    g.b_bytes = (uint32_t)((size_t)(epi == EPI_QUAD ? 4 * g.N : g.N) * g.K * (g.prec == PREC_FP8 ? 1 : 2));
     // implicit-conv A modes and the sub-pixel store come with the plain bias
     // epilogue only (conv_bias, transposed_conv)
     if (amode != A_PLAIN || dmode != D_PLAIN) {
@@ -536,20 +687,12 @@ hipError_t gemm_launch(const GemmArgs& g_in, int epi, int amode, int dmode, hipS
         default: return hipErrorInvalidValue;
         }
     }
-#define DCVC_EPI_CASE(E) \
-    case E: return dispatch_tiles<E, A_PLAIN, D_PLAIN>(g, stream);
-    switch (epi) {
-        DCVC_EPI_CASE(EPI_BIAS)
-        DCVC_EPI_CASE(EPI_WSILU)
-        DCVC_EPI_CASE(EPI_SHORTCUT)
-        DCVC_EPI_CASE(EPI_SHORTCUT2)
-        DCVC_EPI_CASE(EPI_SHORTCUT_QUANT)
-        DCVC_EPI_CASE(EPI_QUANT)
-        DCVC_EPI_CASE(EPI_QUAD)
-    default:
-        return hipErrorInvalidValue;
+    switch (g.prec) {
+    case PREC_F16: return dispatch_plain<PREC_F16>(g, epi, stream);
+    case PREC_BF16: return dispatch_plain<PREC_BF16>(g, epi, stream);
+    case PREC_FP8: return dispatch_plain<PREC_FP8>(g, epi, stream);
+    default: return hipErrorInvalidValue;
     }
-#undef DCVC_EPI_CASE
 }
 
 static bool aligned16(const void* p) { return ((uintptr_t)p) % 16 == 0; }
@@ -557,7 +700,8 @@ static bool aligned16(const void* p) { return ((uintptr_t)p) % 16 == 0; }
 // Host-side validation shared by every GEMM entry point.
 int gemm_validate(const GemmArgs& g, int epi, int amode, int dmode) {
     if (!g.A || !g.B || !g.D || !g.bias) return DCVC_ERR_ARG;
-    if ((amode != A_PLAIN || dmode != D_PLAIN) && epi != EPI_BIAS) return DCVC_ERR_ARG;
+    if ((amode != A_PLAIN || dmode != D_PLAIN) && (epi != EPI_BIAS || g.prec != PREC_F16)) return DCVC_ERR_ARG;
+    if (g.prec < PREC_F16 || g.prec > PREC_FP8 || (g.prec == PREC_FP8 && !g.wscale)) return DCVC_ERR_ARG;
     if (g.M <= 0 || g.N <= 0 || g.K <= 0) return DCVC_ERR_ARG;
     if (g.K % 64 != 0 || g.N % 8 != 0) return DCVC_ERR_ARG;
     if (g.lda % 8 != 0 || g.lda < (amode == A_PLAIN ? g.K : g.cin)) return DCVC_ERR_ARG;
@@ -588,6 +732,56 @@ int gemm_validate(const GemmArgs& g, int epi, int amode, int dmode) {
     if (epi == EPI_SHORTCUT2 && (!g.S2 || g.lds2 < g.N || g.lds2 % 8 != 0 || !aligned16(g.S2))) return DCVC_ERR_ARG;
     if ((epi == EPI_SHORTCUT_QUANT || epi == EPI_QUANT) && !g.q) return DCVC_ERR_ARG;
     return DCVC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// weight conversion
+// ---------------------------------------------------------------------------
+
+__global__ void f16_to_bf16_kernel(const f16* __restrict__ w, long n, __bf16* __restrict__ out) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (__bf16)(float)w[i];
+}
+
+// one block per row: max |w|, then q = w / scale as e4m3 (|q| <= 448 by construction)
+__global__ __launch_bounds__(256) void rows_to_fp8_kernel(const f16* __restrict__ w, int K, uint8_t* __restrict__ out,
+                                                          float* __restrict__ scale) {
+    __shared__ float red[256];
+    const f16* row = w + (size_t)blockIdx.x * K;
+    float m = 0.f;
+    for (int k = threadIdx.x; k < K; k += 256) m = fmaxf(m, fabsf((float)row[k]));
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    const float sc = red[0] > 0.f ? red[0] / 448.f : 1.f;
+    if (threadIdx.x == 0) scale[blockIdx.x] = sc;
+    const float inv = 1.f / sc;
+    uint8_t* orow = out + (size_t)blockIdx.x * K;
+    for (int k = 2 * threadIdx.x; k < K; k += 512) {
+        const float a = fminf(fmaxf((float)row[k] * inv, -448.f), 448.f);
+        const float b = fminf(fmaxf((float)row[k + 1] * inv, -448.f), 448.f);
+        const int p = __builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false);
+        orow[k] = (uint8_t)(p & 0xff);
+        orow[k + 1] = (uint8_t)((p >> 8) & 0xff);
+    }
+}
+
+hipError_t weights_to_bf16(const f16* w, long n, void* out, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(f16_to_bf16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, w, n,
+                       static_cast<__bf16*>(out));
+    return hipGetLastError();
+}
+
+hipError_t weights_to_fp8(const f16* w, int rows, int K, void* out, float* scale, hipStream_t stream) {
+    if (rows <= 0) return hipSuccess;
+    if (K % 2 != 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(rows_to_fp8_kernel, dim3(rows), dim3(256), 0, stream, w, K, static_cast<uint8_t*>(out),
+                       scale);
+    return hipGetLastError();
 }
 
 int gemm_set_tile_policy(int policy) {

@@ -106,6 +106,33 @@ int dcvc_conv1x1_bias_wsilu_chunk_add(const void* A, int lda, int M, int K, cons
     return run_gemm(g, EPI_QUAD, A_PLAIN, D_PLAIN, stream);
 }
 
+int dcvc_weights_to_lowp(const void* W, int rows, int K, int prec, void* Wq, float* scale, void* stream) {
+    if (!W || !Wq || rows <= 0 || K <= 0 || K % 2 != 0) return DCVC_ERR_ARG;
+    if (prec == PREC_BF16) return st(weights_to_bf16(H(W), (long)rows * K, Wq, S(stream)));
+    if (prec == PREC_FP8) {
+        if (!scale) return DCVC_ERR_ARG;
+        return st(weights_to_fp8(H(W), rows, K, Wq, scale, S(stream)));
+    }
+    return DCVC_ERR_ARG;
+}
+
+int dcvc_gemm_lowp(const void* A, int lda, int M, int K, const void* Wq, int N, const float* scale, int prec,
+                   int epi, const float* bias, const void* S1, int lds1, const void* S2, int lds2, const void* q,
+                   int qstride, const int* qp, void* D, int ldd, void* stream) {
+    if (epi < EPI_BIAS || epi > EPI_QUAD) return DCVC_ERR_ARG;
+    GemmArgs g = plain_args(A, lda, M, K, Wq, N, bias, D, ldd);
+    g.prec = prec;
+    g.wscale = scale;
+    g.S1 = H(S1);
+    g.lds1 = lds1;
+    g.S2 = H(S2);
+    g.lds2 = lds2;
+    g.q = H(q);
+    g.qstride = qstride;
+    g.qp = qp;
+    return run_gemm(g, epi, A_PLAIN, D_PLAIN, stream);
+}
+
 int dcvc_conv_bias(const void* X, int ldx, int n_img, int H_, int W_, int Cin, const void* Wt, int N,
                    const float* bias, int kernel, int stride, void* D, int ldd, void* stream) {
     int amode;

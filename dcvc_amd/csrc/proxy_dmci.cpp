@@ -56,6 +56,7 @@ void DMCIWeights::load(const HostDict& d, float skip) {
         cdf_len[i] = to_i32(l);
         cdf_row[i] = (int)(c.numel() / l.numel());
     }
+    hip_check(hipDeviceSynchronize(), "weight conversion");
 }
 
 DMCIProxy::DMCIProxy() {

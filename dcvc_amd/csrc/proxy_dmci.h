@@ -36,6 +36,12 @@ public:
                  const uint8_t** bits, long* nbytes);
     void decompress(const uint8_t* bits, long nbytes, int qp, int height, int width, int ec_parallel, hipStream_t s);
     void set_use_graphs(bool on) { m_use_graphs = on; reset_graphs(); }
+    // GemmPrec of every 1x1-conv product (precision form); recaptures the graphs
+    void set_precision(int prec) {
+        if (prec < PREC_F16 || prec > PREC_FP8) throw std::invalid_argument("precision must be 0, 1 or 2");
+        m_prec = prec;
+        reset_graphs();
+    }
     long step_count(int step) const { return m_counts[step]; }
     void copy_debug(int what, void* host, long bytes);
 
@@ -50,7 +56,7 @@ private:
     void dec_index(int step);
     void dec_restore(int step);
     void decode_step_on_host(int step);
-    Exec exec() { return Exec{m_stream, m_d_qp, &m_scratch}; }
+    Exec exec() { return Exec{m_stream, m_d_qp, &m_scratch, m_prec}; }
 
     std::unique_ptr<DMCIWeights> m_w;
     RansEncoder m_enc;
@@ -58,6 +64,7 @@ private:
     hipStream_t m_stream = nullptr, m_copy = nullptr;
     hipEvent_t m_ev_in = nullptr, m_ev_out = nullptr, m_ev_sym = nullptr;
     bool m_use_graphs = true;
+    int m_prec = PREC_F16;
     Graph m_g_enc0, m_g_enc1, m_g_dec[5];
 
     int m_Hp = 0, m_Wp = 0, m_H8 = 0, m_W8 = 0, m_H16 = 0, m_W16 = 0, m_H16p = 0, m_W16p = 0, m_H64 = 0, m_W64 = 0;

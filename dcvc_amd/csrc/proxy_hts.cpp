@@ -72,6 +72,7 @@ void HTSWeights::load(const HostDict& d, float skip) {
         cdf_len[i] = to_i32(l);
         cdf_row[i] = (int)(c.numel() / l.numel());
     }
+    hip_check(hipDeviceSynchronize(), "weight conversion");
 }
 
 // ---------------------------------------------------------------------------

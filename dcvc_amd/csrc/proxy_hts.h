@@ -49,6 +49,12 @@ public:
     void decompress(const uint8_t* bits, long nbytes, int qp, int height, int width, int ec_parallel, bool reset,
                     hipStream_t s);
     void set_use_graphs(bool on) { m_use_graphs = on; reset_graphs(); }
+    // GemmPrec of every 1x1-conv product (precision form); recaptures the graphs
+    void set_precision(int prec) {
+        if (prec < PREC_F16 || prec > PREC_FP8) throw std::invalid_argument("precision must be 0, 1 or 2");
+        m_prec = prec;
+        reset_graphs();
+    }
     size_t device_bytes() const { return m_buf.bytes(); }
     // diagnostics for tests: last compacted symbol count and y / symbols on host
     long last_symbol_count() const { return m_last_count; }
@@ -67,7 +73,7 @@ private:
     void hyper_and_prior();
     void spatial_means(int step);
     void decoder_and_recon(bool recon_all, bool reset_for_encoder);
-    Exec exec() { return Exec{m_stream, m_d_qp, &m_scratch}; }
+    Exec exec() { return Exec{m_stream, m_d_qp, &m_scratch, m_prec}; }
 
     std::shared_ptr<HTSWeights> m_w;
     RansEncoder m_enc;
@@ -75,6 +81,7 @@ private:
     hipStream_t m_stream = nullptr, m_copy = nullptr;
     hipEvent_t m_ev_in = nullptr, m_ev_out = nullptr, m_ev_sym = nullptr;
     bool m_use_graphs = true;
+    int m_prec = PREC_F16;
     Graph m_g_enc0, m_g_enc1[2], m_g_dec0[2], m_g_dec1, m_g_dec2, m_g_dec3;
 
     // geometry

@@ -94,16 +94,22 @@ struct Exec {
     hipStream_t stream = nullptr;
     const int* d_qp = nullptr;
     Scratch* sc = nullptr;
+    int prec = PREC_F16;  // GemmPrec of the 1x1-conv products
 };
 
 struct Conv1x1 {
     f16* w = nullptr;
     float* b = nullptr;
+    void* w_bf16 = nullptr;   // low-precision copies made at load (PREC_BF16 / PREC_FP8)
+    void* w_fp8 = nullptr;
+    float* w_scale = nullptr;
     int N = 0, K = 0;
     bool quad = false;
     // weight [N,K,1,1] (quad: [4N,K,1,1] in the reference ffn.0 order)
     void load(DevMem& mem, const HostDict& d, const std::string& prefix, bool quad_ = false);
     void load_raw(DevMem& mem, const std::vector<float>& wt, const std::vector<float>& bias, int N_, int K_);
+    // bf16 and fp8 (per-row scale) copies of the fp16 weight, on the null stream
+    void make_low_precision(DevMem& mem);
     void run(const Exec& e, const Act& a, const Act& dst, int epi, const Act* s1 = nullptr,
              const Act* s2 = nullptr, const QuantSel* q = nullptr) const;
 };

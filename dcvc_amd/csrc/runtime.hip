@@ -58,6 +58,16 @@ void Conv1x1::load_raw(DevMem& mem, const std::vector<float>& wt, const std::vec
     K = K_;
     w = upload_f16(mem, wt);
     b = upload_f32(mem, bias);
+    make_low_precision(mem);
+}
+
+void Conv1x1::make_low_precision(DevMem& mem) {
+    const int rows = quad ? 4 * N : N;
+    w_bf16 = mem.alloc((size_t)rows * K * 2);
+    w_fp8 = mem.alloc((size_t)rows * K);
+    w_scale = mem.alloc_n<float>(rows);
+    hip_check(weights_to_bf16(w, (long)rows * K, w_bf16, nullptr), "weights to bf16");
+    hip_check(weights_to_fp8(w, rows, K, w_fp8, w_scale, nullptr), "weights to fp8");
 }
 
 void Conv1x1::load(DevMem& mem, const HostDict& d, const std::string& prefix, bool quad_) {
@@ -74,6 +84,7 @@ void Conv1x1::load(DevMem& mem, const HostDict& d, const std::string& prefix, bo
     }
     w = upload_f16(mem, wt.data);
     b = upload_f32(mem, bias);
+    make_low_precision(mem);
 }
 
 void Conv1x1::run(const Exec& e, const Act& a, const Act& dst, int epi, const Act* s1, const Act* s2,
@@ -90,6 +101,13 @@ void Conv1x1::run(const Exec& e, const Act& a, const Act& dst, int epi, const Ac
     g.ldd = dst.ld;
     g.outH = a.H;
     g.outW = a.W;
+    g.prec = e.prec;
+    if (e.prec == PREC_BF16) {
+        g.B = static_cast<const f16*>(w_bf16);
+    } else if (e.prec == PREC_FP8) {
+        g.B = static_cast<const f16*>(w_fp8);
+        g.wscale = w_scale;
+    }
     if (s1) {
         g.S1 = s1->p;
         g.lds1 = s1->ld;

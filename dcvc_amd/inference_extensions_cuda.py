@@ -22,6 +22,9 @@ import torch
 from . import _lib
 
 
+_PREC = {"fp16": 0, "bf16": 1, "fp8": 2}
+
+
 def _stream():
     return torch.cuda.current_stream().cuda_stream
 
@@ -98,6 +101,10 @@ class DMCIProxy:
     def set_use_graphs(self, on):
         _lib.call("dcvc_dmci_set_use_graphs", self._h, int(bool(on)))
 
+    def set_precision(self, prec):
+        """'fp16' (default), 'bf16' or 'fp8': precision of every 1x1-conv product."""
+        _lib.call("dcvc_dmci_set_precision", self._h, _PREC[prec])
+
     def _prepare(self, height, width):
         Hp, Wp = (height + 15) // 16 * 16, (width + 15) // 16 * 16
         if self._size != (Hp, Wp):
@@ -170,6 +177,10 @@ class DMCHTSProxy:
 
     def set_use_graphs(self, on):
         _lib.call("dcvc_hts_set_use_graphs", self._h, int(bool(on)))
+
+    def set_precision(self, prec):
+        """'fp16' (default), 'bf16' or 'fp8': precision of every 1x1-conv product."""
+        _lib.call("dcvc_hts_set_precision", self._h, _PREC[prec])
 
     def _prepare(self, Hp, Wp):
         if self._size != (Hp, Wp):

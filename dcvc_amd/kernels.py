@@ -57,6 +57,44 @@ def host_thresholds():
 
 # ----------------------------- GEMMs ---------------------------------------
 
+EPI_CODES = {"bias": 0, "wsilu": 1, "shortcut": 2, "shortcut2": 3, "shortcut_quant": 4, "quant": 5, "quad": 6}
+PRECISIONS = {"fp16": 0, "bf16": 1, "fp8": 2}
+
+
+def weights_to_lowp(w, prec):
+    """fp16 weights [rows, K] -> (bf16 [rows, K], None) or (fp8 e4m3 bytes
+    [rows, K] as uint8, fp32 per-row scales) on the device."""
+    _check(w)
+    w = w.contiguous()
+    rows, K = w.shape
+    p = PRECISIONS[prec]
+    if p == 1:
+        out = torch.empty((rows, K), dtype=torch.bfloat16, device=w.device)
+        _lib_call("dcvc_weights_to_lowp", _p(w), rows, K, p, _p(out), None, _stream())
+        return out, None
+    if p == 2:
+        out = torch.empty((rows, K), dtype=torch.uint8, device=w.device)
+        scale = torch.empty(rows, dtype=torch.float32, device=w.device)
+        _lib_call("dcvc_weights_to_lowp", _p(w), rows, K, p, _p(out), _p(scale), _stream())
+        return out, scale
+    raise ValueError(prec)
+
+
+def gemm_lowp(a, wq, scale, prec, bias, epi="bias", s1=None, s2=None, q=None, qp=None, out=None):
+    """conv1x1 with the product in `prec` ('fp16', 'bf16', 'fp8'): wq from
+    weights_to_lowp (or fp16 weights for 'fp16')."""
+    _check(a)
+    M, K = a.shape
+    N = wq.shape[0] // 4 if epi == "quad" else wq.shape[0]
+    if out is None:
+        out = torch.empty((M, N), dtype=torch.float16, device=a.device)
+    bias = bias.float().contiguous()
+    qs = 0 if q is None else (q.stride(0) if q.dim() == 2 else q.numel())
+    _lib_call("dcvc_gemm_lowp", _p(a), _ld(a), M, K, _p(wq.contiguous()), N, _p(scale), PRECISIONS[prec],
+              EPI_CODES[epi], _p(bias), _p(s1), _ld(s1) if s1 is not None else 0, _p(s2),
+              _ld(s2) if s2 is not None else 0, _p(q), qs, _p(qp), _p(out), _ld(out), _stream())
+    return out
+
 def conv1x1(a, w, bias, epi="bias", s1=None, s2=None, q=None, qp=None, out=None):
     """a [M,K] fp16, w [N,K] fp16 (or [4N,K] for 'quad'), bias [N] fp32."""
     _check(a)

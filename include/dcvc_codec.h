@@ -104,6 +104,13 @@ int dcvc_hts_decompress(void* h, const uint8_t* bits, long nbytes, int qp, int h
 /* Switch hipGraph capture on (default) or off (eager launches). */
 int dcvc_hts_set_use_graphs(void* h, int on);
 int dcvc_dmci_set_use_graphs(void* h, int on);
+/* Precision form (no reference counterpart; the reference runs FP16 only):
+ * every 1x1-conv product of the proxy in 0 FP16 (default), 1 BF16 or 2 FP8
+ * E4M3 with per-output-channel weight scales, FP32 accumulate.  Activations,
+ * depthwise, strided / sub-pixel convs and the quantise / index steps stay
+ * FP16 / FP32.  Encoder and decoder must use the same precision. */
+int dcvc_hts_set_precision(void* h, int prec);
+int dcvc_dmci_set_precision(void* h, int prec);
 /* Test hook: copies an internal buffer (0 y_hat|reduced, 1 y, 2 q_dec|scales|
  * means, 3 memory|feature, 4 x|ctx, 5 dense symbols, 6 z) to host memory. */
 int dcvc_hts_debug_copy(void* h, int what, void* host, long bytes);

@@ -74,6 +74,20 @@ int dcvc_conv1x1_bias_with_quant(const void* A, int lda, int M, int K, const voi
 int dcvc_conv1x1_bias_wsilu_chunk_add(const void* A, int lda, int M, int K, const void* W, int N,
                                       const float* bias, void* D, int ldd, void* stream);
 
+/* Low-precision 1x1-conv product (precision form; no reference counterpart).
+ * dcvc_weights_to_lowp converts fp16 weights [rows x K] on the device:
+ * prec 1 -> bf16 (Wq: rows*K*2 bytes, scale unused), prec 2 -> fp8 e4m3
+ * (Wq: rows*K bytes) with scale[r] = max|w[r,:]| / 448.  dcvc_gemm_lowp runs
+ * the product with epilogue `epi` (0 bias, 1 wsilu, 2 shortcut, 3 shortcut2,
+ * 4 shortcut_with_quant, 5 with_quant, 6 quad: N outputs from 4N rows) on
+ * fp16 activations converted to the operand precision (fp8 at scale 1,
+ * saturated at +-448); prec 0 takes fp16 weights.  Other arguments as the
+ * conv1x1 entry points; unused shortcut / quant pointers may be NULL. */
+int dcvc_weights_to_lowp(const void* W, int rows, int K, int prec, void* Wq, float* scale, void* stream);
+int dcvc_gemm_lowp(const void* A, int lda, int M, int K, const void* Wq, int N, const float* scale, int prec,
+                   int epi, const float* bias, const void* S1, int lds1, const void* S2, int lds2, const void* q,
+                   int qstride, const int* qp, void* D, int ldd, void* stream);
+
 /* conv_bias (cutlass/conv_bias.cu:131-151): k x k conv, stride s, pad (k-1)/2,
  * over n_img NHWC images of H x W x Cin.  (k, s) in {(2,2), (3,2), (3,1)}.
  * W is [N][k*k][Cin] (tap-major, i.e. OHWI).  Output [n_img, Ho, Wo, N]. */

@@ -1,0 +1,199 @@
+"""Precision form: BF16 and FP8 (E4M3, per-output-channel weight scales)
+1x1-conv products with FP32 accumulation, against float64 references of the
+same operand rounding, and the codec's encoder / decoder agreement in each
+precision."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import dcvc_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rng(seed):
+    return np.random.Generator(np.random.PCG64(seed))
+
+
+def f16(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a).astype(np.float16)).to(dev)
+
+
+def ulp16(x):
+    a = np.abs(np.asarray(x, np.float64))
+    return 2.0 ** (np.floor(np.log2(np.maximum(a, 2.0 ** -14))) - 10)
+
+
+def e4m3_decode(b):
+    """OCP FP8 E4M3 (e4m3fn) bytes -> float64: bias 7, no infinities,
+    S.1111.111 = NaN, subnormals (m / 8) * 2^-6."""
+    b = np.asarray(b, np.uint8).astype(np.int64)
+    s = np.where(b & 0x80, -1.0, 1.0)
+    e = (b >> 3) & 0xF
+    m = (b & 7).astype(np.float64)
+    v = np.where(e == 0, m / 8.0 * 2.0 ** -6, (1.0 + m / 8.0) * 2.0 ** (e - 7))
+    v = np.where((e == 15) & (m == 7), np.nan, v)
+    return s * v
+
+
+def e4m3_round(x):
+    """Nearest E4M3 value (ties to even mantissa), saturated at +-448."""
+    x = np.clip(np.asarray(x, np.float64), -448.0, 448.0)
+    a = np.abs(x)
+    step = 2.0 ** (np.floor(np.log2(np.maximum(a, 2.0 ** -6))) - 3)
+    return np.sign(x) * np.minimum(np.round(a / step) * step, 448.0)
+
+
+def test_fp8_weight_conversion_format(gpu):
+    """Row scale = max|w| / 448, the row maximum encodes as +-448 exactly, and
+    every byte decodes (as OCP E4M3) to the nearest E4M3 value of w / scale."""
+    from dcvc_amd import kernels
+    r = rng(1)
+    w = (r.standard_normal((96, 256)) * np.exp(r.uniform(-6, 2, (96, 1)))).astype(np.float16)
+    w[5] = 0
+    wq, scale = kernels.weights_to_lowp(f16(w, gpu), "fp8")
+    torch.cuda.synchronize()
+    q = wq.cpu().numpy()
+    sc = scale.cpu().numpy().astype(np.float64)
+    amax = np.abs(w.astype(np.float64)).max(1)
+    want_sc = np.where(amax > 0, (amax.astype(np.float32) / np.float32(448)).astype(np.float64), 1.0)
+    np.testing.assert_allclose(sc, want_sc, rtol=1e-7)
+    dec = e4m3_decode(q)
+    assert not np.isnan(dec).any()
+    nz = amax > 0
+    assert np.all(np.abs(dec[nz]).max(1) == 448.0)
+    x = w.astype(np.float64) / sc[:, None].astype(np.float32).astype(np.float64)
+    near = e4m3_round(x)
+    # a value exactly between two E4M3 neighbours may round either way after the fp32 divide
+    tie_ok = np.abs(dec - near) <= 2.0 ** (np.floor(np.log2(np.maximum(np.abs(x), 2.0 ** -6))) - 3)
+    assert np.mean(dec == near) > 0.999 and tie_ok.all()
+
+
+def test_bf16_weight_conversion_is_round_to_nearest_even(gpu):
+    from dcvc_amd import kernels
+    w = f16(rng(2).standard_normal((64, 512)), gpu)
+    wq, _ = kernels.weights_to_lowp(w, "bf16")
+    torch.cuda.synchronize()
+    assert torch.equal(wq, w.float().to(torch.bfloat16))
+
+
+def _product_case(M, K, N, epi, seed):
+    r = rng(seed)
+    a = r.standard_normal((M, K)).astype(np.float16)
+    nb = 4 * N if epi == "quad" else N
+    w = (r.standard_normal((nb, K)) / np.sqrt(K)).astype(np.float16)
+    b = (r.standard_normal(nb) * 0.1).astype(np.float32)
+    s1 = r.standard_normal((M, N)).astype(np.float16)
+    return a, w, b, s1
+
+
+@pytest.mark.parametrize("M,K,N", [(1000, 256, 192), (4096, 512, 256), (3000, 768, 128)])
+@pytest.mark.parametrize("epi", ["bias", "wsilu", "shortcut", "quad"])
+def test_bf16_product_matches_bf16_operand_reference(gpu, M, K, N, epi):
+    """The BF16 product equals the float64 product of the bf16-rounded operands
+    up to fp32 accumulation and the fp16 output rounding."""
+    from dcvc_amd import kernels
+    a, w, b, s1 = _product_case(M, K, N, epi, M + K)
+    wq, _ = kernels.weights_to_lowp(f16(w, gpu), "bf16")
+    out = kernels.gemm_lowp(f16(a, gpu), wq, None, "bf16", torch.from_numpy(b).to(gpu), epi=epi, s1=f16(s1, gpu))
+    torch.cuda.synchronize()
+    ab = torch.from_numpy(a).float().to(torch.bfloat16).double().numpy()
+    wb = torch.from_numpy(w).float().to(torch.bfloat16).double().numpy()
+    ref = O.ref_gemm_epilogue(ab, wb, b, epi, s1)
+    absacc = np.abs(ab) @ np.abs(wb).T
+    if epi == "quad":
+        absacc = absacc[:, 0::4] + absacc[:, 1::4] + absacc[:, 2::4] + absacc[:, 3::4]
+    err = np.abs(out.float().cpu().numpy().astype(np.float64) - ref)
+    assert (err <= ulp16(ref) + 2e-6 * absacc + 4e-6 * np.abs(ref) + 1e-7).all()
+
+
+@pytest.mark.parametrize("M,K,N", [(1000, 256, 192), (4096, 512, 256)])
+@pytest.mark.parametrize("epi", ["bias", "shortcut", "quad"])
+def test_fp8_product_matches_e4m3_operand_model(gpu, M, K, N, epi):
+    """The FP8 product equals the float64 product of E4M3-rounded activations
+    (scale 1) and E4M3 weights times their row scales, up to fp32
+    accumulation and fp16 output rounding; and its error against the exact
+    fp16-operand product stays within the E4M3 rounding model."""
+    from dcvc_amd import kernels
+    a, w, b, s1 = _product_case(M, K, N, epi, 7 * M + K)
+    wq, scale = kernels.weights_to_lowp(f16(w, gpu), "fp8")
+    out = kernels.gemm_lowp(f16(a, gpu), wq, scale, "fp8", torch.from_numpy(b).to(gpu), epi=epi, s1=f16(s1, gpu))
+    torch.cuda.synchronize()
+    got = out.float().cpu().numpy().astype(np.float64)
+    wdeq = e4m3_decode(wq.cpu().numpy()) * scale.cpu().numpy().astype(np.float64)[:, None]
+    aq = e4m3_round(a.astype(np.float64))
+    ref = O.ref_gemm_epilogue(aq, wdeq, b, epi, s1)
+    absacc = np.abs(aq) @ np.abs(wdeq).T
+    exact = O.ref_gemm_epilogue(a, w, b, epi, s1)
+    absx = np.abs(a.astype(np.float64)) @ np.abs(w.astype(np.float64)).T
+    if epi == "quad":
+        absacc = absacc[:, 0::4] + absacc[:, 1::4] + absacc[:, 2::4] + absacc[:, 3::4]
+        absx = absx[:, 0::4] + absx[:, 1::4] + absx[:, 2::4] + absx[:, 3::4]
+    ok = np.abs(got - ref) <= ulp16(ref) + 4e-6 * absacc + 4e-6 * np.abs(ref) + 1e-7
+    # activation values halfway between E4M3 neighbours may round the other way
+    assert ok.mean() > 0.995
+    # E4M3 relative rounding <= 2^-4 on each operand, plus the subnormal floor
+    assert (np.abs(got - exact) <= ulp16(exact) + 0.14 * absx + 0.05).all()
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp8"])
+@pytest.mark.parametrize("epi", ["wsilu", "shortcut", "quad"])
+def test_lowp_tile_configs_bit_identical(gpu, prec, epi):
+    from dcvc_amd import _lib, kernels
+    lib = _lib.load()
+    a, w, b, s1 = _product_case(2999, 512, 200, epi, 3)
+    wq, scale = kernels.weights_to_lowp(f16(w, gpu), prec)
+    outs = []
+    try:
+        for policy in (1, 2, 3, 4, 5):
+            assert lib.dcvc_gemm_set_tile_policy(policy) == 0
+            outs.append(kernels.gemm_lowp(f16(a, gpu), wq, scale, prec, torch.from_numpy(b).to(gpu), epi=epi,
+                                          s1=f16(s1, gpu)))
+    finally:
+        lib.dcvc_gemm_set_tile_policy(0)
+    torch.cuda.synchronize()
+    for o in outs[1:]:
+        assert torch.equal(o.view(torch.int16), outs[0].view(torch.int16))
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp8"])
+def test_hts_precision_form_encoder_decoder_bitexact(gpu, prec):
+    """In each precision the decoder still rebuilds the encoder's y_hat and
+    temporal context bit for bit; PSNR against the fp16 path is reported."""
+    from dcvc_amd import entropy, synthetic, weights
+    from dcvc_amd.inference_extensions_cuda import DMCIProxy, DMCHTSProxy
+    H = W = 128
+    sdi = entropy.add_cdf_to_state_dict(weights.to_torch(weights.round_to_half(weights.init_state_dict("dmci", 0))))
+    sdp = entropy.add_cdf_to_state_dict(weights.to_torch(weights.round_to_half(weights.init_state_dict("hts", 1))))
+    frames = synthetic.motion_sequence_yuv420(H, W, 9, seed=6)
+    to = lambda arr: torch.from_numpy(arr)[None].half().to(gpu).contiguous(memory_format=torch.channels_last)
+    i_enc = DMCIProxy()
+    i_enc.set_param(sdi, 0.0)
+    _, x_hat_i, _ = i_enc.compress(to(synthetic.frame_to_input(frames[0])), 32, 0, 0)
+    chunk = to(synthetic.chunk_to_input(frames[1:9]))
+    src = chunk.float()
+    psnr = {}
+    for p in ("fp16", prec):
+        enc, dec = DMCHTSProxy(), DMCHTSProxy()
+        enc.set_param(sdp, 0.15)
+        dec.set_param(sdp, 0.15)
+        enc.set_precision(p)
+        dec.set_precision(p)
+        enc.add_ref_feature_from_frame(x_hat_i, True)
+        dec.add_ref_feature_from_frame(x_hat_i, False)
+        P16, P8 = (H // 16) * (W // 16), (H // 8) * (W // 8)
+        bits, ec = enc.compress(chunk, 32, False, 0, 0)
+        yh_e = enc.debug_copy(0, P16 * 512 * 2, np.float16).reshape(P16, 512)[:, :256].copy()
+        outs = dec.decompress(bits, 32, H, W, ec, False)
+        torch.cuda.synchronize()
+        yh_d = dec.debug_copy(0, P16 * 512 * 2, np.float16).reshape(P16, 512)[:, :256]
+        assert np.array_equal(yh_e.view(np.uint16), yh_d.view(np.uint16)), p
+        fe = enc.debug_copy(3, P8 * 1024 * 2, np.float16).reshape(P8, 1024)[:, 512:]
+        fd = dec.debug_copy(3, P8 * 1024 * 2, np.float16).reshape(P8, 1024)[:, 512:]
+        assert np.array_equal(fe.view(np.uint16), fd.view(np.uint16)), p
+        rec = torch.cat([o.float() for o in outs], 1)
+        assert torch.isfinite(rec).all()
+        mse = float(((rec - src) ** 2).mean())
+        psnr[p] = 10 * np.log10(1.0 / max(mse, 1e-12))
+        print(f"{p}: {len(bits)} bytes, PSNR {psnr[p]:.3f} dB")
+    assert abs(psnr[prec] - psnr["fp16"]) < 3.0
