@@ -195,6 +195,51 @@ def precision_forms(sdp, p_enc_ref, x_hat_i, chunks, H, W, pad_b, pad_r, skip, d
     return out
 
 
+def batching_form(sdi, sdp, rounds=2):
+    """16 concurrent streams of uneven resolution (4x 854x480, 6x 1280x720,
+    4x 1920x1080, 2x 3840x2160), one 4-frame chunk each per round at a random
+    qp: one stream at a time vs all at once."""
+    import torch
+    from dcvc_amd import batching
+    setup, _ = batching.build_streams(sdi, sdp, batching.MIX_16, n_chunks=rounds + 1, seed=0)
+    n = len(setup.contexts)
+    frames = 4 * n * rounds
+    batching.code_concurrently(setup.contexts, [c[0] for c in setup.chunks], setup.qps)  # warm-up, graph capture
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r in range(rounds):
+        batching.code_sequentially(setup.contexts, [c[1 + r] for c in setup.chunks], setup.qps)
+    t_seq = time.perf_counter() - t0
+    setup2, _ = batching.build_streams(sdi, sdp, batching.MIX_16, n_chunks=rounds + 1, seed=0)
+    batching.code_concurrently(setup2.contexts, [c[0] for c in setup2.chunks], setup2.qps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r in range(rounds):
+        batching.code_concurrently(setup2.contexts, [c[1 + r] for c in setup2.chunks], setup2.qps)
+    t_con = time.perf_counter() - t0
+    return {"streams": "4x 854x480, 6x 1280x720, 4x 1920x1080, 2x 3840x2160", "rounds": rounds,
+            "frames_counted": frames, "sequential_frames_per_s": round(frames / t_seq, 2),
+            "concurrent_frames_per_s": round(frames / t_con, 2), "speedup": round(t_seq / t_con, 3),
+            "precision": "fp16"}
+
+
+def serving_form(sdi, sdp, duration=10.0, rate=120.0, n_streams=64, slots=8):
+    """Poisson arrivals (encode or decode of one 4-frame chunk of one of 64
+    persistent streams, resolutions 480p 30 %, 720p 40 %, 1080p 25 %,
+    2160p 5 %) served by the continuous-batching loop, HBM budget 200 GB."""
+    from dcvc_amd import serving
+    res = serving.resolution_mix(n_streams, seed=0)
+    ex = serving.GpuExecutor(sdi, sdp, res, chunks_per_stream=4, seed=0)
+    reqs = serving.poisson_trace(rate, duration, n_streams, seed=0)
+    srv = serving.Server(ex, slots=slots, hbm_budget=200 << 30)
+    srv.run(reqs, realtime=True)
+    st = serving.Server.stats(reqs)
+    st.update({"rate_per_s": rate, "duration_s": duration, "streams": n_streams, "slots": slots,
+               "resolution_mix": serving.MIX_64, "max_running": srv.max_running,
+               "resident_gb": round(srv.max_resident_bytes / 2 ** 30, 2)})
+    return st
+
+
 def cpu_baseline(budget_s=20.0):
     """FP32 CPU oracle on this host: HT-S encode (+context update) and decode
     of a cropped sample of the same synthetic chunk, frames/s scaled by area
@@ -289,6 +334,9 @@ def main():
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--profile-step", action="store_true")
     ap.add_argument("--no-precision-forms", action="store_true")
+    ap.add_argument("--no-batching", action="store_true")
+    ap.add_argument("--serving-seconds", type=float, default=10.0,
+                    help="length of the Poisson serving trace (0 skips it)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -398,6 +446,12 @@ def main():
         log("precision forms")
         out["precision_forms"] = precision_forms(sdp, p_enc, enc_i["x_hat"], chunks, H, W, pad_b, pad_r,
                                                  args.skip_thres, dev)
+    if not args.no_batching:
+        log("batching form")
+        out["batching_form"] = batching_form(sdi, sdp)
+    if args.serving_seconds > 0:
+        log("serving form")
+        out["serving_form"] = serving_form(sdi, sdp, duration=args.serving_seconds)
     if not args.no_cpu_baseline:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline()

@@ -89,6 +89,7 @@ SIGNATURES = {
     "dcvc_hts_decompress": [P, P, L, I, I, I, I, I, P],
     "dcvc_hts_set_use_graphs": [P, I],
     "dcvc_hts_set_precision": [P, I],
+    "dcvc_hts_device_bytes": [P],
     "dcvc_hts_debug_copy": [P, I, P, L],
     "dcvc_hts_last_symbol_count": [P],
     "dcvc_trace_begin": [I],
@@ -98,6 +99,7 @@ _RESTYPE = {"dcvc_last_error": C.c_char_p, "dcvc_rans_encoder_create": C.c_void_
             "dcvc_rans_decoder_create": C.c_void_p, "dcvc_rans_encoder_destroy": None,
             "dcvc_rans_decoder_destroy": None, "dcvc_dmci_create": C.c_void_p, "dcvc_hts_create": C.c_void_p,
             "dcvc_dmci_destroy": None, "dcvc_hts_destroy": None, "dcvc_hts_last_symbol_count": C.c_long,
+            "dcvc_hts_device_bytes": C.c_long,
             "dcvc_dmci_step_symbol_count": C.c_long}
 
 

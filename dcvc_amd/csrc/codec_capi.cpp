@@ -146,6 +146,8 @@ int dcvc_hts_set_use_graphs(void* h, int on) {
     return guarded([&] { checked<HTSProxy>(h)->set_use_graphs(on != 0); });
 }
 
+long dcvc_hts_device_bytes(void* h) { return h ? (long)static_cast<HTSProxy*>(h)->device_bytes() : -1; }
+
 int dcvc_hts_set_precision(void* h, int prec) {
     return guarded([&] { checked<HTSProxy>(h)->set_precision(prec); });
 }

@@ -585,13 +585,11 @@ static hipError_t launch_tile(GemmArgs g, hipStream_t stream) {
     const int tiles_m = ceil_div(g.M, BM);
     const int tiles_n = ceil_div(g.N, T::BNO);
     auto kern = gemm_kernel<T>;
-    static bool attr_set = false;  // set once per instantiation (host-side, before first launch)
-    if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)T::kLdsBytes);
-        if (e != hipSuccess) return e;
-        attr_set = true;
-    }
+    // once per instantiation; C++ static initialisation is thread-safe, so
+    // concurrent first launches from several host threads are fine
+    static const hipError_t attr = hipFuncSetAttribute((const void*)kern,
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)T::kLdsBytes);
+    if (attr != hipSuccess) return attr;
     hipLaunchKernelGGL(kern, dim3(tiles_m * tiles_n), dim3(T::kThreads), T::kLdsBytes, stream, g, tiles_n);
     return hipGetLastError();
 }
@@ -618,7 +616,8 @@ static hipError_t dispatch_tiles(const GemmArgs& g, hipStream_t stream) {
         // shortcut epilogues (HBM-bound) stay on 128 x 128, whose registers
         // also hold the shortcut tile.
         constexpr bool kShort = (EPI == EPI_SHORTCUT || EPI == EPI_SHORTCUT2 || EPI == EPI_SHORTCUT_QUANT);
-        const bool big = !kShort && (t == 1 || t == 2 || t == 4);
+        // measured on the 1080p shapes: 256 x 256 pays off only for K >= 1024
+        const bool big = !kShort && (g_tile_policy != 0 ? (t == 1 || t == 2) : g.K >= 1024);
         if constexpr (EPI == EPI_QUAD) {
             if (big) return launch_tile<256, 256, 2, 2, 3, EPI, AMODE, DMODE, PREC>(g, stream);
             return launch_tile<128, 128, 4, 1, 3, EPI, AMODE, DMODE, PREC>(g, stream);

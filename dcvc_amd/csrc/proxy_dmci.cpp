@@ -388,7 +388,8 @@ void DMCIProxy::copy_debug(int what, void* host, long bytes) {
         throw std::invalid_argument("unknown debug buffer");
     }
     if (bytes > cap) throw std::invalid_argument("debug copy too large");
-    hip_check(hipMemcpy(host, src, bytes, hipMemcpyDeviceToHost), "debug copy");
+    hip_check(hipMemcpyAsync(host, src, bytes, hipMemcpyDeviceToHost, m_stream), "debug copy");
+    hip_check(hipStreamSynchronize(m_stream), "debug copy");
 }
 
 }  // namespace dcvc

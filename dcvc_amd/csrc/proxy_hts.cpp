@@ -190,8 +190,11 @@ void HTSProxy::prepare(int Hp, int Wp, f16* const frames[kFrames], int ld_out) {
     m_cond = M.alloc_n<uint8_t>(m_ny);
     m_idx = M.alloc_n<uint8_t>(m_ny);
     m_idxc = M.alloc_n<uint8_t>(m_ny);
-    hip_check(hipMemset(m_cat_enc, 0, P8 * 2048 * sizeof(f16)), "memset");
-    hip_check(hipMemset(m_cat_fam, 0, P8 * 1024 * sizeof(f16)), "memset");
+    // on the proxy's own stream: a legacy-stream operation here would break a
+    // graph capture running on another thread's proxy
+    hip_check(hipMemsetAsync(m_cat_enc, 0, P8 * 2048 * sizeof(f16), m_stream), "memset");
+    hip_check(hipMemsetAsync(m_cat_fam, 0, P8 * 1024 * sizeof(f16), m_stream), "memset");
+    hip_check(hipStreamSynchronize(m_stream), "memset");
     hip_check(hipHostMalloc((void**)&m_h_count, sizeof(int)), "pinned");
     hip_check(hipHostMalloc((void**)&m_h_sym, m_ny * sizeof(int16_t)), "pinned");
     hip_check(hipHostMalloc((void**)&m_h_idx, m_ny), "pinned");
@@ -499,7 +502,8 @@ void HTSProxy::copy_debug(int what, void* host, long bytes) {
     default: throw std::invalid_argument("unknown debug buffer");
     }
     if (bytes > cap) throw std::invalid_argument("debug copy too large");
-    hip_check(hipMemcpy(host, src, bytes, hipMemcpyDeviceToHost), "debug copy");
+    hip_check(hipMemcpyAsync(host, src, bytes, hipMemcpyDeviceToHost, m_stream), "debug copy");
+    hip_check(hipStreamSynchronize(m_stream), "debug copy");
 }
 
 }  // namespace dcvc

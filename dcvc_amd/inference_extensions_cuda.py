@@ -182,6 +182,10 @@ class DMCHTSProxy:
         """'fp16' (default), 'bf16' or 'fp8': precision of every 1x1-conv product."""
         _lib.call("dcvc_hts_set_precision", self._h, _PREC[prec])
 
+    def device_bytes(self):
+        """HBM held by this context's activation buffers (weights excluded)."""
+        return int(self._lib.dcvc_hts_device_bytes(self._h))
+
     def _prepare(self, Hp, Wp):
         if self._size != (Hp, Wp):
             self._frames = [torch.empty((1, 3, Hp, Wp), dtype=torch.float16, device=self._device,

@@ -62,7 +62,11 @@ int dcvc_rans_decoder_decode_y(void* dec, const uint8_t* indexes, long n, int8_t
  * bit_estimator_z.{quantized_cdf,cdf_length}; skip_thres as in the reference.
  * Pictures are NHWC fp16 device buffers (pixel stride ld), values in
  * [-0.5, 0.5].  `stream` is the caller's hipStream_t; the proxy orders its
- * own stream after it and the caller's stream after the proxy's work. */
+ * own stream after it and the caller's stream after the proxy's work.
+ * Threading: one proxy is not re-entrant; different proxies may run
+ * concurrently from different host threads (each captures its graphs on its
+ * own stream), except set_param, which uploads on the legacy stream and
+ * must not overlap other proxies' calls. */
 
 /* DMCIProxy (dmci_proxy.h:132-254) */
 void* dcvc_dmci_create(void);
@@ -110,6 +114,10 @@ int dcvc_dmci_set_use_graphs(void* h, int on);
  * depthwise, strided / sub-pixel convs and the quantise / index steps stay
  * FP16 / FP32.  Encoder and decoder must use the same precision. */
 int dcvc_hts_set_precision(void* h, int prec);
+/* Device bytes held by the proxy's activation buffers (its resident stream
+ * context; weights excluded).  No reference counterpart: used for admission
+ * by HBM footprint. */
+long dcvc_hts_device_bytes(void* h);
 int dcvc_dmci_set_precision(void* h, int prec);
 /* Test hook: copies an internal buffer (0 y_hat|reduced, 1 y, 2 q_dec|scales|
  * means, 3 memory|feature, 4 x|ctx, 5 dense symbols, 6 z) to host memory. */
