@@ -26,12 +26,14 @@ SIGNATURES = {
     "dcvc_conv1x1_bias_shortcut": [P, I, I, I, P, I, P, P, I, P, I, P],
     "dcvc_conv1x1_bias_shortcut2": [P, I, I, I, P, I, P, P, I, P, I, P, I, P],
     "dcvc_conv1x1_bias_shortcut_with_quant": [P, I, I, I, P, I, P, P, I, P, I, P, P, I, P],
+    "dcvc_conv1x1_bias_shortcut_then_quant": [P, I, I, I, P, I, P, P, I, P, I, P, P, I, P],
     "dcvc_conv1x1_bias_with_quant": [P, I, I, I, P, I, P, P, I, P, P, I, P],
     "dcvc_conv1x1_bias_wsilu_chunk_add": [P, I, I, I, P, I, P, P, I, P],
     "dcvc_conv_bias": [P, I, I, I, I, I, P, I, P, I, I, P, I, P],
     "dcvc_weights_to_lowp": [P, I, I, I, P, P, P],
     "dcvc_gemm_lowp": [P, I, I, I, P, I, P, I, I, P, P, I, P, I, P, I, P, P, I, P],
     "dcvc_transposed_conv": [P, I, I, I, I, I, P, I, P, P, I, P],
+    "dcvc_subpel_conv3x3": [P, I, I, I, I, I, P, I, P, P, I, P],
     "dcvc_d3x3": [P, I, I, I, I, I, P, P, I, P],
     "dcvc_pad_and_unshuffle": [P, I, I, I, I, I, I, I, I, P, I, P],
     "dcvc_pixel_shuffle_8": [P, I, I, I, I, I, P, I, P],
@@ -45,6 +47,8 @@ SIGNATURES = {
     "dcvc_intra_quant_step": [P, I, I, I, I, I, P, I, P, I, P, I, P, I, P, P, I, P, P, P, F, I, P],
     "dcvc_intra_build_index_dec": [P, I, I, I, I, I, P, F, I, P, P, P],
     "dcvc_intra_restore_step": [P, I, I, I, I, P, I, P, I, P, I, P, P, I, I, P],
+    "dcvc_htl_quant_step": [P, I, I, I, I, I, P, I, P, I, P, I, P, I, P, I, P, P, P, F, I, P],
+    "dcvc_htl_restore_step": [P, I, I, I, I, P, I, P, I, P, I, P, I, I, P],
     "dcvc_compact_blocks": [L],
     "dcvc_conditional_index": [P, I, P, L, P, P, P, P],
     "dcvc_conditional_recover": [P, P, L, P, P, P],
@@ -92,6 +96,19 @@ SIGNATURES = {
     "dcvc_hts_device_bytes": [P],
     "dcvc_hts_debug_copy": [P, I, P, L],
     "dcvc_hts_last_symbol_count": [P],
+    "dcvc_htl_create": [],
+    "dcvc_htl_destroy": [P],
+    "dcvc_htl_set_param": [P, I, P, P, P, P, F],
+    "dcvc_htl_share_weights": [P, P],
+    "dcvc_htl_prepare": [P, I, I, P, I],
+    "dcvc_htl_add_ref_feature_from_frame": [P, P, I, I, I, I, P],
+    "dcvc_htl_compress": [P, P, I, I, I, I, I, I, I, P, P, P, P],
+    "dcvc_htl_decompress": [P, P, L, I, I, I, I, I, P],
+    "dcvc_htl_set_use_graphs": [P, I],
+    "dcvc_htl_set_precision": [P, I],
+    "dcvc_htl_device_bytes": [P],
+    "dcvc_htl_debug_copy": [P, I, P, L],
+    "dcvc_htl_step_symbol_count": [P, I],
     "dcvc_trace_begin": [I],
     "dcvc_trace_end": [P, I],
 }
@@ -100,7 +117,8 @@ _RESTYPE = {"dcvc_last_error": C.c_char_p, "dcvc_rans_encoder_create": C.c_void_
             "dcvc_rans_decoder_destroy": None, "dcvc_dmci_create": C.c_void_p, "dcvc_hts_create": C.c_void_p,
             "dcvc_dmci_destroy": None, "dcvc_hts_destroy": None, "dcvc_hts_last_symbol_count": C.c_long,
             "dcvc_hts_device_bytes": C.c_long,
-            "dcvc_dmci_step_symbol_count": C.c_long}
+            "dcvc_dmci_step_symbol_count": C.c_long, "dcvc_htl_create": C.c_void_p, "dcvc_htl_destroy": None,
+            "dcvc_htl_device_bytes": C.c_long, "dcvc_htl_step_symbol_count": C.c_long}
 
 
 def declared_symbols():

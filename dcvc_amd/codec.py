@@ -1,6 +1,6 @@
 """Host-side codec objects and the sequence loop around the proxies.
 
-``IntraCodec`` / ``VideoCodecHTS`` mirror the inference interface of the
+``IntraCodec`` / ``VideoCodecHTS`` / ``VideoCodecHTL`` mirror the inference interface of the
 reference model classes (DMCI.compress/decompress, image_model.py:194-217;
 DMC.add_ref_feature_from_frame/compress/decompress, video_model_ht.py:
 413-450) over a state dict with the reference's keys.  ``encode_sequence``
@@ -20,15 +20,16 @@ import torch
 import torch.nn.functional as F
 
 from . import entropy, stream_helper, synthetic, weights
-from .inference_extensions_cuda import DMCHTSProxy, DMCIProxy
+from .inference_extensions_cuda import DMCHTLProxy, DMCHTSProxy, DMCIProxy
 
 FRAME_DELAY = 8
 
 
-def make_state_dicts(seed=0, perturb_bias=False):
-    """(intra, hts) state dicts: seeded init, fp16-rounded, plus CDF tables."""
+def make_state_dicts(seed=0, perturb_bias=False, video="hts"):
+    """(intra, video) state dicts: seeded init, fp16-rounded, plus CDF tables.
+    ``video`` is the chunk model structure, 'hts' or 'htl'."""
     out = []
-    for i, model in enumerate(("dmci", "hts")):
+    for i, model in enumerate(("dmci", video)):
         sd = weights.init_state_dict(model, seed + i)
         if perturb_bias:
             sd = weights.perturb_biases(sd, seed + 100 + i)
@@ -59,8 +60,10 @@ class IntraCodec:
 
 
 class VideoCodecHTS:
+    PROXY = DMCHTSProxy
+
     def __init__(self, state_dict=None, skip_thres=0.0, share_from: "VideoCodecHTS | None" = None):
-        self.proxy = DMCHTSProxy()
+        self.proxy = self.PROXY()
         if share_from is not None:
             self.proxy.share_weights_from(share_from.proxy)
         else:
@@ -80,6 +83,16 @@ class VideoCodecHTS:
         x_hat = self.proxy.decompress(np.frombuffer(bit_stream, dtype=np.uint8), qp, sps["height"], sps["width"],
                                       ec_part, reset_feature_memory)
         return {"x_hat": x_hat}
+
+
+class VideoCodecHTL(VideoCodecHTS):
+    """DMC with ModelStructure.HTL (video_model_ht.py:413-450 select DMCHTLProxy)."""
+    PROXY = DMCHTLProxy
+
+
+def video_codec(structure: str):
+    """'hts' or 'htl' -> the codec class (video_model_ht.py:413-424)."""
+    return {"hts": VideoCodecHTS, "htl": VideoCodecHTL}[structure]
 
 
 def calc_psnr(a, b, data_range=255.0):

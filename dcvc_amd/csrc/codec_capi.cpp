@@ -3,6 +3,7 @@
 #include "capi_util.h"
 #include "proxy_dmci.h"
 #include "proxy_hts.h"
+#include "proxy_htl.h"
 
 using namespace dcvc;
 
@@ -160,6 +161,83 @@ int dcvc_hts_debug_copy(void* h, int what, void* host, long bytes) {
 }
 
 long dcvc_hts_last_symbol_count(void* h) { return h ? static_cast<HTSProxy*>(h)->last_symbol_count() : -1; }
+
+void* dcvc_htl_create(void) { return create<HTLProxy>(); }
+void dcvc_htl_destroy(void* h) { delete static_cast<HTLProxy*>(h); }
+
+int dcvc_htl_set_param(void* h, int n, const char** names, const float** data, const int64_t* shapes,
+                       const int* ndims, float skip_thres) {
+    return guarded([&] { checked<HTLProxy>(h)->set_param(make_dict(n, names, data, shapes, ndims), skip_thres); });
+}
+
+int dcvc_htl_share_weights(void* h, void* src) {
+    return guarded([&] {
+        auto w = checked<HTLProxy>(src)->weights();
+        if (!w) throw std::logic_error("source proxy has no parameters");
+        checked<HTLProxy>(h)->share_weights(w);
+    });
+}
+
+int dcvc_htl_prepare(void* h, int Hp, int Wp, void* const* frames, int ld_out) {
+    return guarded([&] {
+        if (!frames || ld_out < 3) throw std::invalid_argument("bad output buffers");
+        f16* f[HTLProxy::kFrames];
+        for (int i = 0; i < HTLProxy::kFrames; ++i) {
+            if (!frames[i]) throw std::invalid_argument("null output frame");
+            f[i] = static_cast<f16*>(frames[i]);
+        }
+        checked<HTLProxy>(h)->prepare(Hp, Wp, f, ld_out);
+    });
+}
+
+int dcvc_htl_add_ref_feature_from_frame(void* h, const void* frame, int ld, int Hp, int Wp, int apply_adaptor,
+                                        void* stream) {
+    return guarded([&] {
+        if (!frame || ld < 3) throw std::invalid_argument("bad frame");
+        checked<HTLProxy>(h)->add_ref_feature_from_frame(static_cast<const f16*>(frame), ld, Hp, Wp, apply_adaptor != 0,
+                                                         S(stream));
+    });
+}
+
+int dcvc_htl_compress(void* h, const void* x, int ldx, int H, int W, int qp, int reset_feature_memory, int pad_b,
+                      int pad_r, void* stream, const uint8_t** bits, long* nbytes, int* ec_parallel) {
+    return guarded([&] {
+        if (!x || !bits || !nbytes || !ec_parallel) throw std::invalid_argument("null argument");
+        *ec_parallel = checked<HTLProxy>(h)->compress(static_cast<const f16*>(x), ldx, H, W, qp,
+                                                      reset_feature_memory != 0, pad_b, pad_r, S(stream), bits, nbytes);
+    });
+}
+
+int dcvc_htl_decompress(void* h, const uint8_t* bits, long nbytes, int qp, int height, int width, int ec_parallel,
+                        int reset_feature_memory, void* stream) {
+    return guarded([&] {
+        if (!bits && nbytes > 0) throw std::invalid_argument("null bitstream");
+        checked<HTLProxy>(h)->decompress(bits, nbytes, qp, height, width, ec_parallel, reset_feature_memory != 0,
+                                         S(stream));
+    });
+}
+
+int dcvc_htl_set_use_graphs(void* h, int on) {
+    return guarded([&] { checked<HTLProxy>(h)->set_use_graphs(on != 0); });
+}
+
+int dcvc_htl_set_precision(void* h, int prec) {
+    return guarded([&] { checked<HTLProxy>(h)->set_precision(prec); });
+}
+
+long dcvc_htl_device_bytes(void* h) { return h ? (long)static_cast<HTLProxy*>(h)->device_bytes() : -1; }
+
+int dcvc_htl_debug_copy(void* h, int what, void* host, long bytes) {
+    return guarded([&] {
+        if (!host || bytes < 0) throw std::invalid_argument("bad host buffer");
+        checked<HTLProxy>(h)->copy_debug(what, host, bytes);
+    });
+}
+
+long dcvc_htl_step_symbol_count(void* h, int step) {
+    if (!h || step < 0 || step > 3) return -1;
+    return static_cast<HTLProxy*>(h)->step_count(step);
+}
 
 int dcvc_trace_begin(int timed) {
     return guarded([&] { trace_set(true, timed != 0); });

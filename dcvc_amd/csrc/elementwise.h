@@ -51,6 +51,15 @@ hipError_t intra_dec_step(const Act& geom, const int8_t* yq, const f16* means, i
                           int ldsf, const f16* q, int qstride, const int* qp, f16* yout, int ldo,
                           int step, hipStream_t s);
 
+// HT-L (four entropy round trips over quarter tensors, elementwise
+// quantisation step max(q_dec, 0.5) from the common params).  The decoder
+// side builds its indexes with intra_dec_index.
+hipError_t htl_enc_step(const Act& y, const f16* scales, int lds, const f16* means, int ldm, const f16* qdec, int ldq,
+                        f16* ysf, int ldsf, f16* yout, int ldo, int16_t* sym, uint8_t* cond, const float* th, float thr,
+                        int step, hipStream_t s);
+hipError_t htl_dec_step(const Act& geom, const int8_t* yq, const f16* means, int ldm, const f16* qdec, int ldq,
+                        f16* ysf, int ldsf, f16* yout, int ldo, int step, hipStream_t s);
+
 // Stable stream compaction by a 0/1 byte condition (element size 1 or 2).
 // `block_offsets` needs compact_blocks(N) ints and stays valid for recover.
 constexpr int COMPACT_TILE = 4096;

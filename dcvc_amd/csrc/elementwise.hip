@@ -299,16 +299,24 @@ hipError_t hts_dec_step(const Act& g, const int8_t* yq, const f16* means, int ld
 }
 
 // ---------------------------------------------------------------------------
-// DMCI quantisation boundary over quarter tensors (dmci_proxy.cpp:333-393, 497-597)
+// DMCI and HT-L quantisation boundary over quarter tensors
+// (dmci_proxy.cpp:333-393, 497-597; dmc_htl_proxy.cpp:614-688, 744-903)
 // Quarter element (pixel m, channel c' < C/4) maps to channel g*C/4 + c' of the
 // one group g active at the pixel's 2x2 position in step k
 // (single_part_for_writing/reading_4x, stream.cu:896-962).
 // ---------------------------------------------------------------------------
-__global__ void intra_enc_step_kernel(Act y, const f16* __restrict__ scales, int lds, const f16* __restrict__ means,
-                                      int ldm, f16* __restrict__ ysf, int ldsf, const f16* __restrict__ q,
-                                      int qstride, const int* __restrict__ qp, f16* __restrict__ yout, int ldo,
-                                      int16_t* __restrict__ sym, uint8_t* __restrict__ cond,
-                                      const float* __restrict__ th, float thr, int step) {
+// kElemQ = false: DMCI, y enters as is and y_hat is scaled by the per-QP
+// channel vector q[qp] at the end.  kElemQ = true: HT-L, the quantisation
+// step is elementwise, qs = max(q_dec, 0.5) from the common params (q at
+// ldq): y / qs enters the step and y_hat * qs leaves it
+// (divide_with_clamp_min / add_and_multiply_with_clamp_min, stream.cu:40-66,
+// 422-443).
+template <bool kElemQ>
+__global__ void quarter_enc_step_kernel(Act y, const f16* __restrict__ scales, int lds, const f16* __restrict__ means,
+                                        int ldm, f16* __restrict__ ysf, int ldsf, const f16* __restrict__ q, int ldq,
+                                        const int* __restrict__ qp, f16* __restrict__ yout, int ldo,
+                                        int16_t* __restrict__ sym, uint8_t* __restrict__ cond,
+                                        const float* __restrict__ th, float thr, int step) {
     const int C4 = y.C >> 2;
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= y.pixels() * C4) return;
@@ -321,7 +329,9 @@ __global__ void intra_enc_step_kernel(Act y, const f16* __restrict__ scales, int
     const int ca = ga * C4 + cq;
     const float mean = (float)means[m * ldm + ca];
     const float sc = (float)scales[m * lds + ca];
-    const int yq = quant_elem((float)y.p[m * y.ld + ca], mean, sc, thr);
+    float yv = (float)y.p[m * y.ld + ca];
+    if constexpr (kElemQ) yv = yv * (1.0f / qstep((float)q[m * ldq + ca]));
+    const int yq = quant_elem(yv, mean, sc, thr);
     sym[i] = (int16_t)((yq << 8) + scale_index(sc, th));
     cond[i] = sc > thr ? 1 : 0;
     const f16 yh = yhat_elem(yq, mean);
@@ -333,12 +343,13 @@ __global__ void intra_enc_step_kernel(Act y, const f16* __restrict__ scales, int
             ysf[m * ldsf + ca] = yh;
         }
     } else {
-        const f16* qr = q + (size_t)(qp ? *qp : 0) * qstride;
+        const f16* qr = kElemQ ? q + m * ldq : q + (size_t)(qp ? *qp : 0) * ldq;
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             const int c = g * C4 + cq;
             const float v = (g == ga) ? (float)yh : (float)ysf[m * ldsf + c];
-            yout[m * ldo + c] = (f16)(v * (float)qr[c]);
+            const float qc = kElemQ ? qstep((float)qr[c]) : (float)qr[c];
+            yout[m * ldo + c] = (f16)(v * qc);
         }
     }
 }
@@ -346,8 +357,16 @@ __global__ void intra_enc_step_kernel(Act y, const f16* __restrict__ scales, int
 hipError_t intra_enc_step(const Act& y, const f16* scales, int lds, const f16* means, int ldm, f16* ysf, int ldsf,
                           const f16* q, int qstride, const int* qp, f16* yout, int ldo, int16_t* sym, uint8_t* cond,
                           const float* th, float thr, int step, hipStream_t s) {
-    hipLaunchKernelGGL(intra_enc_step_kernel, grid1d(y.pixels() * (y.C / 4)), dim3(256), 0, s, y, scales, lds, means,
-                       ldm, ysf, ldsf, q, qstride, qp, yout, ldo, sym, cond, th, thr, step);
+    hipLaunchKernelGGL(quarter_enc_step_kernel<false>, grid1d(y.pixels() * (y.C / 4)), dim3(256), 0, s, y, scales, lds,
+                       means, ldm, ysf, ldsf, q, qstride, qp, yout, ldo, sym, cond, th, thr, step);
+    return hipGetLastError();
+}
+
+hipError_t htl_enc_step(const Act& y, const f16* scales, int lds, const f16* means, int ldm, const f16* qdec, int ldq,
+                        f16* ysf, int ldsf, f16* yout, int ldo, int16_t* sym, uint8_t* cond, const float* th, float thr,
+                        int step, hipStream_t s) {
+    hipLaunchKernelGGL(quarter_enc_step_kernel<true>, grid1d(y.pixels() * (y.C / 4)), dim3(256), 0, s, y, scales, lds,
+                       means, ldm, ysf, ldsf, qdec, ldq, nullptr, yout, ldo, sym, cond, th, thr, step);
     return hipGetLastError();
 }
 
@@ -373,9 +392,10 @@ hipError_t intra_dec_index(const Act& g, const f16* scales, int lds, const float
     return hipGetLastError();
 }
 
-__global__ void intra_dec_step_kernel(Act g, const int8_t* __restrict__ yq, const f16* __restrict__ means, int ldm,
-                                      f16* __restrict__ ysf, int ldsf, const f16* __restrict__ q, int qstride,
-                                      const int* __restrict__ qp, f16* __restrict__ yout, int ldo, int step) {
+template <bool kElemQ>
+__global__ void quarter_dec_step_kernel(Act g, const int8_t* __restrict__ yq, const f16* __restrict__ means, int ldm,
+                                        f16* __restrict__ ysf, int ldsf, const f16* __restrict__ q, int ldq,
+                                        const int* __restrict__ qp, f16* __restrict__ yout, int ldo, int step) {
     const int C4 = g.C >> 2;
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= g.pixels() * C4) return;
@@ -394,20 +414,28 @@ __global__ void intra_dec_step_kernel(Act g, const int8_t* __restrict__ yq, cons
             ysf[m * ldsf + ca] = yh;
         }
     } else {
-        const f16* qr = q + (size_t)(qp ? *qp : 0) * qstride;
+        const f16* qr = kElemQ ? q + m * ldq : q + (size_t)(qp ? *qp : 0) * ldq;
 #pragma unroll
         for (int gg = 0; gg < 4; ++gg) {
             const int c = gg * C4 + cq;
             const float v = (gg == ga) ? (float)yh : (float)ysf[m * ldsf + c];
-            yout[m * ldo + c] = (f16)(v * (float)qr[c]);
+            const float qc = kElemQ ? qstep((float)qr[c]) : (float)qr[c];
+            yout[m * ldo + c] = (f16)(v * qc);
         }
     }
 }
 
 hipError_t intra_dec_step(const Act& g, const int8_t* yq, const f16* means, int ldm, f16* ysf, int ldsf,
                           const f16* q, int qstride, const int* qp, f16* yout, int ldo, int step, hipStream_t s) {
-    hipLaunchKernelGGL(intra_dec_step_kernel, grid1d(g.pixels() * (g.C / 4)), dim3(256), 0, s, g, yq, means, ldm,
-                       ysf, ldsf, q, qstride, qp, yout, ldo, step);
+    hipLaunchKernelGGL(quarter_dec_step_kernel<false>, grid1d(g.pixels() * (g.C / 4)), dim3(256), 0, s, g, yq, means,
+                       ldm, ysf, ldsf, q, qstride, qp, yout, ldo, step);
+    return hipGetLastError();
+}
+
+hipError_t htl_dec_step(const Act& g, const int8_t* yq, const f16* means, int ldm, const f16* qdec, int ldq, f16* ysf,
+                        int ldsf, f16* yout, int ldo, int step, hipStream_t s) {
+    hipLaunchKernelGGL(quarter_dec_step_kernel<true>, grid1d(g.pixels() * (g.C / 4)), dim3(256), 0, s, g, yq, means,
+                       ldm, ysf, ldsf, qdec, ldq, nullptr, yout, ldo, step);
     return hipGetLastError();
 }
 

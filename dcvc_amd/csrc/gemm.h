@@ -34,6 +34,7 @@ enum GemmPrec {
 enum GemmDMode {
     D_PLAIN = 0,   // D[m, n] = D + m * ldd + n
     D_SUBPEL = 1,  // n = sub * (N/4) + c stored at pixel (2h + sub/2, 2w + sub%2) of the 2x grid
+                   // (with A_PLAIN: 1x1 sub-pixel conv; with A_S1K3: 3x3 pad-1 sub-pixel conv)
 };
 
 struct GemmArgs {
@@ -52,6 +53,8 @@ struct GemmArgs {
     const f16* q = nullptr;     // per-QP table, row stride qstride
     int qstride = 0;
     const int* qp = nullptr;    // device QP index (nullptr -> row 0)
+    int q_round = 0;            // quant epilogues: round to fp16 before the q multiply,
+                                // as a separate fp16 multiply_with_broadcast would
     f16* D = nullptr;
     int ldd = 0;
     int prec = PREC_F16;             // GemmPrec; B is bf16 / fp8 bytes accordingly

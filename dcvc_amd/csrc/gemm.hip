@@ -540,6 +540,10 @@ __global__ __launch_bounds__(T::kThreads, T::kWavesPerSimd) void gemm_kernel(Gem
                 for (int e = 0; e < 8; ++e) x[e] += (float)s[e];
             }
             if constexpr (EPI == EPI_SHORTCUT_QUANT || EPI == EPI_QUANT) {
+                if (g.q_round) {
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) x[e] = (float)(f16)x[e];
+                }
 #pragma unroll
                 for (int e = 0; e < 8; ++e) x[e] *= (float)q[n + e];
             }
@@ -678,7 +682,11 @@ hipError_t gemm_launch(const GemmArgs& g_in, int epi, int amode, int dmode, hipS
     // epilogue only (conv_bias, transposed_conv)
     if (amode != A_PLAIN || dmode != D_PLAIN) {
         if (epi != EPI_BIAS) return hipErrorInvalidValue;
-        if (dmode == D_SUBPEL) return dispatch_tiles<EPI_BIAS, A_PLAIN, D_SUBPEL>(g, stream);
+        if (dmode == D_SUBPEL) {
+            if (amode == A_PLAIN) return dispatch_tiles<EPI_BIAS, A_PLAIN, D_SUBPEL>(g, stream);
+            if (amode == A_S1K3) return dispatch_tiles<EPI_BIAS, A_S1K3, D_SUBPEL>(g, stream);
+            return hipErrorInvalidValue;
+        }
         switch (amode) {
         case A_S2K2: return dispatch_tiles<EPI_BIAS, A_S2K2, D_PLAIN>(g, stream);
         case A_S2K3: return dispatch_tiles<EPI_BIAS, A_S2K3, D_PLAIN>(g, stream);
@@ -720,6 +728,7 @@ int gemm_validate(const GemmArgs& g, int epi, int amode, int dmode) {
     if (a_rows * (size_t)g.lda * 2 >= kOOB) return DCVC_ERR_ARG;
     if ((size_t)(epi == EPI_QUAD ? 4 * g.N : g.N) * g.K * 2 >= kOOB) return DCVC_ERR_ARG;
     if (dmode == D_SUBPEL) {
+        if (amode != A_PLAIN && amode != A_S1K3) return DCVC_ERR_ARG;
         if (g.N % 32 != 0 || g.outH <= 0 || g.outW <= 0 || g.M % (g.outH * g.outW) != 0) return DCVC_ERR_ARG;
         if (g.ldd < g.N / 4) return DCVC_ERR_ARG;
     } else if (g.ldd < g.N) {

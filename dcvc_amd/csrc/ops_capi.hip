@@ -91,6 +91,19 @@ int dcvc_conv1x1_bias_shortcut_with_quant(const void* A, int lda, int M, int K, 
     return run_gemm(g, EPI_SHORTCUT_QUANT, A_PLAIN, D_PLAIN, stream);
 }
 
+int dcvc_conv1x1_bias_shortcut_then_quant(const void* A, int lda, int M, int K, const void* W, int N,
+                                          const float* bias, const void* Sc, int lds, const void* q, int qstride,
+                                          const int* qp, void* D, int ldd, void* stream) {
+    GemmArgs g = plain_args(A, lda, M, K, W, N, bias, D, ldd);
+    g.S1 = H(Sc);
+    g.lds1 = lds;
+    g.q = H(q);
+    g.qstride = qstride;
+    g.qp = qp;
+    g.q_round = 1;
+    return run_gemm(g, EPI_SHORTCUT_QUANT, A_PLAIN, D_PLAIN, stream);
+}
+
 int dcvc_conv1x1_bias_with_quant(const void* A, int lda, int M, int K, const void* W, int N, const float* bias,
                                  const void* q, int qstride, const int* qp, void* D, int ldd, void* stream) {
     GemmArgs g = plain_args(A, lda, M, K, W, N, bias, D, ldd);
@@ -187,6 +200,27 @@ int dcvc_transposed_conv(const void* X, int ldx, int n_img, int H_, int W_, int 
     g.ldd = ldd;
     if (!bias) return DCVC_ERR_ARG;  // callers pass a zero vector for bias-free layers
     return run_gemm(g, EPI_BIAS, A_PLAIN, D_SUBPEL, stream);
+}
+
+int dcvc_subpel_conv3x3(const void* X, int ldx, int n_img, int H_, int W_, int Cin, const void* Wt, int Cout,
+                        const float* bias, void* D, int ldd, void* stream) {
+    GemmArgs g;
+    g.A = H(X);
+    g.lda = ldx;
+    g.inH = H_;
+    g.inW = W_;
+    g.outH = H_;
+    g.outW = W_;
+    g.cin = Cin;
+    g.M = n_img * H_ * W_;
+    g.K = 9 * Cin;
+    g.B = H(Wt);
+    g.N = 4 * Cout;
+    g.bias = bias;
+    g.D = Hm(D);
+    g.ldd = ldd;
+    if (!bias) return DCVC_ERR_ARG;
+    return run_gemm(g, EPI_BIAS, A_S1K3, D_SUBPEL, stream);
 }
 
 int dcvc_d3x3(const void* X, int ldx, int n_img, int H_, int W_, int C, const void* Wt, void* Y, int ldy,
@@ -295,6 +329,23 @@ int dcvc_intra_restore_step(const int8_t* yq, int n_img, int H_, int W_, int C, 
     if (step == 3 && (!q || !Yout)) return DCVC_ERR_ARG;
     return st(intra_dec_step(act(nullptr, 0, n_img, H_, W_, C), yq, H(Mn), ldm, Hm(Ysf), ldsf, H(q), qstride, qp,
                              Hm(Yout), ldo, step, S(stream)));
+}
+
+int dcvc_htl_quant_step(const void* Y, int ldy, int n_img, int H_, int W_, int C, const void* Sc, int lds,
+                        const void* Mn, int ldm, const void* Qd, int ldq, void* Ysf, int ldsf, void* Yout, int ldo,
+                        int16_t* sym, uint8_t* cond, const float* th, float thr, int step, void* stream) {
+    if (!Y || !Sc || !Mn || !Qd || !Ysf || !sym || !cond || !th || step < 0 || step > 3 || C % 4) return DCVC_ERR_ARG;
+    if (step == 3 && !Yout) return DCVC_ERR_ARG;
+    return st(htl_enc_step(act(Y, ldy, n_img, H_, W_, C), H(Sc), lds, H(Mn), ldm, H(Qd), ldq, Hm(Ysf), ldsf, Hm(Yout),
+                           ldo, sym, cond, th, thr, step, S(stream)));
+}
+
+int dcvc_htl_restore_step(const int8_t* yq, int n_img, int H_, int W_, int C, const void* Mn, int ldm, const void* Qd,
+                          int ldq, void* Ysf, int ldsf, void* Yout, int ldo, int step, void* stream) {
+    if (!yq || !Mn || !Qd || !Ysf || step < 0 || step > 3 || C % 4) return DCVC_ERR_ARG;
+    if (step == 3 && !Yout) return DCVC_ERR_ARG;
+    return st(htl_dec_step(act(nullptr, 0, n_img, H_, W_, C), yq, H(Mn), ldm, H(Qd), ldq, Hm(Ysf), ldsf, Hm(Yout), ldo,
+                           step, S(stream)));
 }
 
 int dcvc_compact_blocks(long N) { return compact_blocks(N); }

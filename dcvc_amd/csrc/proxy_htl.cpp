@@ -1,0 +1,536 @@
+// DMC HT-L codec proxy (see proxy_htl.h).
+//
+// Buffer layout (NHWC, pixel rows), concatenation views as in the reference
+// (dmc_htl_proxy.cpp:1088-1112) so no concatenation is ever copied:
+//   cat_enc [P8, 2048] = x_unshuffled[0:1536) | ctx[1536:2048);
+//                        the decoder input is [1024:2048) = up(y_hat) | ctx
+//   cat_fam [P8, 1024] = memory[0:512) | feature_p[512:1024)
+//   cat_pf  [P16, 768] = hyper params[0:256) | temporal params[256:768)
+//   common  [P16, 768] = q_dec | scales | means
+//   cat_sp  [P16, 512] = y_hat so far[0:256) | reduced params[256:512)
+//   sp_o    [P16, 512] = scales | means of checkerboard steps 1-3
+// Unlike HT-S, every checkerboard step has its own scales and its own
+// quarter-tensor symbol stream, so the decoder makes four entropy round trips
+// (dmc_htl_proxy.cpp:740-910): graphs dec_1 (prior + step-0 indexes) and
+// dec[k] (restore step k, spatial prior, indexes of step k+1; the last one
+// runs the decoder and the recon head) alternate with the host decoding one
+// quarter each.  The QP is a device scalar, so one graph serves all 64 QPs.
+#include "proxy_htl.h"
+
+#include <cstring>
+
+namespace dcvc {
+
+static std::vector<int32_t> to_i32(const HostTensor& t) {
+    std::vector<int32_t> v(t.data.size());
+    for (size_t i = 0; i < v.size(); ++i) v[i] = (int32_t)t.data[i];
+    return v;
+}
+
+static f16* upload_qtable(DevMem& mem, const HostDict& d, const std::string& key, int C) {
+    const HostTensor& t = need(d, key);
+    if (t.numel() != 64L * C) throw std::invalid_argument(key + ": expected [64, C]");
+    return upload_f16(mem, t.data);
+}
+
+// DMCHTLProxy::set_param (dmc_htl_proxy.cpp:917-967) and the submodule
+// set_param calls: the stride-2 and up-sampling blocks of the hyper coder and
+// the temporal prior keep their block shortcut in HT-L (:274-279, 302-307, 471-474).
+void HTLWeights::load(const HostDict& d, float skip) {
+    mem.free_all();
+    fa_i.load(mem, d, "feature_adaptor_i.conv.", 3);
+    fa_m.load(mem, d, "feature_adaptor_m.conv.", 10);
+    fe.load(mem, d, "feature_extractor.conv.", 2);
+    enc.load(mem, d, "encoder.conv1.", 7);
+    enc_down.load(mem, d, "encoder.down.", 3);
+    he0.load(mem, d, "hyper_encoder.conv.0.");
+    he1.load(mem, d, "hyper_encoder.conv.1.", true);
+    he2.load(mem, d, "hyper_encoder.conv.2.", true);
+    hd0.load(mem, d, "hyper_decoder.conv.0.", true);
+    hd1.load(mem, d, "hyper_decoder.conv.1.", true);
+    hd2.load(mem, d, "hyper_decoder.conv.2.");
+    tpe.load(mem, d, "temporal_prior_encoder.conv.", true);
+    pf.load(mem, d, "y_prior_fusion.conv.", 3);
+    pf_out.load(mem, d, "y_prior_fusion.conv.3.");
+    reduction.load(mem, d, "y_spatial_prior_reduction.");
+    for (int i = 0; i < 3; ++i) spa[i].load(mem, d, "y_spatial_prior_adaptor_" + std::to_string(i + 1) + ".");
+    sp.load(mem, d, "y_spatial_prior.conv.", 3);
+    sp_out.load(mem, d, "y_spatial_prior.conv.3.");
+    dec_up.load(mem, d, "decoder.up.");
+    if (dec_up.k != 3) throw std::invalid_argument("decoder.up: HT-L expects a 3x3 sub-pixel conv");
+    dec.load(mem, d, "decoder.conv1.", 11);
+    for (int i = 0; i < 8; ++i) {
+        rh[i].load(mem, d, "recon_head.conv." + std::to_string(i) + ".", 5);
+        rh_head[i].load(mem, d, "recon_head.conv." + std::to_string(i) + ".5.");
+    }
+    q_encoder = upload_qtable(mem, d, "q_encoder", 512);
+    q_decoder = upload_qtable(mem, d, "q_decoder", 512);
+    q_feature = upload_qtable(mem, d, "q_feature", 512);
+    float th[128];
+    host_scale_index_thresholds(th);
+    thresholds = upload_f32(mem, std::vector<float>(th, th + 128));
+    skip_thres = skip;
+    const char* names[2] = {"bit_estimator_z.", "gaussian_encoder."};
+    for (int i = 0; i < 2; ++i) {
+        const HostTensor& c = need(d, std::string(names[i]) + "quantized_cdf");
+        const HostTensor& l = need(d, std::string(names[i]) + "cdf_length");
+        cdf[i] = to_i32(c);
+        cdf_len[i] = to_i32(l);
+        cdf_row[i] = (int)(c.numel() / l.numel());
+    }
+    hip_check(hipDeviceSynchronize(), "weight conversion");
+}
+
+// ---------------------------------------------------------------------------
+HTLProxy::HTLProxy() {
+    hip_check(hipStreamCreateWithFlags(&m_stream, hipStreamNonBlocking), "stream");
+    hip_check(hipStreamCreateWithFlags(&m_copy, hipStreamNonBlocking), "stream");
+    hip_check(hipEventCreateWithFlags(&m_ev_in, hipEventDisableTiming), "event");
+    hip_check(hipEventCreateWithFlags(&m_ev_out, hipEventDisableTiming), "event");
+    hip_check(hipEventCreateWithFlags(&m_ev_sym, hipEventDisableTiming), "event");
+}
+
+static void free_pinned(int*& counts, std::array<int16_t*, 4>& sym, uint8_t*& idx, int8_t*& dec, int8_t*& z) {
+    hipHostFree(counts);
+    for (auto& p : sym) {
+        hipHostFree(p);
+        p = nullptr;
+    }
+    hipHostFree(idx);
+    hipHostFree(dec);
+    hipHostFree(z);
+    counts = nullptr;
+    idx = nullptr;
+    dec = nullptr;
+    z = nullptr;
+}
+
+HTLProxy::~HTLProxy() {
+    if (m_stream) hipStreamSynchronize(m_stream);
+    reset_graphs();
+    free_pinned(m_h_counts, m_h_sym, m_h_idx, m_h_dec, m_h_z);
+    m_buf.free_all();
+    hipEventDestroy(m_ev_in);
+    hipEventDestroy(m_ev_out);
+    hipEventDestroy(m_ev_sym);
+    hipStreamDestroy(m_copy);
+    hipStreamDestroy(m_stream);
+}
+
+void HTLProxy::set_param(const HostDict& d, float skip_thres) {
+    auto w = std::make_shared<HTLWeights>();
+    w->load(d, skip_thres);
+    share_weights(w);
+}
+
+void HTLProxy::share_weights(const std::shared_ptr<HTLWeights>& w) {
+    m_w = w;
+    for (int i = 0; i < 2; ++i) {
+        m_enc.set_cdf(w->cdf[i].data(), (int)w->cdf_len[i].size(), w->cdf_row[i], w->cdf_len[i].data(), i);
+        m_dec.set_cdf(w->cdf[i].data(), (int)w->cdf_len[i].size(), w->cdf_row[i], w->cdf_len[i].data(), i);
+    }
+    reset_graphs();
+}
+
+void HTLProxy::reset_graphs() {
+    m_g_enc0.reset();
+    for (auto& g : m_g_enc1) g.reset();
+    for (auto& g : m_g_dec0) g.reset();
+    m_g_dec1.reset();
+    for (auto& g : m_g_dec) g.reset();
+}
+
+// pre_allocate_tensors (dmc_htl_proxy.cpp:1042-1178)
+void HTLProxy::prepare(int Hp, int Wp, f16* const frames[kFrames], int ld_out) {
+    if (Hp <= 0 || Wp <= 0 || Hp % 16 || Wp % 16) throw std::invalid_argument("picture size must be padded to 16");
+    bool same = (Hp == m_Hp && Wp == m_Wp && ld_out == m_ld_out);
+    for (int i = 0; i < kFrames && same; ++i) same = (frames[i] == m_frames[i]);
+    if (same && m_cat_enc) return;
+    hip_check(hipStreamSynchronize(m_stream), "sync");
+    reset_graphs();
+    m_buf.free_all();
+    free_pinned(m_h_counts, m_h_sym, m_h_idx, m_h_dec, m_h_z);
+    m_Hp = Hp;
+    m_Wp = Wp;
+    m_H8 = Hp / 8;
+    m_W8 = Wp / 8;
+    m_H16 = Hp / 16;
+    m_W16 = Wp / 16;
+    m_H16p = (m_H16 + 3) / 4 * 4;
+    m_W16p = (m_W16 + 3) / 4 * 4;
+    m_pad4 = (m_H16p != m_H16 || m_W16p != m_W16);
+    m_H64 = m_H16p / 4;
+    m_W64 = m_W16p / 4;
+    const long P8 = (long)m_H8 * m_W8, P16 = (long)m_H16 * m_W16, P16p = (long)m_H16p * m_W16p;
+    const long P32p = P16p / 4, P64 = (long)m_H64 * m_W64;
+    m_nq = P16 * 64;
+    m_nz = P64 * 128;
+    DevMem& M = m_buf;
+    m_scratch.halves = (size_t)P8 * 768;
+    m_scratch.t1 = M.alloc_n<f16>(m_scratch.halves);
+    m_scratch.t2 = M.alloc_n<f16>(m_scratch.halves);
+    m_scratch.so = M.alloc_n<f16>(m_scratch.halves);
+    m_scratch.x1 = M.alloc_n<f16>(m_scratch.halves);
+    m_scratch.x2 = M.alloc_n<f16>(m_scratch.halves);
+    m_d_qp = M.alloc_n<int>(1);
+    m_d_counts = M.alloc_n<int>(4);
+    m_d_offs = M.alloc_n<int>(compact_blocks(m_nq) + 1);
+    m_cat_enc = M.alloc_n<f16>(P8 * 2048);
+    m_cat_fam = M.alloc_n<f16>(P8 * 1024);
+    m_feature_i = M.alloc_n<f16>(P8 * 192);
+    m_tin = M.alloc_n<f16>(P8 * 512);
+    m_enc_out = M.alloc_n<f16>(P8 * 512);
+    m_y = M.alloc_n<f16>(P16 * 256);
+    m_y_pad = m_pad4 ? M.alloc_n<f16>(P16p * 256) : m_y;
+    m_he_a = M.alloc_n<f16>(P16p * 256);
+    m_he_b = M.alloc_n<f16>(P32p * 256);
+    m_z = M.alloc_n<f16>(P64 * 128);
+    m_z_hat = M.alloc_n<f16>(P64 * 128);
+    m_hd_a = M.alloc_n<f16>(P32p * 256);
+    m_hd_b = M.alloc_n<f16>(P16p * 256);
+    m_cat_pf = M.alloc_n<f16>(P16 * 768);
+    m_pf_a = M.alloc_n<f16>(P16 * 768);
+    m_common = M.alloc_n<f16>(P16 * 768);
+    m_cat_sp = M.alloc_n<f16>(P16 * 512);
+    m_sp_a = M.alloc_n<f16>(P16 * 512);
+    m_sp_o = M.alloc_n<f16>(P16 * 512);
+    m_y_hat = M.alloc_n<f16>(P16 * 256);
+    m_rh_a = M.alloc_n<f16>(P8 * 256);
+    m_rh_head = M.alloc_n<f16>(P8 * 192);
+    m_z_i8 = M.alloc_n<int8_t>(m_nz);
+    m_yq = M.alloc_n<int8_t>(m_nq);
+    m_dec_c = M.alloc_n<int8_t>(m_nq);
+    for (int i = 0; i < 4; ++i) {
+        m_sym[i] = M.alloc_n<int16_t>(m_nq);
+        m_symc[i] = M.alloc_n<int16_t>(m_nq);
+        m_condq[i] = M.alloc_n<uint8_t>(m_nq);
+    }
+    m_cond = M.alloc_n<uint8_t>(m_nq);
+    m_idx = M.alloc_n<uint8_t>(m_nq);
+    m_idxc = M.alloc_n<uint8_t>(m_nq);
+    // on the proxy's own stream: a legacy-stream operation here would break a
+    // graph capture running on another thread's proxy
+    hip_check(hipMemsetAsync(m_cat_enc, 0, P8 * 2048 * sizeof(f16), m_stream), "memset");
+    hip_check(hipMemsetAsync(m_cat_fam, 0, P8 * 1024 * sizeof(f16), m_stream), "memset");
+    hip_check(hipStreamSynchronize(m_stream), "memset");
+    hip_check(hipHostMalloc((void**)&m_h_counts, 4 * sizeof(int)), "pinned");
+    for (int i = 0; i < 4; ++i) hip_check(hipHostMalloc((void**)&m_h_sym[i], m_nq * sizeof(int16_t)), "pinned");
+    hip_check(hipHostMalloc((void**)&m_h_idx, m_nq), "pinned");
+    hip_check(hipHostMalloc((void**)&m_h_dec, m_nq), "pinned");
+    hip_check(hipHostMalloc((void**)&m_h_z, m_nz), "pinned");
+    for (int i = 0; i < kFrames; ++i) m_frames[i] = frames[i];
+    m_ld_out = ld_out;
+    m_memory_ready = false;
+    m_memory_has_value = false;
+}
+
+void HTLProxy::enter(hipStream_t caller) {
+    hip_check(hipEventRecord(m_ev_in, caller), "event record");
+    hip_check(hipStreamWaitEvent(m_stream, m_ev_in, 0), "stream wait");
+}
+
+void HTLProxy::leave(hipStream_t caller) {
+    hip_check(hipEventRecord(m_ev_out, m_stream), "event record");
+    hip_check(hipStreamWaitEvent(caller, m_ev_out, 0), "stream wait");
+}
+
+// memory = FA_M(cat(memory, feature_p)) or FA_I(feature_i) (dmc_htl_proxy.cpp:981-988)
+void HTLProxy::feature_adaptor(bool use_m) {
+    const HTLWeights& w = *m_w;
+    Exec e = exec();
+    const Act mem = make_act(m_cat_fam, 1, m_H8, m_W8, 512, 1024);
+    const Act tmp = make_act(m_scratch.x1, 1, m_H8, m_W8, 512, 512);
+    if (use_m)
+        w.fa_m.forward(e, make_act(m_cat_fam, 1, m_H8, m_W8, 1024, 1024), mem, tmp);
+    else
+        w.fa_i.forward(e, make_act(m_feature_i, 1, m_H8, m_W8, 192, 192), mem, tmp);
+}
+
+void HTLProxy::add_ref_feature_from_frame(const f16* frame, int ld, int Hp, int Wp, bool apply_adaptor, hipStream_t s) {
+    if (!m_w) throw std::logic_error("set_param must be called first");
+    if (Hp != m_Hp || Wp != m_Wp) throw std::logic_error("prepare() was not called for this resolution");
+    enter(s);
+    // pixel_unshuffle(frame, 8) (dmc_htl_proxy.cpp:587); eager because the frame pointer changes
+    hip_check(pad_unshuffle(frame, ld, 1, Hp, Wp, 3, 8, 0, 0, m_feature_i, 192, m_stream), "unshuffle");
+    if (apply_adaptor) {
+        Exec e = exec();
+        feature_adaptor(false);
+        const Act mem = make_act(m_cat_fam, 1, m_H8, m_W8, 512, 1024);
+        const Act tmp = make_act(m_scratch.x1, 1, m_H8, m_W8, 512, 512);
+        m_w->fe.forward(e, mem, make_act(m_cat_enc + 1536, 1, m_H8, m_W8, 512, 2048), tmp);
+    }
+    m_memory_ready = apply_adaptor;
+    m_memory_has_value = apply_adaptor;
+    leave(s);
+}
+
+// hyper decoder -> crop -> prior fusion -> common params (shared by both sides)
+void HTLProxy::hyper_and_prior() {
+    const HTLWeights& w = *m_w;
+    Exec e = exec();
+    const Act zh = make_act(m_z_hat, 1, m_H64, m_W64, 128, 128);
+    const Act hda = make_act(m_hd_a, 1, m_H64 * 2, m_W64 * 2, 256, 256);
+    const Act hdb = make_act(m_hd_b, 1, m_H16p, m_W16p, 256, 256);
+    const Act hyper = make_act(m_cat_pf, 1, m_H16, m_W16, 256, 768);
+    w.hd0.forward(e, zh, hda);
+    w.hd1.forward(e, hda, hdb);
+    if (m_pad4) {
+        w.hd2.forward(e, hdb, hdb);
+        hip_check(replicate_pad(m_hd_b, 256, 1, m_H16p, m_W16p, 256, m_H16, m_W16, m_cat_pf, 768, m_stream), "crop");
+    } else {
+        w.hd2.forward(e, hdb, hyper);
+    }
+    const Act catpf = make_act(m_cat_pf, 1, m_H16, m_W16, 768, 768);
+    const Act pfa = make_act(m_pf_a, 1, m_H16, m_W16, 768, 768);
+    const Act tmp = make_act(m_scratch.x1, 1, m_H16, m_W16, 768, 768);
+    w.pf.forward(e, catpf, pfa, tmp);
+    const Act common = make_act(m_common, 1, m_H16, m_W16, 768, 768);
+    w.pf_out.run(e, pfa, common, EPI_BIAS);
+    w.reduction.run(e, common, make_act(m_cat_sp + 256, 1, m_H16, m_W16, 256, 512), EPI_BIAS);
+}
+
+// adaptor_k(cat(y_hat so far, reduced)) -> spatial prior -> scales | means
+void HTLProxy::spatial_prior(int step) {
+    const HTLWeights& w = *m_w;
+    Exec e = exec();
+    const Act catsp = make_act(m_cat_sp, 1, m_H16, m_W16, 512, 512);
+    const Act spa = make_act(m_sp_a, 1, m_H16, m_W16, 512, 512);
+    const Act tmp = make_act(m_scratch.x1, 1, m_H16, m_W16, 512, 512);
+    w.spa[step - 1].forward(e, catsp, spa);
+    w.sp.forward(e, spa, spa, tmp);
+    w.sp_out.run(e, spa, make_act(m_sp_o, 1, m_H16, m_W16, 512, 512), EPI_BIAS);
+}
+
+// lambda_enc_0 (dmc_htl_proxy.cpp:607-691)
+void HTLProxy::enc_stage0() {
+    const HTLWeights& w = *m_w;
+    Exec e = exec();
+    const Act cat = make_act(m_cat_enc, 1, m_H8, m_W8, 2048, 2048);
+    const Act eo = make_act(m_enc_out, 1, m_H8, m_W8, 512, 512);
+    const Act tmp8 = make_act(m_scratch.x1, 1, m_H8, m_W8, 512, 512);
+    QuantSel qe{w.q_encoder, 512, true};  // separate fp16 multiply after the last block (:103)
+    w.enc.forward(e, cat, eo, tmp8, &qe);
+    const Act y = make_act(m_y, 1, m_H16, m_W16, 256, 256);
+    w.enc_down.run(e, eo, y);
+    const Act ypad = make_act(m_y_pad, 1, m_H16p, m_W16p, 256, 256);
+    if (m_pad4) hip_check(replicate_pad(m_y, 256, 1, m_H16, m_W16, 256, m_H16p, m_W16p, m_y_pad, 256, m_stream), "pad");
+    const Act hea = make_act(m_he_a, 1, m_H16p, m_W16p, 256, 256);
+    const Act heb = make_act(m_he_b, 1, m_H16p / 2, m_W16p / 2, 256, 256);
+    const Act z = make_act(m_z, 1, m_H64, m_W64, 128, 128);
+    w.he0.forward(e, ypad, hea);
+    w.he1.forward(e, hea, heb);
+    w.he2.forward(e, heb, z);
+    hip_check(round_z(m_z, 128, (long)m_H64 * m_W64, 128, m_z_hat, 128, m_z_i8, m_stream), "round_z");
+    hip_check(mul_channel(m_cat_fam, 1024, (long)m_H8 * m_W8, 512, w.q_feature, 512, m_d_qp, m_tin, 512, m_stream),
+              "mul");
+    w.tpe.forward(e, make_act(m_tin, 1, m_H8, m_W8, 512, 512), make_act(m_cat_pf + 256, 1, m_H16, m_W16, 512, 768));
+    hyper_and_prior();
+    for (int step = 0; step < 4; ++step) {
+        if (step > 0) spatial_prior(step);
+        const int ld = step_ld(step);
+        hip_check(htl_enc_step(y, step_scales(step), ld, step_means(step), ld, m_common, 768, m_cat_sp, 512, m_y_hat,
+                               256, m_sym[step], m_condq[step], w.thresholds, w.skip_thres, step, m_stream),
+                  "quant step");
+        hip_check(compact(m_sym[step], 2, m_condq[step], m_nq, m_symc[step], m_d_offs, m_d_counts + step, m_stream),
+                  "compact");
+    }
+}
+
+// decoder + recon head (+ the reset branch for the encoder's context update)
+void HTLProxy::decoder_and_recon(bool recon_all, bool reset_for_encoder) {
+    const HTLWeights& w = *m_w;
+    Exec e = exec();
+    const Act yhat = make_act(m_y_hat, 1, m_H16, m_W16, 256, 256);
+    w.dec_up.run(e, yhat, make_act(m_cat_enc + 1024, 1, m_H8, m_W8, 512, 2048));
+    const Act decin = make_act(m_cat_enc + 1024, 1, m_H8, m_W8, 1024, 2048);
+    const Act fp = make_act(m_cat_fam + 512, 1, m_H8, m_W8, 512, 1024);
+    const Act tmp = make_act(m_scratch.x1, 1, m_H8, m_W8, 512, 512);
+    QuantSel qd{w.q_decoder, 512, true};  // separate fp16 multiply after the last block (:40)
+    w.dec.forward(e, decin, fp, tmp, &qd);
+    const Act ra = make_act(m_rh_a, 1, m_H8, m_W8, 256, 256);
+    const Act tmp256 = make_act(m_scratch.x2, 1, m_H8, m_W8, 256, 256);
+    const Act fi = make_act(m_feature_i, 1, m_H8, m_W8, 192, 192);
+    if (recon_all) {
+        // DMCHTLReconHeadProxy::forward (:361-380); frame 7's head is also the reset feature
+        for (int i = 0; i < kFrames; ++i) {
+            w.rh[i].forward(e, fp, ra, tmp256);
+            const Act head = (i == kFrames - 1) ? fi : make_act(m_rh_head, 1, m_H8, m_W8, 192, 192);
+            w.rh_head[i].run(e, ra, head, EPI_BIAS);
+            hip_check(pixel_shuffle8_clamp(head.p, 192, 1, m_H8, m_W8, 3, m_frames[i], m_ld_out, m_stream), "shuffle");
+        }
+    } else if (reset_for_encoder) {
+        // forward_reset (:347-359): frame 7's head only
+        w.rh[7].forward(e, fp, ra, tmp256);
+        w.rh_head[7].run(e, ra, fi, EPI_BIAS);
+    }
+}
+
+// lambda_enc_1 (dmc_htl_proxy.cpp:701-708, 969-979)
+void HTLProxy::enc_stage1(bool reset) {
+    Exec e = exec();
+    decoder_and_recon(false, reset);
+    feature_adaptor(!reset);
+    const Act mem = make_act(m_cat_fam, 1, m_H8, m_W8, 512, 1024);
+    const Act tmp = make_act(m_scratch.x1, 1, m_H8, m_W8, 512, 512);
+    m_w->fe.forward(e, mem, make_act(m_cat_enc + 1536, 1, m_H8, m_W8, 512, 2048), tmp);
+}
+
+int HTLProxy::compress(const f16* x, int ldx, int H, int W, int qp, bool reset, int pad_b, int pad_r, hipStream_t s,
+                       const uint8_t** bits, long* nbytes) {
+    if (!m_w) throw std::logic_error("set_param must be called first");
+    if (qp < 0 || qp >= 64) throw std::invalid_argument("qp must be in [0, 64)");
+    if (H + pad_b != m_Hp || W + pad_r != m_Wp || pad_b < 0 || pad_r < 0 || ldx < 24)
+        throw std::invalid_argument("input size does not match the prepared resolution");
+    if (!m_memory_ready) throw std::logic_error("add_ref_feature_from_frame(apply_adaptor=True) must precede compress");
+    m_qp = qp;
+    enter(s);
+    set_device_int(m_d_qp, qp, m_stream);
+    // pad_and_unshuffle_8 is eager: x changes every call (dmc_htl_proxy.cpp:603-604)
+    hip_check(pad_unshuffle(x, ldx, 1, H, W, 24, 8, pad_b, pad_r, m_cat_enc, 2048, m_stream), "unshuffle");
+    m_g_enc0.run(m_stream, m_use_graphs, [&] { enc_stage0(); });
+    hip_check(hipEventRecord(m_ev_sym, m_stream), "event");
+    m_g_enc1[reset ? 1 : 0].run(m_stream, m_use_graphs, [&] { enc_stage1(reset); });
+    leave(s);
+    // entropy coding on the host (steps 3..0, then z; :514-545) while the GPU
+    // runs the decoder and the context update
+    hip_check(hipStreamWaitEvent(m_copy, m_ev_sym, 0), "wait");
+    hip_check(hipMemcpyAsync(m_h_counts, m_d_counts, 4 * sizeof(int), hipMemcpyDeviceToHost, m_copy), "d2h");
+    hip_check(hipMemcpyAsync(m_h_z, m_z_i8, m_nz, hipMemcpyDeviceToHost, m_copy), "d2h");
+    hip_check(hipStreamSynchronize(m_copy), "sync");
+    long total = 0;
+    for (int i = 0; i < 4; ++i) {
+        m_counts[i] = m_h_counts[i];
+        if (m_counts[i] < 0 || m_counts[i] > m_nq) throw StatusError(DCVC_ERR_HIP, "symbol count out of range");
+        total += m_counts[i];
+        hip_check(hipMemcpyAsync(m_h_sym[i], m_symc[i], m_counts[i] * sizeof(int16_t), hipMemcpyDeviceToHost, m_copy),
+                  "d2h");
+    }
+    hip_check(hipStreamSynchronize(m_copy), "sync");
+    const int ec = compute_ec_parallel(total);
+    m_enc.reset();
+    m_enc.set_parallel(ec);
+    for (int i = 3; i >= 0; --i) m_enc.encode_y(m_h_sym[i], m_counts[i]);
+    m_enc.encode_z(m_h_z, m_nz, qp * 128, 128);
+    m_enc.flush();
+    m_memory_ready = true;
+    *bits = m_enc.stream().data();
+    *nbytes = (long)m_enc.stream().size();
+    return ec;
+}
+
+// lambda_dec_0 (dmc_htl_proxy.cpp:742-746)
+void HTLProxy::dec_stage0(bool memory_has_value) {
+    const HTLWeights& w = *m_w;
+    Exec e = exec();
+    feature_adaptor(memory_has_value);
+    hip_check(mul_channel(m_cat_fam, 1024, (long)m_H8 * m_W8, 512, w.q_feature, 512, m_d_qp, m_tin, 512, m_stream),
+              "mul");
+    w.tpe.forward(e, make_act(m_tin, 1, m_H8, m_W8, 512, 512), make_act(m_cat_pf + 256, 1, m_H16, m_W16, 512, 768));
+}
+
+// indexes of checkerboard step k, compacted for the host (decode_y_step_build_index, :1008-1017)
+void HTLProxy::dec_index(int step) {
+    const HTLWeights& w = *m_w;
+    const Act g = make_act(nullptr, 1, m_H16, m_W16, 256, 256);
+    hip_check(intra_dec_index(g, step_scales(step), step_ld(step), w.thresholds, w.skip_thres, step, m_idx, m_cond,
+                              m_stream),
+              "index");
+    hip_check(compact(m_idx, 1, m_cond, m_nq, m_idxc, m_d_offs, m_d_counts, m_stream), "compact");
+}
+
+// lambda_dec_1 (dmc_htl_proxy.cpp:765-785)
+void HTLProxy::dec_stage1() {
+    Exec e = exec();
+    hip_check(i8_to_f16(m_z_i8, (long)m_H64 * m_W64, 128, m_z_hat, 128, m_stream), "i8");
+    hyper_and_prior();
+    const Act mem = make_act(m_cat_fam, 1, m_H8, m_W8, 512, 1024);
+    const Act tmp = make_act(m_scratch.x1, 1, m_H8, m_W8, 512, 512);
+    m_w->fe.forward(e, mem, make_act(m_cat_enc + 1536, 1, m_H8, m_W8, 512, 2048), tmp);
+    dec_index(0);
+}
+
+// recover + restore of step k (lambda_dec_2..5, :803-910)
+void HTLProxy::dec_restore(int step) {
+    hip_check(recover(m_dec_c, m_cond, m_nq, m_d_offs, m_yq, m_stream), "recover");
+    const Act g = make_act(nullptr, 1, m_H16, m_W16, 256, 256);
+    hip_check(htl_dec_step(g, m_yq, step_means(step), step_ld(step), m_common, 768, m_cat_sp, 512, m_y_hat, 256, step,
+                           m_stream),
+              "restore");
+}
+
+void HTLProxy::decode_step_on_host(int step) {
+    hip_check(hipEventRecord(m_ev_sym, m_stream), "event");
+    hip_check(hipStreamWaitEvent(m_copy, m_ev_sym, 0), "wait");
+    hip_check(hipMemcpyAsync(m_h_counts, m_d_counts, sizeof(int), hipMemcpyDeviceToHost, m_copy), "d2h");
+    hip_check(hipStreamSynchronize(m_copy), "sync");
+    const long n = m_h_counts[0];
+    if (n < 0 || n > m_nq) throw StatusError(DCVC_ERR_HIP, "index count out of range");
+    m_counts[step] = n;
+    hip_check(hipMemcpyAsync(m_h_idx, m_idxc, n, hipMemcpyDeviceToHost, m_copy), "d2h");
+    hip_check(hipStreamSynchronize(m_copy), "sync");
+    m_dec.decode_y(m_h_idx, n, m_h_dec);
+    hip_check(hipMemcpyAsync(m_dec_c, m_h_dec, n, hipMemcpyHostToDevice, m_stream), "h2d");
+    // m_h_dec is reused by the next step: make sure the upload has completed
+    hip_check(hipStreamSynchronize(m_stream), "sync");
+}
+
+void HTLProxy::decompress(const uint8_t* bits, long nbytes, int qp, int height, int width, int ec_parallel,
+                          bool reset, hipStream_t s) {
+    if (!m_w) throw std::logic_error("set_param must be called first");
+    if (qp < 0 || qp >= 64) throw std::invalid_argument("qp must be in [0, 64)");
+    if ((height + 15) / 16 * 16 != m_Hp || (width + 15) / 16 * 16 != m_Wp)
+        throw std::invalid_argument("picture size does not match the prepared resolution");
+    m_qp = qp;
+    enter(s);
+    set_device_int(m_d_qp, qp, m_stream);
+    m_g_dec0[m_memory_has_value ? 1 : 0].run(m_stream, m_use_graphs, [&] { dec_stage0(m_memory_has_value); });
+    // z decode on the host, concurrent with dec_0 on the GPU
+    m_dec.set_parallel(ec_parallel);
+    m_dec.set_stream(bits, nbytes);
+    m_dec.decode_z(m_nz, qp * 128, 128, m_h_z);
+    hip_check(hipMemcpyAsync(m_z_i8, m_h_z, m_nz, hipMemcpyHostToDevice, m_stream), "h2d");
+    m_g_dec1.run(m_stream, m_use_graphs, [&] { dec_stage1(); });
+    for (int step = 0; step < 4; ++step) {
+        decode_step_on_host(step);
+        m_g_dec[step].run(m_stream, m_use_graphs, [&] {
+            dec_restore(step);
+            if (step < 3) {
+                spatial_prior(step + 1);
+                dec_index(step + 1);
+            } else {
+                decoder_and_recon(true, false);
+            }
+        });
+    }
+    leave(s);
+    m_memory_has_value = !reset;  // host state, outside the graphs (:912)
+    m_memory_ready = false;
+}
+
+void HTLProxy::copy_debug(int what, void* host, long bytes) {
+    hip_check(hipStreamSynchronize(m_stream), "sync");
+    const void* src = nullptr;
+    long cap = 0;
+    if (what >= 5 && what < 9) {  // dense symbols of encoder step what-5
+        src = m_sym[what - 5];
+        cap = m_nq * 2;
+    } else if (what >= 9 && what < 13) {  // dense conditions of encoder step what-9
+        src = m_condq[what - 9];
+        cap = m_nq;
+    } else {
+        switch (what) {
+        case 0: src = m_y_hat; cap = (long)m_H16 * m_W16 * 256 * 2; break;   // final y_hat
+        case 1: src = m_y; cap = (long)m_H16 * m_W16 * 256 * 2; break;       // y (before / q_dec)
+        case 2: src = m_common; cap = (long)m_H16 * m_W16 * 768 * 2; break;  // q_dec | scales | means
+        case 3: src = m_cat_fam; cap = (long)m_H8 * m_W8 * 1024 * 2; break;  // memory | feature_p
+        case 4: src = m_cat_enc; cap = (long)m_H8 * m_W8 * 2048 * 2; break;  // x_unshuffled | ctx
+        case 13: src = m_z_i8; cap = m_nz; break;
+        default: throw std::invalid_argument("unknown debug buffer");
+        }
+    }
+    if (bytes > cap) throw std::invalid_argument("debug copy too large");
+    hip_check(hipMemcpyAsync(host, src, bytes, hipMemcpyDeviceToHost, m_stream), "debug copy");
+    hip_check(hipStreamSynchronize(m_stream), "debug copy");
+}
+
+}  // namespace dcvc

@@ -73,10 +73,14 @@ inline Act channels(const Act& a, int c0, int c) {
     return v;
 }
 
-// Per-QP channel scale: row *qp of a [64, C] fp16 table.
+// Per-QP channel scale: row *qp of a [64, C] fp16 table.  round_first rounds
+// the block output to fp16 before the multiply, which reproduces a separate
+// fp16 multiply_with_broadcast after the block (the HT-L proxy's order) inside
+// the fused epilogue.
 struct QuantSel {
     const f16* table = nullptr;
     int stride = 0;
+    bool round_first = false;
 };
 
 // Scratch activations shared by all layers of one proxy (one stream, so the
@@ -155,12 +159,13 @@ struct ConvS2 {
     void run(const Exec& e, const Act& in, const Act& out) const;
 };
 
-// SubpelConv2x with a 1x1 kernel: conv to 4*Cout then pixel_shuffle(2)
-// (layers.py:92-103; the proxy's transposed_conv, layers_proxy.cpp:269-276).
+// SubpelConv2x: conv (1x1, or 3x3 pad 1) to 4*Cout then pixel_shuffle(2)
+// (layers.py:92-103; the proxy's transposed_conv / conv2d + pixel_shuffle_2,
+// layers_proxy.cpp:269-276), with the shuffle fused into the GEMM store.
 struct Subpel {
-    f16* w = nullptr;  // [4][Cout][Cin] sub-pixel major
+    f16* w = nullptr;  // [4][Cout][k*k][Cin] sub-pixel major
     float* b = nullptr;
-    int Cout = 0, Cin = 0;
+    int Cout = 0, Cin = 0, k = 1;
     void load(DevMem& mem, const HostDict& d, const std::string& prefix);
     void run(const Exec& e, const Act& in, const Act& out) const;
 };

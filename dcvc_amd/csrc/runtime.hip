@@ -120,6 +120,7 @@ void Conv1x1::run(const Exec& e, const Act& a, const Act& dst, int epi, const Ac
         g.q = q->table;
         g.qstride = q->stride;
         g.qp = e.d_qp;
+        g.q_round = q->round_first ? 1 : 0;
     }
     if (a.C != K) throw std::invalid_argument("conv1x1: input channels do not match the weight");
     const int v = gemm_validate(g, epi, A_PLAIN, D_PLAIN);
@@ -297,16 +298,26 @@ void ConvS2::run(const Exec& e, const Act& in, const Act& out) const {
 
 // ---------------------------------------------------------------- Subpel
 void Subpel::load(DevMem& mem, const HostDict& d, const std::string& prefix) {
-    const HostTensor& wt = need(d, prefix + "conv.0.weight");  // [4*Cout, Cin, 1, 1]
+    const HostTensor& wt = need(d, prefix + "conv.0.weight");  // [4*Cout, Cin, k, k]
     Cout = (int)wt.shape[0] / 4;
     Cin = (int)wt.shape[1];
-    std::vector<float> t((size_t)4 * Cout * Cin);
+    k = wt.shape.size() == 4 ? (int)wt.shape[2] : 1;
+    if ((k != 1 && k != 3) || (wt.shape.size() == 4 && wt.shape[3] != k))
+        throw std::invalid_argument(prefix + ": sub-pixel conv kernel must be 1x1 or 3x3");
+    const int taps = k * k;
+    // conv channel c*4 + sub lands at sub-pixel sub of channel c (pixel_shuffle(2));
+    // GEMM row sub*Cout + c, K index tap*Cin + ci
+    std::vector<float> t((size_t)4 * Cout * taps * Cin);
     std::vector<float> bias((size_t)4 * Cout, 0.f);
     const HostTensor* bt = has(d, prefix + "conv.0.bias") ? &need(d, prefix + "conv.0.bias") : nullptr;
     for (int c = 0; c < Cout; ++c)
         for (int sub = 0; sub < 4; ++sub) {
-            std::memcpy(&t[((size_t)sub * Cout + c) * Cin], &wt.data[((size_t)c * 4 + sub) * Cin], Cin * sizeof(float));
-            if (bt) bias[(size_t)sub * Cout + c] = to_half_value(bt->data[c * 4 + sub]);
+            const size_t row = (size_t)sub * Cout + c;
+            const size_t src = (size_t)c * 4 + sub;
+            for (int ci = 0; ci < Cin; ++ci)
+                for (int tap = 0; tap < taps; ++tap)
+                    t[(row * taps + tap) * Cin + ci] = wt.data[(src * Cin + ci) * taps + tap];
+            if (bt) bias[row] = to_half_value(bt->data[src]);
         }
     w = upload_f16(mem, t);
     b = upload_f32(mem, bias);
@@ -319,17 +330,31 @@ void Subpel::run(const Exec& e, const Act& in, const Act& out) const {
     g.outH = in.H;
     g.outW = in.W;
     g.M = (int)in.pixels();
-    g.K = Cin;
+    g.K = k * k * Cin;
     g.B = w;
     g.N = 4 * Cout;
     g.bias = b;
     g.D = out.p;
     g.ldd = out.ld;
-    const int v = gemm_validate(g, EPI_BIAS, A_PLAIN, D_SUBPEL);
+    const int amode = (k == 3) ? A_S1K3 : A_PLAIN;
+    if (k == 3) {
+        g.inH = in.H;
+        g.inW = in.W;
+        g.cin = Cin;
+    }
+    const int v = gemm_validate(g, EPI_BIAS, amode, D_SUBPEL);
     if (v != DCVC_OK || in.C != Cin || out.C != Cout || out.H != 2 * in.H || out.W != 2 * in.W)
         throw StatusError(DCVC_ERR_ARG, "subpel conv: invalid arguments");
-    trace_gemm(2, EPI_BIAS, g.M, 4 * Cout, Cin, false, 0, e.stream);
-    hip_check(gemm_launch(g, EPI_BIAS, A_PLAIN, D_SUBPEL, e.stream), "subpel launch");
+    if (k == 1) {
+        trace_gemm(2, EPI_BIAS, g.M, 4 * Cout, Cin, false, 0, e.stream);
+    } else if (trace_enabled()) {
+        // input read once + weights + output
+        const long long M = in.pixels();
+        trace_record(TraceRecord{2, EPI_BIAS, (int)M, 4 * Cout, g.K, 2LL * M * 4 * Cout * g.K,
+                                 2LL * (M * Cin + 4LL * Cout * g.K + M * 4LL * Cout), -1},
+                     e.stream);
+    }
+    hip_check(gemm_launch(g, EPI_BIAS, amode, D_SUBPEL, e.stream), "subpel launch");
     trace_launched(e.stream);
 }
 

@@ -144,15 +144,19 @@ class DMCIProxy:
         return out
 
 
-class DMCHTSProxy:
-    """DMCHTSProxy (dmc_hts_proxy.h:215-355)."""
+class _ChunkProxy:
+    """Shared body of the 8-frame chunk proxies (HT-S, HT-L): the same methods
+    and argument meaning as the reference classes, over the C ABI entry
+    points named dcvc_<prefix>_*."""
     FRAMES = 8
+    _PREFIX = ""
+    _NAME = ""
 
     def __init__(self):
         self._lib = _lib.load()
-        self._h = self._lib.dcvc_hts_create()
+        self._h = getattr(self._lib, f"dcvc_{self._PREFIX}_create")()
         if not self._h:
-            raise _lib.DcvcError("dcvc_hts_create", 2)
+            raise _lib.DcvcError(f"dcvc_{self._PREFIX}_create", 2)
         self._frames = None
         self._size = None
         self._device = None
@@ -160,85 +164,99 @@ class DMCHTSProxy:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            self._lib.dcvc_hts_destroy(h)
+            getattr(self._lib, f"dcvc_{self._PREFIX}_destroy")(h)
             self._h = None
+
+    def _call(self, fn, *args):
+        return _lib.call(f"dcvc_{self._PREFIX}_{fn}", self._h, *args)
 
     def set_param(self, state_dict, skip_threshold):
         q = state_dict["q_decoder"]
         dev = q.device if isinstance(q, torch.Tensor) else None
         self._device = dev if dev is not None and dev.type == "cuda" else torch.device("cuda", torch.cuda.current_device())
         a = _StateDictArgs(state_dict)
-        _lib.call("dcvc_hts_set_param", self._h, *a.args(), float(skip_threshold))
+        self._call("set_param", *a.args(), float(skip_threshold))
 
-    def share_weights_from(self, other: "DMCHTSProxy"):
+    def share_weights_from(self, other):
         """Use `other`'s weights (one HBM copy for many stream contexts)."""
+        if type(other) is not type(self):
+            raise TypeError("weights can only be shared between proxies of the same model structure")
         self._device = other._device
-        _lib.call("dcvc_hts_share_weights", self._h, other._h)
+        self._call("share_weights", other._h)
 
     def set_use_graphs(self, on):
-        _lib.call("dcvc_hts_set_use_graphs", self._h, int(bool(on)))
+        self._call("set_use_graphs", int(bool(on)))
 
     def set_precision(self, prec):
         """'fp16' (default), 'bf16' or 'fp8': precision of every 1x1-conv product."""
-        _lib.call("dcvc_hts_set_precision", self._h, _PREC[prec])
+        self._call("set_precision", _PREC[prec])
 
     def device_bytes(self):
         """HBM held by this context's activation buffers (weights excluded)."""
-        return int(self._lib.dcvc_hts_device_bytes(self._h))
+        return int(getattr(self._lib, f"dcvc_{self._PREFIX}_device_bytes")(self._h))
 
     def _prepare(self, Hp, Wp):
         if self._size != (Hp, Wp):
             self._frames = [torch.empty((1, 3, Hp, Wp), dtype=torch.float16, device=self._device,
                                         memory_format=torch.channels_last) for _ in range(self.FRAMES)]
             ptrs = (C.c_void_p * self.FRAMES)(*[f.data_ptr() for f in self._frames])
-            _lib.call("dcvc_hts_prepare", self._h, Hp, Wp, C.cast(ptrs, C.c_void_p), 3)
+            self._call("prepare", Hp, Wp, C.cast(ptrs, C.c_void_p), 3)
             self._size = (Hp, Wp)
 
     def add_ref_feature_from_frame(self, frame, apply_adaptor=True):
-        """(dmc_hts_proxy.cpp:492-502) frame: [1,3,Hp,Wp], a decoded I frame."""
+        """frame: [1,3,Hp,Wp], a decoded I frame."""
         frame = _nhwc(frame)
         _, c, Hp, Wp = frame.shape
         if c != 3 or Hp % 16 or Wp % 16:
             raise ValueError("reference frame must be [1,3,H,W] with H, W multiples of 16")
         self._prepare(Hp, Wp)
-        _lib.call("dcvc_hts_add_ref_feature_from_frame", self._h, frame.data_ptr(), 3, Hp, Wp,
-                  int(bool(apply_adaptor)), _stream())
+        self._call("add_ref_feature_from_frame", frame.data_ptr(), 3, Hp, Wp, int(bool(apply_adaptor)), _stream())
 
     def compress(self, x, qp, reset_feature_memory, padding_b, padding_r):
-        """-> (np.uint8 bitstream, ec_parallel)  (dmc_hts_proxy.cpp:504-585); x: [1,24,H,W]"""
+        """-> (np.uint8 bitstream, ec_parallel); x: [1,24,H,W]"""
         x = _nhwc(x)
         _, c, H, W = x.shape
         if c != 24:
-            raise ValueError("HT-S chunk input must have 24 channels (8 frames)")
+            raise ValueError(f"{self._NAME} chunk input must have 24 channels (8 frames)")
         bits = C.c_void_p()
         n = C.c_long()
         ec = C.c_int()
-        _lib.call("dcvc_hts_compress", self._h, x.data_ptr(), 24, H, W, int(qp), int(bool(reset_feature_memory)),
-                  int(padding_b), int(padding_r), _stream(), C.byref(bits), C.byref(n), C.byref(ec))
+        self._call("compress", x.data_ptr(), 24, H, W, int(qp), int(bool(reset_feature_memory)), int(padding_b),
+                   int(padding_r), _stream(), C.byref(bits), C.byref(n), C.byref(ec))
         return _bits_to_numpy(bits, n), ec.value
 
     def decompress(self, bit_stream, qp, height, width, entropy_coder_parallel, reset_feature_memory):
-        """-> list of 8 tensors [1,3,Hp,Wp]  (dmc_hts_proxy.cpp:587-710)"""
+        """-> list of 8 tensors [1,3,Hp,Wp]"""
         b = np.ascontiguousarray(np.asarray(bit_stream, dtype=np.uint8))
-        _lib.call("dcvc_hts_decompress", self._h, b.ctypes.data, b.size, int(qp), int(height), int(width),
-                  int(entropy_coder_parallel), int(bool(reset_feature_memory)), _stream())
+        self._call("decompress", b.ctypes.data, b.size, int(qp), int(height), int(width), int(entropy_coder_parallel),
+                   int(bool(reset_feature_memory)), _stream())
         return list(self._frames)
 
     # --- test hooks (not in the reference interface) ---
     def debug_copy(self, what, nbytes, dtype):
         out = np.zeros(nbytes // np.dtype(dtype).itemsize, dtype=dtype)
-        _lib.call("dcvc_hts_debug_copy", self._h, int(what), out.ctypes.data, out.nbytes)
+        self._call("debug_copy", int(what), out.ctypes.data, out.nbytes)
         return out
+
+
+class DMCHTSProxy(_ChunkProxy):
+    """DMCHTSProxy (dmc_hts_proxy.h:215-355; add_ref_feature_from_frame
+    dmc_hts_proxy.cpp:492-502, compress :504-585, decompress :587-710)."""
+    _PREFIX = "hts"
+    _NAME = "HT-S"
 
     def last_symbol_count(self):
         return int(self._lib.dcvc_hts_last_symbol_count(self._h))
 
 
-class DMCHTLProxy:
-    """DMCHTLProxy (dmc_htl_proxy.h): not built on MI355X yet."""
+class DMCHTLProxy(_ChunkProxy):
+    """DMCHTLProxy (dmc_htl_proxy.h; add_ref_feature_from_frame
+    dmc_htl_proxy.cpp:583-593, compress :595-716, decompress :718-915)."""
+    _PREFIX = "htl"
+    _NAME = "HT-L"
 
-    def __init__(self):
-        raise NotImplementedError("the HT-L model structure is not implemented by the MI355X build; use 'hts'")
+    def step_symbol_count(self, step):
+        return int(self._lib.dcvc_htl_step_symbol_count(self._h, step))
 
 
 class DMCLDProxy:

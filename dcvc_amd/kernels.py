@@ -119,6 +119,9 @@ def conv1x1(a, w, bias, epi="bias", s1=None, s2=None, q=None, qp=None, out=None)
     elif epi == "shortcut_quant":
         _lib_call("dcvc_conv1x1_bias_shortcut_with_quant", *base, _p(s1), _ld(s1), _p(q), qs, _p(qp),
                   _p(out), _ld(out), st)
+    elif epi == "shortcut_then_quant":
+        _lib_call("dcvc_conv1x1_bias_shortcut_then_quant", *base, _p(s1), _ld(s1), _p(q), qs, _p(qp),
+                  _p(out), _ld(out), st)
     elif epi == "quant":
         _lib_call("dcvc_conv1x1_bias_with_quant", *base, _p(q), qs, _p(qp), _p(out), _ld(out), st)
     elif epi == "quad":
@@ -153,6 +156,28 @@ def transposed_conv(x, n_img, H, W, wt_sub, cout, bias=None, out=None):
     _lib_call("dcvc_transposed_conv", _p(x), _ld(x), n_img, H, W, Cin, _p(wt_sub.contiguous()), cout,
               _p(bias.float().contiguous()), _p(out), _ld(out), _stream())
     return out
+
+
+def subpel_conv3x3(x, n_img, H, W, wt_sub, cout, bias, out=None):
+    """wt_sub [4*Cout, 9*Cin] sub-pixel-major rows, tap-major K; output [n_img*2H*2W, Cout]."""
+    Cin = wt_sub.shape[1] // 9
+    if out is None:
+        out = torch.empty((n_img * 4 * H * W, cout), dtype=torch.float16, device=x.device)
+    _lib_call("dcvc_subpel_conv3x3", _p(x), _ld(x), n_img, H, W, Cin, _p(wt_sub.contiguous()), cout,
+              _p(bias.float().contiguous()), _p(out), _ld(out), _stream())
+    return out
+
+
+def htl_quant_step(y, n_img, H, W, scales, means, qdec, ysf, yout, sym, cond, thr, step):
+    C = y.shape[1]
+    _lib_call("dcvc_htl_quant_step", _p(y), _ld(y), n_img, H, W, C, _p(scales), _ld(scales), _p(means),
+              _ld(means), _p(qdec), _ld(qdec), _p(ysf), _ld(ysf), _p(yout), 0 if yout is None else _ld(yout),
+              _p(sym), _p(cond), _p(thresholds(y.device)), float(thr), step, _stream())
+
+
+def htl_restore_step(yq, n_img, H, W, C, means, qdec, ysf, yout, step):
+    _lib_call("dcvc_htl_restore_step", _p(yq), n_img, H, W, C, _p(means), _ld(means), _p(qdec), _ld(qdec), _p(ysf),
+              _ld(ysf), _p(yout), 0 if yout is None else _ld(yout), step, _stream())
 
 
 def d3x3(x, n_img, H, W, wt9c, out=None):

@@ -20,6 +20,7 @@ torch-initialised reference model.
 from __future__ import annotations
 
 import math
+import re
 from collections import OrderedDict
 
 import numpy as np
@@ -159,7 +160,57 @@ def dmc_hts_schema():
     return s
 
 
-SCHEMAS = {"dmci": dmci_schema, "hts": dmc_hts_schema}
+def dmc_htl_schema():
+    """DMC with ModelStructure.HTL (video_model_ht.py:26-351, the ``not is_hts``
+    branches): ratio-1 DepthConvBlocks, a 3x3 biased sub-pixel decoder up
+    conv, force_bias hyper-decoder up convs, a scales+means spatial prior and
+    a 5-block recon head per frame."""
+    fd, src_i, src, y, z, d, m, rc = 8, 192, 192 * 8, 256, 128, 512, 512, 256
+    s = []
+    s += bit_estimator_schema("bit_estimator_z.", z)
+    s += dcb_schema("feature_adaptor_i.conv.0.", src_i, m)
+    for i in range(1, 3):
+        s += dcb_schema(f"feature_adaptor_i.conv.{i}.", m, m)
+    s += dcb_schema("feature_adaptor_m.conv.0.", m + d, m)
+    for i in range(1, 10):
+        s += dcb_schema(f"feature_adaptor_m.conv.{i}.", m, m)
+    s += dcb_schema("feature_extractor.conv.0.", m, d)
+    s += dcb_schema("feature_extractor.conv.1.", d, d)
+    s += dcb_schema("encoder.conv1.0.", src + d, d)
+    for i in range(1, 7):
+        s += dcb_schema(f"encoder.conv1.{i}.", d, d)
+    s += _conv("encoder.down.", y, d, k=3)
+    s += dcb_schema("hyper_encoder.conv.0.", y, y)
+    s += rbs2_schema("hyper_encoder.conv.1.", y, y)
+    s += rbs2_schema("hyper_encoder.conv.2.", y, z)
+    s += rbu_schema("hyper_decoder.conv.0.", z, y, force_bias=True)
+    s += rbu_schema("hyper_decoder.conv.1.", y, y, force_bias=True)
+    s += dcb_schema("hyper_decoder.conv.2.", y, y)
+    s += rbs2_schema("temporal_prior_encoder.conv.", d, y * 2)
+    for i in range(3):
+        s += dcb_schema(f"y_prior_fusion.conv.{i}.", y * 3, y * 3)
+    s += _conv("y_prior_fusion.conv.3.", y * 3, y * 3)
+    s += _conv("y_spatial_prior_reduction.", y, y * 3)
+    for i in (1, 2, 3):
+        s += dcb_schema(f"y_spatial_prior_adaptor_{i}.", y * 2, y * 2, force_adaptor=True)
+    for i in range(3):
+        s += dcb_schema(f"y_spatial_prior.conv.{i}.", y * 2, y * 2)
+    s += _conv("y_spatial_prior.conv.3.", y * 2, y * 2)
+    s += _conv("decoder.up.conv.0.", d * 4, y, k=3)
+    s += dcb_schema("decoder.conv1.0.", d * 2, d)
+    for i in range(1, 11):
+        s += dcb_schema(f"decoder.conv1.{i}.", d, d)
+    for i in range(fd):
+        s += dcb_schema(f"recon_head.conv.{i}.0.", d, rc)
+        for j in range(1, 5):
+            s += dcb_schema(f"recon_head.conv.{i}.{j}.", rc, rc)
+        s += _conv(f"recon_head.conv.{i}.5.", src_i, rc)
+    s += [("q_encoder", (QP_NUM, d), "ones"), ("q_decoder", (QP_NUM, d), "ones"),
+          ("q_feature", (QP_NUM, d), "ones")]
+    return s
+
+
+SCHEMAS = {"dmci": dmci_schema, "hts": dmc_hts_schema, "htl": dmc_htl_schema}
 
 
 def _xavier_std(shape, gain=1.0):
@@ -181,11 +232,13 @@ def _is_residual_branch_out(key):
 def _is_recon_projection(key):
     if key.startswith("dec.dec_2.") and key.endswith(".weight"):
         return any(s in key for s in ("adaptor.", "dc.3.", "ffn.2."))
-    return key.startswith("recon_head.conv2.") and key.endswith(".3.weight")
+    if key.startswith("recon_head.conv2.") and key.endswith(".3.weight"):
+        return True
+    return re.match(r"recon_head\.conv\.\d+\.5\.weight$", key) is not None
 
 
 def init_state_dict(model: str, seed: int = 0, q_jitter: float = 0.0, stable: bool = True):
-    """Return an OrderedDict of float32 numpy arrays for ``model`` in {'dmci','hts'}.
+    """Return an OrderedDict of float32 numpy arrays for ``model`` in {'dmci','hts','htl'}.
 
     ``stable`` (default) scales the output conv of every residual branch
     (``dc.3``, ``ffn.2``) by 0.25 and the reconstruction projections by 0.1.

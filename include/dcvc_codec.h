@@ -127,6 +127,38 @@ int dcvc_hts_debug_copy(void* h, int what, void* host, long bytes);
 int dcvc_dmci_debug_copy(void* h, int what, void* host, long bytes);
 /* Symbols handed to the entropy coder by the last call (per step for DMCI). */
 long dcvc_hts_last_symbol_count(void* h);
+/* DMCHTLProxy (dmc_htl_proxy.h; bind.cpp exports it as DMCHTLProxy).  Same
+ * call sequence and buffer contracts as the HT-S entry points above; the y
+ * latent is coded in four entropy round trips (dmc_htl_proxy.cpp:718-915). */
+void* dcvc_htl_create(void);
+void dcvc_htl_destroy(void* h);
+/* set_param (dmc_htl_proxy.cpp:917-967) */
+int dcvc_htl_set_param(void* h, int n, const char** names, const float** data, const int64_t* shapes,
+                       const int* ndims, float skip_thres);
+/* One weight copy for many stream contexts (no reference counterpart). */
+int dcvc_htl_share_weights(void* h, void* src);
+/* pre_allocate_tensors (dmc_htl_proxy.cpp:1042-1178) with 8 caller-owned
+ * [Hp, Wp, 3] reconstruction outputs */
+int dcvc_htl_prepare(void* h, int Hp, int Wp, void* const* frames, int ld_out);
+/* add_ref_feature_from_frame (dmc_htl_proxy.cpp:583-593) */
+int dcvc_htl_add_ref_feature_from_frame(void* h, const void* frame, int ld, int Hp, int Wp,
+                                        int apply_adaptor, void* stream);
+/* compress (dmc_htl_proxy.cpp:595-716): x is [H, W, 24], 8 frames on channels */
+int dcvc_htl_compress(void* h, const void* x, int ldx, int H, int W, int qp, int reset_feature_memory,
+                      int pad_b, int pad_r, void* stream, const uint8_t** bits, long* nbytes,
+                      int* ec_parallel);
+/* decompress (dmc_htl_proxy.cpp:718-915): writes the 8 prepared frames */
+int dcvc_htl_decompress(void* h, const uint8_t* bits, long nbytes, int qp, int height, int width,
+                        int ec_parallel, int reset_feature_memory, void* stream);
+int dcvc_htl_set_use_graphs(void* h, int on);
+/* Precision form, as dcvc_hts_set_precision (no reference counterpart). */
+int dcvc_htl_set_precision(void* h, int prec);
+long dcvc_htl_device_bytes(void* h);
+/* Test hook: 0 y_hat, 1 y, 2 q_dec|scales|means, 3 memory|feature, 4 x|ctx,
+ * 5-8 dense symbols of encoder step k, 9-12 their conditions, 13 z. */
+int dcvc_htl_debug_copy(void* h, int what, void* host, long bytes);
+/* Symbols (encoder) or indexes (decoder) of checkerboard step k in the last call. */
+long dcvc_htl_step_symbol_count(void* h, int step);
 /* Layer-table trace (no reference counterpart): between begin and end, every
  * GEMM / conv / depthwise launch issued from the host is recorded; end
  * returns the record count (or -status) and fills up to max_records rows of

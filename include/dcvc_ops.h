@@ -64,6 +64,13 @@ int dcvc_conv1x1_bias_shortcut_with_quant(const void* A, int lda, int M, int K, 
                                           int N, const float* bias, const void* S, int lds,
                                           const void* q, int qstride, const int* qp, void* D,
                                           int ldd, void* stream);
+/* conv1x1_bias_shortcut followed by multiply_with_broadcast_cuda, the HT-L
+ * block tail (dmc_htl_proxy.cpp:39-40, 102-103), in one pass:
+ * D = fp16(fp16(A W^T + b + S) * q[n]), both fp16 roundings kept. */
+int dcvc_conv1x1_bias_shortcut_then_quant(const void* A, int lda, int M, int K, const void* W,
+                                          int N, const float* bias, const void* S, int lds,
+                                          const void* q, int qstride, const int* qp, void* D,
+                                          int ldd, void* stream);
 /* conv1x1_bias_with_quant (conv1x1_bias_with_quant.cu:30-48): D = (A W^T + b) * q[n] */
 int dcvc_conv1x1_bias_with_quant(const void* A, int lda, int M, int K, const void* W, int N,
                                  const float* bias, const void* q, int qstride, const int* qp,
@@ -93,6 +100,14 @@ int dcvc_gemm_lowp(const void* A, int lda, int M, int K, const void* Wq, int N, 
  * W is [N][k*k][Cin] (tap-major, i.e. OHWI).  Output [n_img, Ho, Wo, N]. */
 int dcvc_conv_bias(const void* X, int ldx, int n_img, int H, int W, int Cin, const void* Wt, int N,
                    const float* bias, int kernel, int stride, void* D, int ldd, void* stream);
+
+/* SubpelConv2x with a 3x3 pad-1 kernel and bias (layers.py:92-103; the
+ * proxy's biased branch conv2d + pixel_shuffle_2_cuda, layers_proxy.cpp:
+ * 270-276), the shuffle fused into the GEMM store.  W is [4][Cout][9][Cin]
+ * (sub-pixel major, taps then input channels), bias [4*Cout] in the same
+ * order.  Output [n_img, 2H, 2W, Cout]. */
+int dcvc_subpel_conv3x3(const void* X, int ldx, int n_img, int H, int W, int Cin, const void* Wt,
+                        int Cout, const float* bias, void* D, int ldd, void* stream);
 
 /* transposed_conv (cutlass/transposed_conv.cu:101-119): SubpelConv2x with a
  * 1x1 kernel, i.e. 1x1 conv to 4*Cout channels then pixel_shuffle(2).
@@ -162,6 +177,23 @@ int dcvc_intra_build_index_dec(const void* S, int lds, int n_img, int H, int W, 
 int dcvc_intra_restore_step(const int8_t* yq, int n_img, int H, int W, int C, const void* Mn,
                             int ldm, void* Ysf, int ldsf, const void* q, int qstride,
                             const int* qp, void* Yout, int ldo, int step, void* stream);
+
+/* HT-L encoder mask step (dmc_htl_proxy.cpp:619-688: divide_with_clamp_min
+ * on y, process_with_mask_cuda, single_part_for_writing_4x, build_index_enc;
+ * add_and_multiply_with_clamp_min after step 3): quarter outputs as
+ * dcvc_intra_quant_step, with the elementwise quantisation step
+ * max(Qd, 0.5): y / max(Qd, .5) is quantised and step 3 writes
+ * Yout = (Ysf + y_hat_3) * max(Qd, .5).  Indexes on the decoder side come
+ * from dcvc_intra_build_index_dec. */
+int dcvc_htl_quant_step(const void* Y, int ldy, int n_img, int H, int W, int C, const void* S,
+                        int lds, const void* Mn, int ldm, const void* Qd, int ldq, void* Ysf,
+                        int ldsf, void* Yout, int ldo, int16_t* sym, uint8_t* cond,
+                        const float* th, float thr, int step, void* stream);
+/* restore_y_4x(_and_add_inplace) + add_and_multiply_with_clamp_min
+ * (dmc_htl_proxy.cpp:803-905) */
+int dcvc_htl_restore_step(const int8_t* yq, int n_img, int H, int W, int C, const void* Mn,
+                          int ldm, const void* Qd, int ldq, void* Ysf, int ldsf, void* Yout,
+                          int ldo, int step, void* stream);
 
 /* conditional_index_part1_cuda (stream.cu:175-340): stable compaction of N
  * elements of esize (1 or 2) bytes where cond != 0; *count (device int)

@@ -41,8 +41,14 @@ mistake; none of them retypes the oracle's formula):
   decoder rebuilds the encoder's y_hat and reconstruction exactly from its
   symbols (test_oracle_pins.py).  The DMCI composition is pinned only
   through its parts and the GPU encoder/decoder agreement.
-* hts/intra *_f32 step restatements: agree with the torch masked steps on
-  symbols, conditions and y_hat (test_oracle_pins.py, HT-S form).
+* hts/intra/htl *_f32 step restatements: agree with the torch masked steps
+  on symbols, conditions and y_hat (test_oracle_pins.py, HT-S and HT-L
+  forms; the HT-L form covers the DMCI quarter steps it is built on).
+* htl_* submodules: the reference HT-L modules on the same seeded weights
+  (golden vectors, test_oracle_golden.py).  htl_compress / htl_decompress:
+  composed of the pinned parts; the decoder rebuilds the encoder's indexes
+  in each of the four steps, y_hat, reconstruction and context exactly
+  (test_oracle_pins.py).
 * ref_gemm_epilogue / ref_dw3x3: torch float64 conv2d / sigmoid forms
   (test_oracle_pins.py).
 """
@@ -625,6 +631,220 @@ def hts_decompress(sd, st: HTSState, z_int8, decode_y, qp, height, width, reset_
 
 
 # ---------------------------------------------------------------------------
+# DMC HT-L — the ``not is_hts`` branches of video_model_ht.py with
+# dmc_htl_proxy.cpp:476-1178 semantics
+# ---------------------------------------------------------------------------
+
+def htl_feature_adaptor_i(sd, f):
+    """FeatureAdaptorI, HT-L: 3 ratio-1 blocks (video_model_ht.py:95-113)."""
+    return seq(sd, "feature_adaptor_i.conv.", 3, f)
+
+
+def htl_feature_adaptor_m(sd, memory, feature):
+    """FeatureAdaptorM, HT-L: 10 blocks over cat(memory, feature) (:116-143)."""
+    return seq(sd, "feature_adaptor_m.conv.", 10, torch.cat([memory, feature], 1))
+
+
+def htl_feature_extractor(sd, memory):
+    """FeatureExtractor, HT-L: 2 blocks (:146-164)."""
+    return seq(sd, "feature_extractor.conv.", 2, memory)
+
+
+def htl_recon_head(sd, feature, only_reset=False):
+    """ReconHead, HT-L (video_model_ht.py:236-248, 266-276): per frame 5
+    blocks and a 1x1 to 192 channels, then pixel_shuffle(8) and the proxy's
+    clamp to [-0.5, 0.5]; the reset branch runs frame 7's head only
+    (dmc_htl_proxy.cpp:347-359)."""
+    outs, heads = [], []
+    for i in ([7] if only_reset else range(8)):
+        out = seq(sd, f"recon_head.conv.{i}.", 5, feature)
+        out = conv(sd, f"recon_head.conv.{i}.5.", out)
+        heads.append(out)
+        outs.append(torch.clamp(F.pixel_shuffle(out, 8), -0.5, 0.5))
+    return outs, heads[-1]
+
+
+def htl_add_ref_feature_from_frame(sd, st: HTSState, frame, apply_adaptor=True):
+    """DMCHTLProxy::add_ref_feature_from_frame (dmc_htl_proxy.cpp:583-593)."""
+    st.feature_i = F.pixel_unshuffle(frame, 8)
+    if apply_adaptor:
+        st.memory = htl_feature_adaptor_i(sd, st.feature_i)
+        st.ctx = htl_feature_extractor(sd, st.memory)
+    st.memory_has_value = apply_adaptor
+    return st
+
+
+def htl_encoder(sd, x_padded, ctx, qp):
+    """Encoder, HT-L (video_model_ht.py:63-92): 7 blocks, * q_encoder as a
+    separate multiply (dmc_htl_proxy.cpp:93-106), 3x3 stride-2 down conv."""
+    feat = torch.cat([F.pixel_unshuffle(x_padded, 8), ctx], 1)
+    feat = seq(sd, "encoder.conv1.", 7, feat)
+    feat = feat * qsel(sd, "q_encoder", qp).reshape(1, -1, 1, 1)
+    return conv(sd, "encoder.down.", feat, stride=2, padding=1)
+
+
+def htl_hyper_encoder(sd, y_pad):
+    """HyperEncoder, HT-L: the stride-2 blocks keep their shortcut (:184-198)."""
+    z = dcb(sd, "hyper_encoder.conv.0.", y_pad)
+    z = rbs2(sd, "hyper_encoder.conv.1.", z, True)
+    return rbs2(sd, "hyper_encoder.conv.2.", z, True)
+
+
+def htl_hyper_decoder(sd, z_hat):
+    """HyperDecoder, HT-L: force_bias up convs, shortcut blocks (:167-181)."""
+    hyper = rbu(sd, "hyper_decoder.conv.0.", z_hat, True)
+    hyper = rbu(sd, "hyper_decoder.conv.1.", hyper, True)
+    return dcb(sd, "hyper_decoder.conv.2.", hyper)
+
+
+def htl_temporal_prior(sd, memory, qp):
+    """TemporalPriorEncoder, HT-L: shortcut stride-2 block (:307-317)."""
+    return rbs2(sd, "temporal_prior_encoder.conv.", memory * qsel(sd, "q_feature", qp).reshape(1, -1, 1, 1), True)
+
+
+def _htl_prior(sd, z_hat, memory, qp):
+    temporal = htl_temporal_prior(sd, memory, qp)
+    hyper = htl_hyper_decoder(sd, z_hat)
+    hyper = hyper[:, :, :temporal.shape[2], :temporal.shape[3]]
+    x = torch.cat([hyper, temporal], 1)
+    for i in range(3):
+        x = dcb(sd, f"y_prior_fusion.conv.{i}.", x)
+    return conv(sd, "y_prior_fusion.conv.3.", x)
+
+
+def _htl_spatial_prior(sd, i, y_hat_so_far, reduced):
+    """SpatialPriorAdaptor_i on cat(y_hat_so_far, reduced) -> SpatialPrior ->
+    (scales, means) (common_model.py:237-282, spatial_prior_has_scales)."""
+    x = dcb(sd, f"y_spatial_prior_adaptor_{i}.", torch.cat([y_hat_so_far, reduced], 1))
+    for j in range(3):
+        x = dcb(sd, f"y_spatial_prior.conv.{j}.", x)
+    return conv(sd, "y_spatial_prior.conv.3.", x).chunk(2, 1)
+
+
+def _htl_decoder(sd, y_hat, ctx, qp):
+    """Decoder, HT-L: 3x3 biased sub-pixel up conv, 11 blocks, * q_decoder
+    as a separate multiply (dmc_htl_proxy.cpp:22-42)."""
+    up = F.pixel_shuffle(conv(sd, "decoder.up.conv.0.", y_hat, padding=1), 2)
+    feat = seq(sd, "decoder.conv1.", 11, torch.cat([up, ctx], 1))
+    return feat * qsel(sd, "q_decoder", qp).reshape(1, -1, 1, 1)
+
+
+def _quarter(x, mask):
+    """single_part_for_writing/reading_4x (stream.cu:896-962): the sum of the
+    four channel groups of x * mask, i.e. the active group's value."""
+    q = x.shape[1] // 4
+    xm = x * mask.to(x.dtype)
+    return xm[:, :q] + xm[:, q:2 * q] + xm[:, 2 * q:3 * q] + xm[:, 3 * q:]
+
+
+@dataclass
+class HTLResult:
+    steps_sym16: list = field(default_factory=list)   # per step: compacted int16 (quarter order)
+    steps_dense: list = field(default_factory=list)
+    steps_cond: list = field(default_factory=list)
+    z_int8: np.ndarray = None
+    y: torch.Tensor = None                             # y * (1 / q_dec)
+    y_hat: torch.Tensor = None
+    common_params: torch.Tensor = None
+    x_hat: list = None
+
+
+def htl_compress(sd, st: HTSState, x, qp, reset_feature_memory, skip_thres=0.0, recon=False):
+    """DMCHTLProxy::compress (dmc_htl_proxy.cpp:595-716): four checkerboard
+    steps, each with its own scales and means (step 0 from the common
+    params, steps 1-3 from the spatial prior over the y_hat accumulated so
+    far) and its own quarter-tensor symbol stream; then the decoder and the
+    context update."""
+    y = htl_encoder(sd, pad_to(x, 16), st.ctx, qp)
+    z = htl_hyper_encoder(sd, pad_to(y, 4))
+    z_hat = round_z(z)
+    params = _htl_prior(sd, z_hat, st.memory, qp)
+    res = htl_code_y(sd, y, params, skip_thres)
+    res.z_int8 = z_hat.permute(0, 2, 3, 1).reshape(-1).numpy().astype(np.int8)
+    st.feature_p = _htl_decoder(sd, res.y_hat, st.ctx, qp)
+    if recon:
+        res.x_hat, _ = htl_recon_head(sd, st.feature_p)
+    if reset_feature_memory:
+        _, head = htl_recon_head(sd, st.feature_p, only_reset=True)
+        st.memory = htl_feature_adaptor_i(sd, head)
+    else:
+        st.memory = htl_feature_adaptor_m(sd, st.memory, st.feature_p)
+    st.ctx = htl_feature_extractor(sd, st.memory)
+    st.memory_has_value = True
+    return res
+
+
+def htl_code_y(sd, y, params, skip_thres=0.0):
+    """The four checkerboard steps of DMCHTLProxy::compress
+    (dmc_htl_proxy.cpp:619-688) from the encoder output y and the common
+    params: y / max(q_dec, .5), step 0 on the common scales / means, steps
+    1-3 on the spatial prior over the y_hat accumulated so far; returns an
+    HTLResult with the per-step symbol streams and the final y_hat."""
+    q_dec, scales, means = params.chunk(3, 1)
+    q_dec = torch.clamp_min(q_dec, 0.5)
+    y = y * (1.0 / q_dec)
+    reduced = conv(sd, "y_spatial_prior_reduction.", params)
+    C, yH, yW = y.shape[1], y.shape[2], y.shape[3]
+    masks = mask_4x(C, yH, yW)
+    res = HTLResult(y=y, common_params=params)
+    so_far = torch.zeros_like(y)
+    for step in range(4):
+        if step > 0:
+            scales, means = _htl_spatial_prior(sd, step, so_far, reduced)
+        y_q, yh = quantize_masked(y, means, scales, masks[step], skip_thres)
+        sym16, _, cond = build_index(_quarter(y_q, masks[step]), _quarter(scales, masks[step]), skip_thres)
+        res.steps_sym16.append(compact(sym16, cond))
+        res.steps_dense.append(sym16)
+        res.steps_cond.append(cond)
+        if step < 3:
+            so_far = so_far + yh
+        else:
+            res.y_hat = (yh + so_far) * q_dec
+    return res
+
+
+def htl_decompress(sd, st: HTSState, z_int8, decode_y, qp, height, width, reset_feature_memory,
+                   skip_thres=0.0):
+    """DMCHTLProxy::decompress (dmc_htl_proxy.cpp:718-915): four entropy
+    round trips.  ``decode_y(step, indexes_uint8) -> int8 symbols``."""
+    if st.memory_has_value:
+        st.memory = htl_feature_adaptor_m(sd, st.memory, st.feature_p)
+    else:
+        st.memory = htl_feature_adaptor_i(sd, st.feature_i)
+    H16, W16 = (height + 15) // 16, (width + 15) // 16
+    H64, W64 = (H16 + 3) // 4, (W16 + 3) // 4
+    z_hat = torch.from_numpy(z_int8.astype(np.float32)).reshape(1, H64, W64, -1).permute(0, 3, 1, 2)
+    z_hat = z_hat.to(st.memory.dtype).contiguous()
+    params = _htl_prior(sd, z_hat, st.memory, qp)
+    st.ctx = htl_feature_extractor(sd, st.memory)
+    q_dec, scales, means = params.chunk(3, 1)
+    q_dec = torch.clamp_min(q_dec, 0.5)
+    reduced = conv(sd, "y_spatial_prior_reduction.", params)
+    C = scales.shape[1]
+    masks = mask_4x(C, H16, W16)
+    so_far = torch.zeros((1, C, H16, W16), dtype=params.dtype)
+    y_hat = None
+    for step in range(4):
+        if step > 0:
+            scales, means = _htl_spatial_prior(sd, step, so_far, reduced)
+        _, idx, cond = build_index(None, _quarter(scales, masks[step]), skip_thres)
+        decoded = decode_y(step, compact(idx, cond))
+        yq = recover(decoded, cond, idx.size)
+        yq = torch.from_numpy(yq.astype(np.float32)).reshape(1, H16, W16, C // 4).permute(0, 3, 1, 2)
+        yq = yq.to(params.dtype).contiguous().repeat(1, 4, 1, 1)
+        yh = (yq + means) * masks[step].to(params.dtype)
+        if step < 3:
+            so_far = so_far + yh
+        else:
+            y_hat = (yh + so_far) * q_dec
+    st.feature_p = _htl_decoder(sd, y_hat, st.ctx, qp)
+    x_hat, head = htl_recon_head(sd, st.feature_p)
+    st.feature_i = head
+    st.memory_has_value = not reset_feature_memory
+    return x_hat, y_hat
+
+
+# ---------------------------------------------------------------------------
 # per-op references used by the GPU kernel parity tests (float64)
 # ---------------------------------------------------------------------------
 
@@ -738,7 +958,7 @@ def intra_quant_step_f32(y, scales, means, ysf_in, q_row, step, thr, C, H, W, n_
     ysf = np.where(act, yh, old).astype(np.float16)
     yout = None
     if step == 3:
-        yout = (ysf.astype(np.float32) * q_row.astype(np.float32)[None, :]).astype(np.float16)
+        yout = (ysf.astype(np.float32) * _qrows(q_row)).astype(np.float16)
         ysf = ysf_in
     return ysf, yout, sym, cond
 
@@ -751,5 +971,26 @@ def intra_restore_step_f32(yq_quarter, means, ysf_in, q_row, step, C, H, W, n_im
     old = np.zeros_like(ysf_in) if step == 0 else ysf_in
     ysf = np.where(act, yh, old).astype(np.float16)
     if step == 3:
-        return ysf_in, (ysf.astype(np.float32) * q_row.astype(np.float32)[None, :]).astype(np.float16)
+        return ysf_in, (ysf.astype(np.float32) * _qrows(q_row)).astype(np.float16)
     return ysf, None
+
+
+def _qrows(q):
+    """A per-channel [C] scale as a row, or an elementwise [P, C] scale as is."""
+    q = np.asarray(q).astype(np.float32)
+    return q[None, :] if q.ndim == 1 else q
+
+
+def htl_quant_step_f32(y, scales, means, qdec, ysf_in, step, thr, C, H, W, n_img=1):
+    """HT-L encoder step: the DMCI quarter step on y / max(q_dec, .5), whose
+    step-3 output is scaled by max(q_dec, .5) elementwise
+    (divide_with_clamp_min, add_and_multiply_with_clamp_min; stream.cu:40-66,
+    422-443).  -> (ysf, yout or None, sym quarter, cond quarter)"""
+    q = np.maximum(qdec.astype(np.float32), f32(0.5))
+    ys = (y.astype(np.float32) * (f32(1.0) / q)).astype(np.float32)
+    return intra_quant_step_f32(ys, scales, means, ysf_in, q, step, thr, C, H, W, n_img)
+
+
+def htl_restore_step_f32(yq_quarter, means, qdec, ysf_in, step, C, H, W, n_img=1):
+    q = np.maximum(qdec.astype(np.float32), f32(0.5))
+    return intra_restore_step_f32(yq_quarter, means, ysf_in, q, step, C, H, W, n_img)

@@ -46,6 +46,8 @@ def load(model, sd):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--reference", default=os.environ.get("DCVC_REFERENCE", "/root/reference"))
+    ap.add_argument("--parts", default="base,htl",
+                    help="comma list: base (DMCI, HT-S, masks, CDFs), htl (HT-L modules)")
     args = ap.parse_args()
     sys.path.insert(0, args.reference)
     sys.path.insert(0, os.path.join(REPO, "oracle", "_ref"))
@@ -53,6 +55,15 @@ def main():
     from src.models.video_model_ht import DMC
     from src.utils.common import ModelStructure
 
+    parts = args.parts.split(",")
+    if "base" in parts:
+        save(base_vectors(DMCI, DMC, ModelStructure), "reference_modules.npz")
+    if "htl" in parts:
+        save(htl_vectors(DMC, ModelStructure), "reference_htl.npz")
+
+
+def base_vectors(DMCI, DMC, ModelStructure):
+    """DMCI and HT-S submodules, masks, scale table and CDF tables."""
     torch.manual_seed(0)
     out = {}
     with torch.no_grad():
@@ -131,6 +142,10 @@ def main():
         out["cdf.hts_z.quantized_cdf"] = torch.from_numpy(zc)
         out["cdf.hts_z.cdf_length"] = torch.from_numpy(zl)
 
+    return out
+
+
+def save(out, name):
     arrays = {}
     for k, v in out.items():
         a = v.detach().numpy() if isinstance(v, torch.Tensor) else np.asarray(v)
@@ -138,9 +153,53 @@ def main():
             assert np.array_equal(a.astype(np.float16).astype(a.dtype), a)
             a = a.astype(np.float16)
         arrays[k] = a
-    path = os.path.join(HERE, "reference_modules.npz")
+    path = os.path.join(HERE, name)
     np.savez_compressed(path, **arrays)
     print(f"wrote {path}: {len(arrays)} arrays, {os.path.getsize(path) / 1e6:.2f} MB")
+
+
+def htl_vectors(DMC, ModelStructure):
+    """Every HT-L submodule the proxy runs (the ``not is_hts`` branches of
+    video_model_ht.py), on seeded inputs at 64 x 64 pixels."""
+    out = {}
+    with torch.no_grad():
+        sd = weights.round_to_half(weights.perturb_biases(weights.init_state_dict("htl", 2), 102))
+        m = load(DMC(ModelStructure.HTL), sd)
+        H8 = W8 = 8
+        qe = torch.from_numpy(sd["q_encoder"][11])[None, :, None, None]
+        qdd = torch.from_numpy(sd["q_decoder"][11])[None, :, None, None]
+        qf = torch.from_numpy(sd["q_feature"][11])[None, :, None, None]
+        fi = seeded((1, 192, H8, W8), 20, 0.3)
+        out["htl.fa_i.in"] = fi
+        out["htl.fa_i.out"] = mem = m.feature_adaptor_i(fi)
+        fp = seeded((1, 512, H8, W8), 21, 0.5)
+        out["htl.fa_m.in_feature"] = fp
+        out["htl.fa_m.out"] = m.feature_adaptor_m(mem, fp)
+        out["htl.fe.out"] = ctx = m.feature_extractor(mem)
+        x = seeded((1, 24, 64, 64), 22, 0.25)
+        out["htl.encoder.in"] = x
+        out["htl.encoder.out"] = y = m.encoder(x, ctx, qe)
+        out["htl.hyper_encoder.out"] = z = m.hyper_encoder(y)
+        z_hat = torch.round(z * 3)
+        out["htl.hyper_decoder.in"] = z_hat
+        out["htl.hyper_decoder.out"] = hp = m.hyper_decoder(z_hat)
+        out["htl.tpe.out"] = tp = m.temporal_prior_encoder(mem, qf)
+        out["htl.prior_fusion.out"] = params = m.y_prior_fusion(hp, tp)
+        out["htl.reduction.out"] = red = m.y_spatial_prior_reduction(params)
+        yh = seeded((1, 256, 4, 4), 23)
+        out["htl.sp.in_yhat"] = yh
+        cat = torch.cat((yh, red), dim=1)
+        for i in (1, 2, 3):
+            a = getattr(m, f"y_spatial_prior_adaptor_{i}")(cat)
+            out[f"htl.sp{i}.out"] = m.y_spatial_prior(a)
+        yd = seeded((1, 256, 4, 4), 24)
+        out["htl.decoder.in"] = yd
+        out["htl.decoder.out"] = feat = m.decoder(yd, ctx, qdd)
+        recon = m.recon_head(feat)
+        for i, r in enumerate(recon):
+            out[f"htl.recon.{i}"] = r
+        out["htl.recon.reset"] = m.recon_head(feat, for_reset=True)
+    return out
 
 
 if __name__ == "__main__":

@@ -134,7 +134,8 @@ def test_gemm_batch_invariance(gpu):
     assert torch.equal(full[:100], small)
 
 
-@pytest.mark.parametrize("epi", ["bias", "wsilu", "shortcut2", "shortcut_quant", "quad", "s2k3", "s1k3", "subpel"])
+@pytest.mark.parametrize("epi", ["bias", "wsilu", "shortcut2", "shortcut_quant", "shortcut_then_quant", "quad", "s2k3",
+                                 "s1k3", "subpel", "subpel3"])
 def test_gemm_tile_configs_bit_identical(gpu, epi):
     """Every tile configuration gives the same bits (same per-element accumulation
     order), including ragged M / N tails, implicit convs and the sub-pixel store."""
@@ -161,6 +162,10 @@ def test_gemm_tile_configs_bit_identical(gpu, epi):
             x = f16(r0.standard_normal((H * W, cin)), gpu)
             w = f16(r0.standard_normal((4 * 64, cin)) / 10, gpu)
             return kernels.transposed_conv(x, 1, H, W, w, 64, torch.ones(4 * 64, device=gpu))
+        if epi == "subpel3":
+            x = f16(r0.standard_normal((H * W, cin)), gpu)
+            w = f16(r0.standard_normal((4 * 64, 9 * cin)) / 30, gpu)
+            return kernels.subpel_conv3x3(x, 1, H, W, w, 64, torch.ones(4 * 64, device=gpu))
         w = f16(r0.standard_normal((N, K)) / 20, gpu)
         return kernels.conv1x1(a, w, torch.full((N,), 0.1, device=gpu), epi=epi, s1=s1, s2=s2, q=qtab, qp=qp)
 
@@ -240,6 +245,51 @@ def test_transposed_conv(gpu, H, W, cin, cout, n_img):
     out = kernels.transposed_conv(f16(x.reshape(-1, cin), gpu), n_img, H, W, f16(w_sub, gpu), cout)
     torch.cuda.synchronize()
     assert_fp16_close(out, ref, acc, "transposed")
+
+
+@pytest.mark.parametrize("H,W,cin,cout,n_img", [(17, 30, 256, 512, 1), (8, 12, 128, 64, 2), (1, 3, 64, 32, 1)])
+def test_subpel_conv3x3(gpu, H, W, cin, cout, n_img):
+    """SubpelConv2x(k=3, padding=1, bias) (layers.py:92-103): pixel_shuffle(conv3x3(x) + b, 2),
+    the HT-L decoder up conv, against a float64 torch reference."""
+    from dcvc_amd import kernels
+    r = rng(H * 3 + cin)
+    x = r.standard_normal((n_img, H, W, cin)).astype(np.float16)
+    w = (r.standard_normal((4 * cout, cin, 3, 3)) / np.sqrt(9 * cin)).astype(np.float16)  # up.conv.0.weight
+    b = (r.standard_normal(4 * cout) * 0.1).astype(np.float16).astype(np.float32)
+    xt = torch.from_numpy(x.astype(np.float64)).permute(0, 3, 1, 2)
+    ref = F.pixel_shuffle(F.conv2d(xt, torch.from_numpy(w.astype(np.float64)), torch.from_numpy(b.astype(np.float64)),
+                                   padding=1), 2)
+    ref = ref.permute(0, 2, 3, 1).reshape(-1, cout).numpy()
+    acc = F.pixel_shuffle(F.conv2d(xt.abs(), torch.from_numpy(np.abs(w.astype(np.float64))), padding=1), 2)
+    acc = acc.permute(0, 2, 3, 1).reshape(-1, cout).numpy()
+    # GEMM row sub*Cout + c <- conv channel c*4 + sub; K = tap*Cin + ci
+    w_sub = w.reshape(cout, 4, cin, 9).transpose(1, 0, 3, 2).reshape(4 * cout, 9 * cin)
+    b_sub = b.reshape(cout, 4).T.reshape(-1)
+    out = kernels.subpel_conv3x3(f16(np.ascontiguousarray(x.reshape(-1, cin)), gpu), n_img, H, W,
+                                 f16(np.ascontiguousarray(w_sub), gpu), cout, torch.from_numpy(b_sub.copy()).to(gpu))
+    torch.cuda.synchronize()
+    assert_fp16_close(out, ref, acc, "subpel3x3")
+
+
+def test_shortcut_then_quant_equals_separate_multiply(gpu):
+    """The HT-L block tail fused in one GEMM equals the reference's two
+    kernels, conv1x1_bias_shortcut then multiply_with_broadcast, bit for bit;
+    the single-rounding fused form (HT-S) differs on some elements."""
+    from dcvc_amd import kernels
+    r = rng(17)
+    M, K, N = 3001, 512, 512
+    a = f16(r.standard_normal((M, K)), gpu)
+    w = f16(r.standard_normal((N, K)) / 20, gpu)
+    b = torch.from_numpy((r.standard_normal(N) * 0.1).astype(np.float32)).to(gpu)
+    s1 = f16(r.standard_normal((M, N)), gpu)
+    qtab = f16(0.5 + r.random((64, N)), gpu)
+    qp = torch.tensor([37], dtype=torch.int32, device=gpu)
+    fused = kernels.conv1x1(a, w, b, epi="shortcut_then_quant", s1=s1, q=qtab, qp=qp)
+    two = kernels.multiply_with_broadcast(kernels.conv1x1(a, w, b, epi="shortcut", s1=s1), qtab, qp)
+    single = kernels.conv1x1(a, w, b, epi="shortcut_quant", s1=s1, q=qtab, qp=qp)
+    torch.cuda.synchronize()
+    assert torch.equal(fused.view(torch.int16), two.view(torch.int16))
+    assert not torch.equal(single.view(torch.int16), two.view(torch.int16))
 
 
 @pytest.mark.parametrize("H,W,C,n_img", [(136, 240, 256, 1), (17, 30, 128, 2), (1, 1, 64, 1), (68, 120, 768, 1)])
@@ -393,6 +443,56 @@ def test_intra_quant_and_restore_steps(gpu, H, W, n_img, thr):
         kernels.intra_restore_step(torch.from_numpy(yq).to(gpu), n_img, H, W, C, f16(mn_steps[step], gpu), ysf_d,
                                    f16(q, gpu), qp, yout_d, step)
     torch.cuda.synchronize()
+    assert np.array_equal(yout_d.cpu().numpy().view(np.uint16), yout_r.view(np.uint16))
+
+
+@pytest.mark.parametrize("H,W,n_img,thr", [(16, 16, 1, 0.0), (17, 30, 2, 0.15), (68, 120, 1, 0.15)])
+def test_htl_quant_and_restore_steps(gpu, H, W, n_img, thr):
+    """HT-L boundary: per-step scales and means, y / max(q_dec, .5) quantised,
+    (y_hat_3 + y_hat_so_far) * max(q_dec, .5) at the end; bit-exact against the
+    float32 restatement, and the decoder's indexes and restore reproduce the
+    encoder's quarters and y_hat from its symbols."""
+    from dcvc_amd import kernels
+    r = rng(H * 7 + n_img)
+    C, P = 256, n_img * H * W
+    y, _, _, qdec = _rand_prior(r, P, C)
+    qdec[:, ::9] = np.float16(0.125)                            # below the 0.5 clamp
+    sc_steps = [np.abs(r.standard_normal((P, C)) * 3).astype(np.float16) for _ in range(4)]
+    for sc in sc_steps:
+        sc[:, ::11] = np.float16(thr)                           # exactly at the skip threshold
+    mn_steps = [(r.standard_normal((P, C)) * 2).astype(np.float16) for _ in range(4)]
+    ysf_g = torch.full((P, C), 9.0, dtype=torch.float16, device=gpu)
+    yout_g = torch.zeros((P, C), dtype=torch.float16, device=gpu)
+    ysf_r = np.full((P, C), 9.0, np.float16)
+    syms = []
+    for step in range(4):
+        sym_g = torch.zeros(P * C // 4, dtype=torch.int16, device=gpu)
+        cond_g = torch.zeros(P * C // 4, dtype=torch.uint8, device=gpu)
+        kernels.htl_quant_step(f16(y, gpu), n_img, H, W, f16(sc_steps[step], gpu), f16(mn_steps[step], gpu),
+                               f16(qdec, gpu), ysf_g, yout_g, sym_g, cond_g, thr, step)
+        ysf_r, yout_r, s, c = O.htl_quant_step_f32(y, sc_steps[step], mn_steps[step], qdec, ysf_r, step, thr,
+                                                   C, H, W, n_img)
+        torch.cuda.synchronize()
+        assert np.array_equal(sym_g.cpu().numpy(), s), f"step {step}"
+        assert np.array_equal(cond_g.cpu().numpy(), c), f"step {step}"
+        if step < 3:
+            assert np.array_equal(ysf_g.cpu().numpy().view(np.uint16), ysf_r.view(np.uint16)), f"step {step}"
+        syms.append((s, c))
+        idx, cnd = kernels.intra_build_index_dec(f16(sc_steps[step], gpu), n_img, H, W, thr, step)
+        assert np.array_equal(cnd.cpu().numpy(), c)
+        assert np.array_equal(idx.cpu().numpy(), (s.astype(np.int64) & 0xFF).astype(np.uint8))
+    assert np.array_equal(yout_g.cpu().numpy().view(np.uint16), yout_r.view(np.uint16))
+    ysf_d = torch.full((P, C), -1.0, dtype=torch.float16, device=gpu)
+    yout_d = torch.zeros((P, C), dtype=torch.float16, device=gpu)
+    ysf_rr = np.full((P, C), -1.0, np.float16)
+    for step in range(4):
+        s, c = syms[step]
+        yq = np.where(c > 0, (s.astype(np.int32) >> 8), 0).astype(np.int8)
+        kernels.htl_restore_step(torch.from_numpy(yq).to(gpu), n_img, H, W, C, f16(mn_steps[step], gpu),
+                                 f16(qdec, gpu), ysf_d, yout_d, step)
+        ysf_rr, yo = O.htl_restore_step_f32(yq, mn_steps[step], qdec, ysf_rr, step, C, H, W, n_img)
+    torch.cuda.synchronize()
+    assert np.array_equal(yo.view(np.uint16), yout_r.view(np.uint16))
     assert np.array_equal(yout_d.cpu().numpy().view(np.uint16), yout_r.view(np.uint16))
 
 

@@ -104,3 +104,42 @@ def test_cdf_tables_match_reference():
                                                  torch.from_numpy(sd["bit_estimator_z.a"]))
         assert np.array_equal(zl, G[f"cdf.{key}.cdf_length"])
         assert np.array_equal(zc, G[f"cdf.{key}.quantized_cdf"])
+
+
+GL = np.load(os.path.join(os.path.dirname(__file__), "golden", "reference_htl.npz"))
+
+
+def gl(key):
+    return torch.from_numpy(GL[key].astype(np.float32))
+
+
+@pytest.fixture(scope="module")
+def sd_htl():
+    sd = weights.round_to_half(weights.perturb_biases(weights.init_state_dict("htl", 2), 102))
+    return weights.to_torch(sd)
+
+
+def test_htl_modules(sd_htl):
+    """Every HT-L submodule of the oracle against the reference modules
+    (video_model_ht.py, ``not is_hts`` branches) on the same weights."""
+    sd = sd_htl
+    mem = O.htl_feature_adaptor_i(sd, gl("htl.fa_i.in"))
+    close(mem, gl("htl.fa_i.out"))
+    close(O.htl_feature_adaptor_m(sd, gl("htl.fa_i.out"), gl("htl.fa_m.in_feature")), gl("htl.fa_m.out"))
+    close(O.htl_feature_extractor(sd, gl("htl.fa_i.out")), gl("htl.fe.out"))
+    close(O.htl_encoder(sd, gl("htl.encoder.in"), gl("htl.fe.out"), 11), gl("htl.encoder.out"))
+    close(O.htl_hyper_encoder(sd, gl("htl.encoder.out")), gl("htl.hyper_encoder.out"))
+    close(O.htl_hyper_decoder(sd, gl("htl.hyper_decoder.in")), gl("htl.hyper_decoder.out"))
+    close(O.htl_temporal_prior(sd, gl("htl.fa_i.out"), 11), gl("htl.tpe.out"))
+    close(O._htl_prior(sd, gl("htl.hyper_decoder.in"), gl("htl.fa_i.out"), 11), gl("htl.prior_fusion.out"))
+    close(O.conv(sd, "y_spatial_prior_reduction.", gl("htl.prior_fusion.out")), gl("htl.reduction.out"))
+    for i in (1, 2, 3):
+        sc, mn = O._htl_spatial_prior(sd, i, gl("htl.sp.in_yhat"), gl("htl.reduction.out"))
+        close(torch.cat([sc, mn], 1), gl(f"htl.sp{i}.out"))
+    close(O._htl_decoder(sd, gl("htl.decoder.in"), gl("htl.fe.out"), 11), gl("htl.decoder.out"))
+    outs, head = O.htl_recon_head(sd, gl("htl.decoder.out"))
+    for i in range(8):
+        close(outs[i], torch.clamp(gl(f"htl.recon.{i}"), -0.5, 0.5))
+    _, reset_head = O.htl_recon_head(sd, gl("htl.decoder.out"), only_reset=True)
+    close(reset_head, gl("htl.recon.reset"))
+    close(head, gl("htl.recon.reset"))

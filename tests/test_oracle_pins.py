@@ -155,3 +155,72 @@ def test_oracle_hts_encode_decode_round_trip():
     for a, b in zip(x_hat, r.x_hat):
         assert torch.equal(a, b)
     assert torch.equal(st_d.feature_p, st_e.feature_p)
+
+
+def test_oracle_htl_encode_decode_round_trip():
+    """HT-L: four entropy round trips.  The oracle decoder, fed each step's
+    compacted symbols, asks for exactly the encoder's CDF indexes in every
+    step and rebuilds y_hat, the reconstruction and the context exactly; the
+    four steps partition y (each element coded once)."""
+    sd = {k: v.float() for k, v in weights.to_torch(weights.round_to_half(weights.init_state_dict("htl", 2))).items()}
+    H = W = 64
+    frames = synthetic.motion_sequence_yuv420(H, W, 9, seed=3)
+    ref = torch.from_numpy(synthetic.frame_to_input(frames[0]))[None]
+    x = torch.from_numpy(synthetic.chunk_to_input(frames[1:9]))[None]
+    st_e, st_d = O.HTSState(), O.HTSState()
+    with torch.inference_mode():
+        O.htl_add_ref_feature_from_frame(sd, st_e, ref, True)
+        O.htl_add_ref_feature_from_frame(sd, st_d, ref, False)
+        r = O.htl_compress(sd, st_e, x, 21, False, skip_thres=0.15, recon=True)
+        seen = []
+
+        def decode_y(step, idx):
+            seen.append(idx)
+            return (r.steps_sym16[step].astype(np.int64) >> 8).astype(np.int8)
+
+        x_hat, y_hat = O.htl_decompress(sd, st_d, r.z_int8, decode_y, 21, H, W, False, skip_thres=0.15)
+    assert len(seen) == 4
+    for step in range(4):
+        assert np.array_equal(seen[step], (r.steps_sym16[step].astype(np.int64) & 0xFF).astype(np.uint8))
+        assert r.steps_dense[step].size == r.y.numel() // 4
+    assert torch.equal(y_hat, r.y_hat)
+    for a, b in zip(x_hat, r.x_hat):
+        assert torch.equal(a, b)
+    assert torch.equal(st_d.feature_p, st_e.feature_p)
+    # y_hat / q_dec - y is the quantisation residual: |.| <= 1/2 where the
+    # element was coded (scale > skip threshold) and not clamped
+    qd = torch.clamp_min(r.common_params[:, :256], 0.5)
+    res = (r.y_hat / qd - r.y).abs()
+    assert float(res.median()) < 0.5
+
+
+def test_htl_fp32_step_restatement_agrees_with_torch_steps():
+    """HT-L form: the float32 quarter-step restatement (NHWC, y / max(q, .5),
+    per-step scales) and the torch NCHW masked steps with single_part_for_
+    writing_4x give the same quarter symbols and conditions, and the same
+    final y_hat = (y_hat_3 + y_hat_so_far) * max(q, .5)."""
+    r = np.random.Generator(np.random.PCG64(8))
+    C, H, W = 8, 4, 6
+    P = H * W
+    y = (r.standard_normal((P, C)) * 3).astype(np.float16)
+    qd = (r.random((P, C)) * 1.5).astype(np.float16)           # some below the 0.5 clamp
+    thr = 0.15
+    to_nchw = lambda a: torch.from_numpy(a.astype(np.float32).reshape(1, H, W, C)).permute(0, 3, 1, 2)
+    masks = O.mask_4x(C, H, W)
+    q32 = np.maximum(qd.astype(np.float32), np.float32(0.5))
+    ys_t = to_nchw((y.astype(np.float32) * (np.float32(1) / q32)).astype(np.float32))
+    ysf = np.full((P, C), 7.0, np.float16)
+    so_far = torch.zeros(1, C, H, W)
+    for step in range(4):
+        sc = np.exp(r.uniform(np.log(0.05), np.log(20), (P, C))).astype(np.float16)
+        mn = r.standard_normal((P, C)).astype(np.float16)
+        ysf, yout, sym, cond = O.htl_quant_step_f32(y, sc, mn, qd, ysf, step, thr, C, H, W)
+        y_q, yh = O.quantize_masked(ys_t, to_nchw(mn), to_nchw(sc), masks[step], thr)
+        sym_t, _, cond_t = O.build_index(O._quarter(y_q, masks[step]), O._quarter(to_nchw(sc), masks[step]), thr)
+        assert np.array_equal(sym, sym_t), f"step {step}"
+        assert np.array_equal(cond.astype(bool), cond_t), f"step {step}"
+        if step < 3:
+            so_far = so_far + yh
+        else:
+            ref = ((yh + so_far) * to_nchw(q32)).permute(0, 2, 3, 1).reshape(P, C).numpy()
+            np.testing.assert_allclose(yout.astype(np.float32), ref, rtol=2 ** -10, atol=2 ** -14)
