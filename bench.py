@@ -11,7 +11,7 @@ followed by its decompress (decoder proxy), including the CPU entropy coder.
 `value` counts the 4 real frames per step.
 
 Also reported: p50/p99 step latency over the timed steps, the native
-8-frame rate, the per-launch layer table of one encode + decode (algorithmic
+8-frame rate, the HT-L chunk model on the same stream, the per-launch layer table of one encode + decode (algorithmic
 FLOPs / bytes, each launch timed with HIP events on its own stream), the
 roofline of the kernel class that takes the most time, and the FP32 CPU
 oracle timed on this host.
@@ -195,6 +195,42 @@ def precision_forms(sdp, p_enc_ref, x_hat_i, chunks, H, W, pad_b, pad_r, skip, d
     return out
 
 
+def htl_form(x_hat_e, x_hat_d, chunks, H, W, pad_b, pad_r, skip, qp, steps=40, warmup=3):
+    """The HT-L chunk model (the second DCVC-UF structure: ratio-1 blocks,
+    four entropy round trips per decode) on the same stream: frames/s,
+    p50 / p99 step latency, and the GEMM work and top kernel classes of one
+    encode + decode."""
+    import torch
+    from dcvc_amd import codec
+    _, sdl = codec.make_state_dicts(seed=0, video="htl")
+    e = codec.VideoCodecHTL(sdl, skip)
+    d = codec.VideoCodecHTL(share_from=e)
+    e.add_ref_feature_from_frame(x_hat_e)
+    d.add_ref_feature_from_frame(x_hat_d, apply_feature_adaptor=False)
+    n = len(chunks)
+    lat, nbytes = [], []
+    for i in range(warmup + steps):
+        t0 = time.perf_counter()
+        bits, ec = e.proxy.compress(chunks[i % n], qp, False, pad_b, pad_r)
+        d.proxy.decompress(bits, qp, H, W, ec, False)
+        torch.cuda.synchronize()
+        if i >= warmup:
+            lat.append(time.perf_counter() - t0)
+            nbytes.append(len(bits))
+    total = float(np.sum(lat))
+    out = {"frames_per_s": round(4.0 * steps / total, 3), "ms_per_step": round(1000 * total / steps, 3),
+           "p50_ms": round(1000 * float(np.percentile(lat, 50)), 3),
+           "p99_ms": round(1000 * float(np.percentile(lat, 99)), 3), "steps": steps,
+           "bpp": round(8 * float(np.mean(nbytes)) / (4 * H * W), 5), "precision": "fp16"}
+    rec_e, rec_d = layer_trace(e, d, chunks[0], qp, pad_b, pad_r, H, W)
+    out["encode_gemm_tflop"] = round(float(rec_e[rec_e[:, 0] <= 2][:, 5].sum()) / 1e12, 4)
+    out["decode_gemm_tflop"] = round(float(rec_d[rec_d[:, 0] <= 2][:, 5].sum()) / 1e12, 4)
+    classes = kernel_classes(np.concatenate([rec_e, rec_d]))
+    out["top_kernel_classes"] = {k: {f: v[f] for f in ("launches", "time_ms", "tflops", "gbs")}
+                                 for k, v in list(classes.items())[:4]}
+    return out
+
+
 def batching_form(sdi, sdp, rounds=2):
     """16 concurrent streams of uneven resolution (4x 854x480, 6x 1280x720,
     4x 1920x1080, 2x 3840x2160), one 4-frame chunk each per round at a random
@@ -335,6 +371,7 @@ def main():
     ap.add_argument("--profile-step", action="store_true")
     ap.add_argument("--no-precision-forms", action="store_true")
     ap.add_argument("--no-batching", action="store_true")
+    ap.add_argument("--no-htl", action="store_true", help="skip the HT-L chunk-model section")
     ap.add_argument("--serving-seconds", type=float, default=10.0,
                     help="length of the Poisson serving trace (0 skips it)")
     args = ap.parse_args()
@@ -446,6 +483,10 @@ def main():
         log("precision forms")
         out["precision_forms"] = precision_forms(sdp, p_enc, enc_i["x_hat"], chunks, H, W, pad_b, pad_r,
                                                  args.skip_thres, dev)
+    if not args.no_htl:
+        log("HT-L form")
+        out["htl_form"] = htl_form(enc_i["x_hat"], dec_i["x_hat"], chunks, H, W, pad_b, pad_r, args.skip_thres,
+                                   args.qp)
     if not args.no_batching:
         log("batching form")
         out["batching_form"] = batching_form(sdi, sdp)
