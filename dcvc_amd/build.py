@@ -23,7 +23,8 @@ ARCH = "gfx950"
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", f"-I{PKG.parent / 'include'}"]
-# Packed-FP32 VALU ops are disabled for all device code.
+# Packed-FP32 VALU ops (v_pk_*_f32) are disabled for all device code: hipcc forms them
+# from ordinary float code, and some of their forms have given wrong results on MI355X.
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-munsafe-fp-atomics"]
 
