@@ -47,6 +47,8 @@ SIGNATURES = {
     "dcvc_intra_quant_step": [P, I, I, I, I, I, P, I, P, I, P, I, P, I, P, P, I, P, P, P, F, I, P],
     "dcvc_intra_build_index_dec": [P, I, I, I, I, I, P, F, I, P, P, P],
     "dcvc_intra_restore_step": [P, I, I, I, I, P, I, P, I, P, I, P, P, I, I, P],
+    "dcvc_ld_quant_step": [P, I, I, I, I, I, P, I, P, I, P, I, P, I, P, P, P, F, I, P],
+    "dcvc_ld_restore_step": [P, I, I, I, I, P, I, P, I, P, I, I, P],
     "dcvc_htl_quant_step": [P, I, I, I, I, I, P, I, P, I, P, I, P, I, P, I, P, P, P, F, I, P],
     "dcvc_htl_restore_step": [P, I, I, I, I, P, I, P, I, P, I, P, I, I, P],
     "dcvc_compact_blocks": [L],
@@ -109,6 +111,19 @@ SIGNATURES = {
     "dcvc_htl_device_bytes": [P],
     "dcvc_htl_debug_copy": [P, I, P, L],
     "dcvc_htl_step_symbol_count": [P, I],
+    "dcvc_ld_create": [],
+    "dcvc_ld_destroy": [P],
+    "dcvc_ld_set_param": [P, I, P, P, P, P, F],
+    "dcvc_ld_share_weights": [P, P],
+    "dcvc_ld_prepare": [P, I, I, P, I],
+    "dcvc_ld_add_ref_feature_from_frame": [P, P, I, I, I, I, P],
+    "dcvc_ld_compress": [P, P, I, I, I, I, I, I, I, P, P, P, P],
+    "dcvc_ld_decompress": [P, P, L, I, I, I, I, I, P],
+    "dcvc_ld_set_use_graphs": [P, I],
+    "dcvc_ld_set_precision": [P, I],
+    "dcvc_ld_device_bytes": [P],
+    "dcvc_ld_debug_copy": [P, I, P, L],
+    "dcvc_ld_last_symbol_count": [P],
     "dcvc_trace_begin": [I],
     "dcvc_trace_end": [P, I],
 }
@@ -118,7 +133,9 @@ _RESTYPE = {"dcvc_last_error": C.c_char_p, "dcvc_rans_encoder_create": C.c_void_
             "dcvc_dmci_destroy": None, "dcvc_hts_destroy": None, "dcvc_hts_last_symbol_count": C.c_long,
             "dcvc_hts_device_bytes": C.c_long,
             "dcvc_dmci_step_symbol_count": C.c_long, "dcvc_htl_create": C.c_void_p, "dcvc_htl_destroy": None,
-            "dcvc_htl_device_bytes": C.c_long, "dcvc_htl_step_symbol_count": C.c_long}
+            "dcvc_htl_device_bytes": C.c_long, "dcvc_htl_step_symbol_count": C.c_long,
+            "dcvc_ld_create": C.c_void_p, "dcvc_ld_destroy": None, "dcvc_ld_device_bytes": C.c_long,
+            "dcvc_ld_last_symbol_count": C.c_long}
 
 
 def declared_symbols():

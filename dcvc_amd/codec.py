@@ -20,14 +20,14 @@ import torch
 import torch.nn.functional as F
 
 from . import entropy, stream_helper, synthetic, weights
-from .inference_extensions_cuda import DMCHTLProxy, DMCHTSProxy, DMCIProxy
+from .inference_extensions_cuda import DMCHTLProxy, DMCHTSProxy, DMCIProxy, DMCLDProxy
 
 FRAME_DELAY = 8
 
 
 def make_state_dicts(seed=0, perturb_bias=False, video="hts"):
     """(intra, video) state dicts: seeded init, fp16-rounded, plus CDF tables.
-    ``video`` is the chunk model structure, 'hts' or 'htl'."""
+    ``video`` is the inter model structure, 'hts', 'htl' or 'ld'."""
     out = []
     for i, model in enumerate(("dmci", video)):
         sd = weights.init_state_dict(model, seed + i)
@@ -90,9 +90,14 @@ class VideoCodecHTL(VideoCodecHTS):
     PROXY = DMCHTLProxy
 
 
+class VideoCodecLD(VideoCodecHTS):
+    """The low-delay DMC (video_model_ld.py:273-308): one frame per call."""
+    PROXY = DMCLDProxy
+
+
 def video_codec(structure: str):
-    """'hts' or 'htl' -> the codec class (video_model_ht.py:413-424)."""
-    return {"hts": VideoCodecHTS, "htl": VideoCodecHTL}[structure]
+    """'hts', 'htl' or 'ld' -> the codec class (video_model_ht.py:413-424, video_model_ld.py:273-289)."""
+    return {"hts": VideoCodecHTS, "htl": VideoCodecHTL, "ld": VideoCodecLD}[structure]
 
 
 def calc_psnr(a, b, data_range=255.0):

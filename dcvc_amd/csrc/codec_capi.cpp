@@ -4,6 +4,7 @@
 #include "proxy_dmci.h"
 #include "proxy_hts.h"
 #include "proxy_htl.h"
+#include "proxy_ld.h"
 
 using namespace dcvc;
 
@@ -238,6 +239,75 @@ long dcvc_htl_step_symbol_count(void* h, int step) {
     if (!h || step < 0 || step > 3) return -1;
     return static_cast<HTLProxy*>(h)->step_count(step);
 }
+
+void* dcvc_ld_create(void) { return create<LDProxy>(); }
+void dcvc_ld_destroy(void* h) { delete static_cast<LDProxy*>(h); }
+
+int dcvc_ld_set_param(void* h, int n, const char** names, const float** data, const int64_t* shapes,
+                      const int* ndims, float skip_thres) {
+    return guarded([&] { checked<LDProxy>(h)->set_param(make_dict(n, names, data, shapes, ndims), skip_thres); });
+}
+
+int dcvc_ld_share_weights(void* h, void* src) {
+    return guarded([&] {
+        auto w = checked<LDProxy>(src)->weights();
+        if (!w) throw std::logic_error("source proxy has no parameters");
+        checked<LDProxy>(h)->share_weights(w);
+    });
+}
+
+int dcvc_ld_prepare(void* h, int Hp, int Wp, void* x_hat, int ld_out) {
+    return guarded([&] {
+        if (!x_hat || ld_out < 3) throw std::invalid_argument("bad output buffer");
+        checked<LDProxy>(h)->prepare(Hp, Wp, static_cast<f16*>(x_hat), ld_out);
+    });
+}
+
+int dcvc_ld_add_ref_feature_from_frame(void* h, const void* frame, int ld, int Hp, int Wp, int apply_adaptor,
+                                       void* stream) {
+    return guarded([&] {
+        if (!frame || ld < 3) throw std::invalid_argument("bad frame");
+        checked<LDProxy>(h)->add_ref_feature_from_frame(static_cast<const f16*>(frame), ld, Hp, Wp, apply_adaptor != 0,
+                                                        S(stream));
+    });
+}
+
+int dcvc_ld_compress(void* h, const void* x, int ldx, int H, int W, int qp, int reset_feature_memory, int pad_b,
+                     int pad_r, void* stream, const uint8_t** bits, long* nbytes, int* ec_parallel) {
+    return guarded([&] {
+        if (!x || !bits || !nbytes || !ec_parallel) throw std::invalid_argument("null argument");
+        *ec_parallel = checked<LDProxy>(h)->compress(static_cast<const f16*>(x), ldx, H, W, qp, reset_feature_memory != 0,
+                                                     pad_b, pad_r, S(stream), bits, nbytes);
+    });
+}
+
+int dcvc_ld_decompress(void* h, const uint8_t* bits, long nbytes, int qp, int height, int width, int ec_parallel,
+                       int reset_feature_memory, void* stream) {
+    return guarded([&] {
+        if (!bits && nbytes > 0) throw std::invalid_argument("null bitstream");
+        checked<LDProxy>(h)->decompress(bits, nbytes, qp, height, width, ec_parallel, reset_feature_memory != 0,
+                                        S(stream));
+    });
+}
+
+int dcvc_ld_set_use_graphs(void* h, int on) {
+    return guarded([&] { checked<LDProxy>(h)->set_use_graphs(on != 0); });
+}
+
+int dcvc_ld_set_precision(void* h, int prec) {
+    return guarded([&] { checked<LDProxy>(h)->set_precision(prec); });
+}
+
+long dcvc_ld_device_bytes(void* h) { return h ? (long)static_cast<LDProxy*>(h)->device_bytes() : -1; }
+
+int dcvc_ld_debug_copy(void* h, int what, void* host, long bytes) {
+    return guarded([&] {
+        if (!host || bytes < 0) throw std::invalid_argument("bad host buffer");
+        checked<LDProxy>(h)->copy_debug(what, host, bytes);
+    });
+}
+
+long dcvc_ld_last_symbol_count(void* h) { return h ? static_cast<LDProxy*>(h)->last_symbol_count() : -1; }
 
 int dcvc_trace_begin(int timed) {
     return guarded([&] { trace_set(true, timed != 0); });

@@ -40,6 +40,14 @@ hipError_t hts_dec_index(const Act& scales, const float* th, float thr, uint8_t*
 hipError_t hts_dec_step(const Act& geom, const int8_t* yq, const f16* means, int ldm,
                         const f16* qdec, int ldq, f16* yhat, int ldh, int step, hipStream_t s);
 
+// LD (one entropy round trip, 2-step mean-only spatial prior over channel
+// halves, get_mask_2x); indexes from hts_dec_index.
+hipError_t ld_enc_step(const Act& y, const f16* scales, int lds, const f16* means, int ldm, const f16* qdec, int ldq,
+                       f16* yhat, int ldh, int16_t* sym, uint8_t* cond, const float* th, float thr, int step,
+                       hipStream_t s);
+hipError_t ld_dec_step(const Act& geom, const int8_t* yq, const f16* means, int ldm, const f16* qdec, int ldq,
+                       f16* yhat, int ldh, int step, hipStream_t s);
+
 // DMCI (four entropy round trips over quarter tensors).
 hipError_t intra_enc_step(const Act& y, const f16* scales, int lds, const f16* means, int ldm,
                           f16* ysf, int ldsf, const f16* q, int qstride, const int* qp, f16* yout,

@@ -130,8 +130,9 @@ hipError_t pixel_shuffle8_clamp(const f16* x, int ldx, int n_img, int H8, int W8
 }
 
 // multiply_with_broadcast_cuda (stream.cu:484-546): y = x * q[qp][c]
-__global__ void mul_channel_kernel(const f16* __restrict__ x, int ldx, long M, int C, const f16* __restrict__ q,
-                                   int qstride, const int* __restrict__ qp, f16* __restrict__ y, int ldy) {
+// x and y may be the same buffer (in place): each lane reads its 8 elements before writing them
+__global__ void mul_channel_kernel(const f16* x, int ldx, long M, int C, const f16* __restrict__ q, int qstride,
+                                   const int* __restrict__ qp, f16* y, int ldy) {
     const int CG = C >> 3;
     const long item = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (item >= M * CG) return;
@@ -215,10 +216,27 @@ hipError_t replicate_pad(const f16* x, int ldx, int n_img, int H, int W, int C, 
 //   y_hat  = y_q + mean (active), unchanged (inactive), 0 for inactive at k=0;
 //   k=3 multiplies every y_hat by max(q_dec, 0.5)          (stream.cu:548-630)
 //   sym    = (y_q << 8) + index(scale), cond = scale > thr   (stream.cu:130-161)
-__global__ void hts_enc_step_kernel(Act y, const f16* __restrict__ scales, int lds, const f16* __restrict__ means,
-                                    int ldm, const f16* __restrict__ qdec, int ldq, f16* __restrict__ yhat, int ldh,
-                                    int16_t* __restrict__ sym, uint8_t* __restrict__ cond,
-                                    const float* __restrict__ th, float thr, int step) {
+// Active test of checkerboard step `step` for element (pixel h, w; channel c
+// of C): kSteps = 4 is the 4x mask (channel quarters, get_mask_4x), kSteps = 2
+// the 2x mask (channel halves, get_mask_2x: half g active where
+// (h + w + g + step) is even).
+template <int kSteps>
+DCVC_DEV bool step_active(int step, int c, int C, int h, int w) {
+    if constexpr (kSteps == 4) {
+        return mask4_pos(step, c / (C >> 2)) == (((h & 1) << 1) | (w & 1));
+    } else {
+        return ((h + w + c / (C >> 1) + step) & 1) == 0;
+    }
+}
+
+// One step of the full-tensor boundary (HT-S: 4 steps, LD: 2): y / max(q, .5)
+// quantised where active; y_hat accumulates; the last step scales by
+// max(q, .5).  Symbols and conditions are written at the active elements.
+template <int kSteps>
+__global__ void full_enc_step_kernel(Act y, const f16* __restrict__ scales, int lds, const f16* __restrict__ means,
+                                     int ldm, const f16* __restrict__ qdec, int ldq, f16* __restrict__ yhat, int ldh,
+                                     int16_t* __restrict__ sym, uint8_t* __restrict__ cond,
+                                     const float* __restrict__ th, float thr, int step) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int C = y.C;
     if (i >= y.pixels() * C) return;
@@ -226,8 +244,8 @@ __global__ void hts_enc_step_kernel(Act y, const f16* __restrict__ scales, int l
     const int c = (int)(i % C);
     const int w = (int)(m % y.W);
     const int h = (int)((m / y.W) % y.H);
-    const int pos = ((h & 1) << 1) | (w & 1);
-    const bool active = mask4_pos(step, c / (C >> 2)) == pos;
+    const bool active = step_active<kSteps>(step, c, C, h, w);
+    const bool last = step == kSteps - 1;
     const float q = qstep((float)qdec[m * ldq + c]);
     float out;
     if (active) {
@@ -241,15 +259,23 @@ __global__ void hts_enc_step_kernel(Act y, const f16* __restrict__ scales, int l
     } else {
         out = step == 0 ? 0.f : (float)yhat[m * ldh + c];
     }
-    if (step == 3) out = out * q;
-    if (active || step == 0 || step == 3) yhat[m * ldh + c] = (f16)out;
+    if (last) out = out * q;
+    if (active || step == 0 || last) yhat[m * ldh + c] = (f16)out;
 }
 
 hipError_t hts_enc_step(const Act& y, const f16* scales, int lds, const f16* means, int ldm, const f16* qdec,
                         int ldq, f16* yhat, int ldh, int16_t* sym, uint8_t* cond, const float* th, float thr,
                         int step, hipStream_t s) {
-    hipLaunchKernelGGL(hts_enc_step_kernel, grid1d(y.pixels() * y.C), dim3(256), 0, s, y, scales, lds, means, ldm,
-                       qdec, ldq, yhat, ldh, sym, cond, th, thr, step);
+    hipLaunchKernelGGL(full_enc_step_kernel<4>, grid1d(y.pixels() * y.C), dim3(256), 0, s, y, scales, lds, means,
+                       ldm, qdec, ldq, yhat, ldh, sym, cond, th, thr, step);
+    return hipGetLastError();
+}
+
+hipError_t ld_enc_step(const Act& y, const f16* scales, int lds, const f16* means, int ldm, const f16* qdec, int ldq,
+                       f16* yhat, int ldh, int16_t* sym, uint8_t* cond, const float* th, float thr, int step,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(full_enc_step_kernel<2>, grid1d(y.pixels() * y.C), dim3(256), 0, s, y, scales, lds, means,
+                       ldm, qdec, ldq, yhat, ldh, sym, cond, th, thr, step);
     return hipGetLastError();
 }
 
@@ -270,8 +296,10 @@ hipError_t hts_dec_index(const Act& scales, const float* th, float thr, uint8_t*
 }
 
 // restore_y* (stream.cu:685-754) in the encoder's rounding order
-__global__ void hts_dec_step_kernel(Act g, const int8_t* __restrict__ yq, const f16* __restrict__ means, int ldm,
-                                    const f16* __restrict__ qdec, int ldq, f16* __restrict__ yhat, int ldh, int step) {
+template <int kSteps>
+__global__ void full_dec_step_kernel(Act g, const int8_t* __restrict__ yq, const f16* __restrict__ means, int ldm,
+                                     const f16* __restrict__ qdec, int ldq, f16* __restrict__ yhat, int ldh,
+                                     int step) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int C = g.C;
     if (i >= g.pixels() * C) return;
@@ -279,22 +307,29 @@ __global__ void hts_dec_step_kernel(Act g, const int8_t* __restrict__ yq, const 
     const int c = (int)(i % C);
     const int w = (int)(m % g.W);
     const int h = (int)((m / g.W) % g.H);
-    const int pos = ((h & 1) << 1) | (w & 1);
-    const bool active = mask4_pos(step, c / (C >> 2)) == pos;
+    const bool active = step_active<kSteps>(step, c, C, h, w);
+    const bool last = step == kSteps - 1;
     float out;
     if (active) {
         out = (float)yhat_elem((int)yq[i], (float)means[m * ldm + c]);
     } else {
         out = step == 0 ? 0.f : (float)yhat[m * ldh + c];
     }
-    if (step == 3) out = out * qstep((float)qdec[m * ldq + c]);
-    if (active || step == 0 || step == 3) yhat[m * ldh + c] = (f16)out;
+    if (last) out = out * qstep((float)qdec[m * ldq + c]);
+    if (active || step == 0 || last) yhat[m * ldh + c] = (f16)out;
 }
 
 hipError_t hts_dec_step(const Act& g, const int8_t* yq, const f16* means, int ldm, const f16* qdec, int ldq,
                         f16* yhat, int ldh, int step, hipStream_t s) {
-    hipLaunchKernelGGL(hts_dec_step_kernel, grid1d(g.pixels() * g.C), dim3(256), 0, s, g, yq, means, ldm, qdec, ldq,
-                       yhat, ldh, step);
+    hipLaunchKernelGGL(full_dec_step_kernel<4>, grid1d(g.pixels() * g.C), dim3(256), 0, s, g, yq, means, ldm, qdec,
+                       ldq, yhat, ldh, step);
+    return hipGetLastError();
+}
+
+hipError_t ld_dec_step(const Act& g, const int8_t* yq, const f16* means, int ldm, const f16* qdec, int ldq, f16* yhat,
+                       int ldh, int step, hipStream_t s) {
+    hipLaunchKernelGGL(full_dec_step_kernel<2>, grid1d(g.pixels() * g.C), dim3(256), 0, s, g, yq, means, ldm, qdec,
+                       ldq, yhat, ldh, step);
     return hipGetLastError();
 }
 

@@ -331,6 +331,21 @@ int dcvc_intra_restore_step(const int8_t* yq, int n_img, int H_, int W_, int C, 
                              Hm(Yout), ldo, step, S(stream)));
 }
 
+int dcvc_ld_quant_step(const void* Y, int ldy, int n_img, int H_, int W_, int C, const void* Sc, int lds,
+                       const void* Mn, int ldm, const void* Qd, int ldq, void* Yhat, int ldh, int16_t* sym,
+                       uint8_t* cond, const float* th, float thr, int step, void* stream) {
+    if (!Y || !Sc || !Mn || !Qd || !Yhat || !sym || !cond || !th || step < 0 || step > 1 || C % 2) return DCVC_ERR_ARG;
+    return st(ld_enc_step(act(Y, ldy, n_img, H_, W_, C), H(Sc), lds, H(Mn), ldm, H(Qd), ldq, Hm(Yhat), ldh, sym, cond,
+                          th, thr, step, S(stream)));
+}
+
+int dcvc_ld_restore_step(const int8_t* yq, int n_img, int H_, int W_, int C, const void* Mn, int ldm, const void* Qd,
+                         int ldq, void* Yhat, int ldh, int step, void* stream) {
+    if (!yq || !Mn || !Qd || !Yhat || step < 0 || step > 1 || C % 2) return DCVC_ERR_ARG;
+    return st(ld_dec_step(act(nullptr, 0, n_img, H_, W_, C), yq, H(Mn), ldm, H(Qd), ldq, Hm(Yhat), ldh, step,
+                          S(stream)));
+}
+
 int dcvc_htl_quant_step(const void* Y, int ldy, int n_img, int H_, int W_, int C, const void* Sc, int lds,
                         const void* Mn, int ldm, const void* Qd, int ldq, void* Ysf, int ldsf, void* Yout, int ldo,
                         int16_t* sym, uint8_t* cond, const float* th, float thr, int step, void* stream) {

@@ -259,8 +259,37 @@ class DMCHTLProxy(_ChunkProxy):
         return int(self._lib.dcvc_htl_step_symbol_count(self._h, step))
 
 
-class DMCLDProxy:
-    """DMCLDProxy (dmc_ld_proxy.h): not built on MI355X yet."""
+class DMCLDProxy(_ChunkProxy):
+    """DMCLDProxy (dmc_ld_proxy.h; add_ref_feature_from_frame
+    dmc_ld_proxy.cpp:407-418, compress :420-485, decompress :487-593): the
+    low-delay model, one frame per call; decompress returns one tensor."""
+    FRAMES = 1
+    _PREFIX = "ld"
+    _NAME = "LD"
 
-    def __init__(self):
-        raise NotImplementedError("the LD model structure is not implemented by the MI355X build; use 'hts'")
+    def _prepare(self, Hp, Wp):
+        if self._size != (Hp, Wp):
+            self._frames = [torch.empty((1, 3, Hp, Wp), dtype=torch.float16, device=self._device,
+                                        memory_format=torch.channels_last)]
+            self._call("prepare", Hp, Wp, self._frames[0].data_ptr(), 3)
+            self._size = (Hp, Wp)
+
+    def compress(self, x, qp, reset_feature_memory, padding_b, padding_r):
+        """-> (np.uint8 bitstream, ec_parallel); x: [1,3,H,W], one frame"""
+        x = _nhwc(x)
+        _, c, H, W = x.shape
+        if c != 3:
+            raise ValueError("LD input must be one frame [1,3,H,W]")
+        bits = C.c_void_p()
+        n = C.c_long()
+        ec = C.c_int()
+        self._call("compress", x.data_ptr(), 3, H, W, int(qp), int(bool(reset_feature_memory)), int(padding_b),
+                   int(padding_r), _stream(), C.byref(bits), C.byref(n), C.byref(ec))
+        return _bits_to_numpy(bits, n), ec.value
+
+    def decompress(self, bit_stream, qp, height, width, entropy_coder_parallel, reset_feature_memory):
+        """-> x_hat [1,3,Hp,Wp]"""
+        return super().decompress(bit_stream, qp, height, width, entropy_coder_parallel, reset_feature_memory)[0]
+
+    def last_symbol_count(self):
+        return int(self._lib.dcvc_ld_last_symbol_count(self._h))

@@ -168,6 +168,18 @@ def subpel_conv3x3(x, n_img, H, W, wt_sub, cout, bias, out=None):
     return out
 
 
+def ld_quant_step(y, n_img, H, W, scales, means, qdec, yhat, sym, cond, thr, step):
+    C = y.shape[1]
+    _lib_call("dcvc_ld_quant_step", _p(y), _ld(y), n_img, H, W, C, _p(scales), _ld(scales), _p(means),
+              _ld(means), _p(qdec), _ld(qdec), _p(yhat), _ld(yhat), _p(sym), _p(cond),
+              _p(thresholds(y.device)), float(thr), step, _stream())
+
+
+def ld_restore_step(yq, n_img, H, W, C, means, qdec, yhat, step):
+    _lib_call("dcvc_ld_restore_step", _p(yq), n_img, H, W, C, _p(means), _ld(means), _p(qdec), _ld(qdec),
+              _p(yhat), _ld(yhat), step, _stream())
+
+
 def htl_quant_step(y, n_img, H, W, scales, means, qdec, ysf, yout, sym, cond, thr, step):
     C = y.shape[1]
     _lib_call("dcvc_htl_quant_step", _p(y), _ld(y), n_img, H, W, C, _p(scales), _ld(scales), _p(means),

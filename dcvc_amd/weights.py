@@ -7,7 +7,8 @@ interface, which ``set_param`` consumes:
 * ``ResidualBlockUpsample``    src/layers/layers.py:162-173 (``SubpelConv2x`` :92-103)
 * ``ResidualBlockWithStride2`` src/layers/layers.py:176-188
 * ``DMCI``                     src/models/image_model.py:21-148
-* ``DMC`` (HT-S structure)     src/models/video_model_ht.py:26-351
+* ``DMC`` (HT-S / HT-L)        src/models/video_model_ht.py:26-351
+* ``DMC`` (low delay)          src/models/video_model_ld.py:24-216
 * ``BitEstimator``             src/models/entropy_models.py:78-89
 
 Initialisation follows the reference's scheme (src/models/common_model.py:49-54):
@@ -54,17 +55,17 @@ def dcb_schema(prefix, in_ch, out_ch, dcb2=False, force_adaptor=False):
     return s
 
 
-def rbu_schema(prefix, in_ch, out_ch, force_bias=False):
+def rbu_schema(prefix, in_ch, out_ch, force_bias=False, dcb2=False):
     """ResidualBlockUpsample: SubpelConv2x(k=1) + DepthConvBlock (layers.py:92-173)."""
     s = _conv(prefix + "up.conv.0.", out_ch * 4, in_ch, bias=force_bias)
-    s += dcb_schema(prefix + "conv.", out_ch, out_ch)
+    s += dcb_schema(prefix + "conv.", out_ch, out_ch, dcb2=dcb2)
     return s
 
 
-def rbs2_schema(prefix, in_ch, out_ch):
+def rbs2_schema(prefix, in_ch, out_ch, dcb2=False):
     """ResidualBlockWithStride2: 1x1 over pixel_unshuffle(x, 2) + DCB (layers.py:176-188)."""
     s = _conv(prefix + "down.", out_ch, in_ch * 4)
-    s += dcb_schema(prefix + "conv.", out_ch, out_ch)
+    s += dcb_schema(prefix + "conv.", out_ch, out_ch, dcb2=dcb2)
     return s
 
 
@@ -210,7 +211,51 @@ def dmc_htl_schema():
     return s
 
 
-SCHEMAS = {"dmci": dmci_schema, "hts": dmc_hts_schema, "htl": dmc_htl_schema}
+def dmc_ld_schema():
+    """DMC low-delay (video_model_ld.py:24-216): one frame per call, dcb2
+    blocks throughout, y = 128, d = m = 256 channels, a 2x checkerboard."""
+    src, y, z, d, m = 192, 128, 128, 256, 256
+    s = []
+    s += bit_estimator_schema("bit_estimator_z.", z)
+    s += dcb_schema("feature_adaptor_i.conv.0.", src, m, dcb2=True)
+    for i in range(1, 4):
+        s += dcb_schema(f"feature_adaptor_i.conv.{i}.", m, m, dcb2=True)
+    s += dcb_schema("feature_adaptor_m.conv.0.", m + d, m, dcb2=True)
+    for i in range(1, 4):
+        s += dcb_schema(f"feature_adaptor_m.conv.{i}.", m, m, dcb2=True)
+    for i in range(5):
+        s += dcb_schema(f"feature_extractor.conv.{i}.", m, m, dcb2=True)
+    s += dcb_schema("encoder.conv1.0.", src + m, d, dcb2=True)
+    s += dcb_schema("encoder.conv1.1.", d, d, dcb2=True)
+    s += dcb_schema("encoder.conv2.", d, d, dcb2=True)
+    s += _conv("encoder.down.", y, d, k=3)
+    s += dcb_schema("hyper_encoder.conv.0.", y, z, dcb2=True)
+    s += rbs2_schema("hyper_encoder.conv.1.", z, z, dcb2=True)
+    s += rbs2_schema("hyper_encoder.conv.2.", z, z, dcb2=True)
+    s += rbu_schema("hyper_decoder.conv.0.", z, z, dcb2=True)
+    s += rbu_schema("hyper_decoder.conv.1.", z, z, dcb2=True)
+    s += dcb_schema("hyper_decoder.conv.2.", z, y, dcb2=True)
+    s += rbs2_schema("temporal_prior_encoder.conv.", m, y * 2, dcb2=True)
+    for i in range(3):
+        s += dcb_schema(f"y_prior_fusion.conv.{i}.", y * 3, y * 3, dcb2=True)
+    s += _conv("y_prior_fusion.conv.3.", y * 3, y * 3)
+    s += dcb_schema("y_spatial_prior.conv.0.", y * 4, y * 2, dcb2=True)
+    s += dcb_schema("y_spatial_prior.conv.1.", y * 2, y * 2, dcb2=True)
+    s += _conv("y_spatial_prior.conv.2.", y, y * 2)
+    s += _conv("decoder.up.conv.0.", d * 4, y, bias=False)
+    s += dcb_schema("decoder.conv1.0.", d + m, d, dcb2=True)
+    s += dcb_schema("decoder.conv1.1.", d, d, dcb2=True)
+    s += dcb_schema("decoder.conv1.2.", d, d, dcb2=True)
+    s += _conv("decoder.conv2.", d, d)
+    for i in range(3):
+        s += dcb_schema(f"recon_head.conv.{i}.", d, d, dcb2=True)
+    s += _conv("recon_head.head.", src, d)
+    s += [("q_encoder", (QP_NUM, d), "ones"), ("q_decoder", (QP_NUM, d), "ones"),
+          ("q_feature", (QP_NUM, y * 2), "ones")]
+    return s
+
+
+SCHEMAS = {"dmci": dmci_schema, "hts": dmc_hts_schema, "htl": dmc_htl_schema, "ld": dmc_ld_schema}
 
 
 def _xavier_std(shape, gain=1.0):
@@ -230,6 +275,8 @@ def _is_residual_branch_out(key):
 
 
 def _is_recon_projection(key):
+    if key == "recon_head.head.weight":
+        return True
     if key.startswith("dec.dec_2.") and key.endswith(".weight"):
         return any(s in key for s in ("adaptor.", "dc.3.", "ffn.2."))
     if key.startswith("recon_head.conv2.") and key.endswith(".3.weight"):
@@ -238,7 +285,7 @@ def _is_recon_projection(key):
 
 
 def init_state_dict(model: str, seed: int = 0, q_jitter: float = 0.0, stable: bool = True):
-    """Return an OrderedDict of float32 numpy arrays for ``model`` in {'dmci','hts','htl'}.
+    """Return an OrderedDict of float32 numpy arrays for ``model`` in {'dmci','hts','htl','ld'}.
 
     ``stable`` (default) scales the output conv of every residual branch
     (``dc.3``, ``ffn.2``) by 0.25 and the reconstruction projections by 0.1.

@@ -159,6 +159,36 @@ long dcvc_htl_device_bytes(void* h);
 int dcvc_htl_debug_copy(void* h, int what, void* host, long bytes);
 /* Symbols (encoder) or indexes (decoder) of checkerboard step k in the last call. */
 long dcvc_htl_step_symbol_count(void* h, int step);
+/* DMCLDProxy (dmc_ld_proxy.h; bind.cpp exports it as DMCLDProxy): the
+ * low-delay model, one frame per call, one entropy round trip. */
+void* dcvc_ld_create(void);
+void dcvc_ld_destroy(void* h);
+/* set_param (dmc_ld_proxy.cpp:595-637) */
+int dcvc_ld_set_param(void* h, int n, const char** names, const float** data, const int64_t* shapes,
+                      const int* ndims, float skip_thres);
+/* One weight copy for many stream contexts (no reference counterpart). */
+int dcvc_ld_share_weights(void* h, void* src);
+/* pre_allocate_tensors (dmc_ld_proxy.cpp:693-794) with a caller-owned
+ * [Hp, Wp, 3] reconstruction output */
+int dcvc_ld_prepare(void* h, int Hp, int Wp, void* x_hat, int ld_out);
+/* add_ref_feature_from_frame (dmc_ld_proxy.cpp:407-418) */
+int dcvc_ld_add_ref_feature_from_frame(void* h, const void* frame, int ld, int Hp, int Wp,
+                                       int apply_adaptor, void* stream);
+/* compress (dmc_ld_proxy.cpp:420-485): x is [H, W, 3], one frame */
+int dcvc_ld_compress(void* h, const void* x, int ldx, int H, int W, int qp, int reset_feature_memory,
+                     int pad_b, int pad_r, void* stream, const uint8_t** bits, long* nbytes,
+                     int* ec_parallel);
+/* decompress (dmc_ld_proxy.cpp:487-593): writes the prepared output */
+int dcvc_ld_decompress(void* h, const uint8_t* bits, long nbytes, int qp, int height, int width,
+                       int ec_parallel, int reset_feature_memory, void* stream);
+int dcvc_ld_set_use_graphs(void* h, int on);
+/* Precision form, as dcvc_hts_set_precision (no reference counterpart). */
+int dcvc_ld_set_precision(void* h, int prec);
+long dcvc_ld_device_bytes(void* h);
+/* Test hook: 0 y_hat|q_dec|scales|means, 1 y, 3 memory|feature, 4 up|x|ctx,
+ * 5 dense symbols, 6 z, 7 hyper|temporal params. */
+int dcvc_ld_debug_copy(void* h, int what, void* host, long bytes);
+long dcvc_ld_last_symbol_count(void* h);
 /* Layer-table trace (no reference counterpart): between begin and end, every
  * GEMM / conv / depthwise launch issued from the host is recorded; end
  * returns the record count (or -status) and fills up to max_records rows of

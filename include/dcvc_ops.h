@@ -178,6 +178,20 @@ int dcvc_intra_restore_step(const int8_t* yq, int n_img, int H, int W, int C, co
                             int ldm, void* Ysf, int ldsf, const void* q, int qstride,
                             const int* qp, void* Yout, int ldo, int step, void* stream);
 
+/* LD encoder half `step` of the 2x checkerboard (process_with_mask_no_scale
+ * and process_with_mask_no_scale_add_and_multiply_inplace after
+ * divide_with_clamp_min, dmc_ld_proxy.cpp:444-452; stream.cu:548-645):
+ * as dcvc_hts_quant_step over channel halves, 2 steps, step 1 scales y_hat
+ * by max(Qd, .5).  Indexes on the decoder side: dcvc_hts_build_index_dec. */
+int dcvc_ld_quant_step(const void* Y, int ldy, int n_img, int H, int W, int C, const void* S,
+                       int lds, const void* Mn, int ldm, const void* Qd, int ldq, void* Yhat,
+                       int ldh, int16_t* sym, uint8_t* cond, const float* th, float thr, int step,
+                       void* stream);
+/* restore_y_cuda / restore_y_and_add_multiply_inplace_cuda (dmc_ld_proxy.cpp:578-580) */
+int dcvc_ld_restore_step(const int8_t* yq, int n_img, int H, int W, int C, const void* Mn,
+                         int ldm, const void* Qd, int ldq, void* Yhat, int ldh, int step,
+                         void* stream);
+
 /* HT-L encoder mask step (dmc_htl_proxy.cpp:619-688: divide_with_clamp_min
  * on y, process_with_mask_cuda, single_part_for_writing_4x, build_index_enc;
  * add_and_multiply_with_clamp_min after step 3): quarter outputs as

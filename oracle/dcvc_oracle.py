@@ -44,6 +44,12 @@ mistake; none of them retypes the oracle's formula):
 * hts/intra/htl *_f32 step restatements: agree with the torch masked steps
   on symbols, conditions and y_hat (test_oracle_pins.py, HT-S and HT-L
   forms; the HT-L form covers the DMCI quarter steps it is built on).
+* ld_* submodules and mask_2x: the reference low-delay modules and
+  get_mask_2x on the same seeded weights (golden vectors,
+  test_oracle_golden.py).  ld_compress / ld_decompress: the decoder rebuilds
+  the encoder's indexes, y_hat, reconstruction and context exactly
+  (test_oracle_pins.py).  ld_*_f32 step restatements: against the torch
+  masked halves (test_oracle_pins.py).
 * htl_* submodules: the reference HT-L modules on the same seeded weights
   (golden vectors, test_oracle_golden.py).  htl_compress / htl_decompress:
   composed of the pinned parts; the decoder rebuilds the encoder's indexes
@@ -845,6 +851,211 @@ def htl_decompress(sd, st: HTSState, z_int8, decode_y, qp, height, width, reset_
 
 
 # ---------------------------------------------------------------------------
+# DMC low delay — video_model_ld.py with dmc_ld_proxy.cpp:380-862 semantics
+# ---------------------------------------------------------------------------
+
+def mask_2x(C, H, W):
+    """get_mask_2x (common_model.py:157-172, dmc_ld_proxy.cpp:678-691)."""
+    m0 = _one_mask(((1, 0), (0, 1)), H, W)
+    m1 = _one_mask(((0, 1), (1, 0)), H, W)
+    q = C // 2
+
+    def cat(a, b):
+        return torch.cat([a.expand(1, q, H, W), b.expand(1, q, H, W)], dim=1)
+    return [cat(m0, m1), cat(m1, m0)]
+
+
+@dataclass
+class LDState:
+    """Per-stream context of DMCLDProxy (dmc_ld_proxy.h): as HT, plus the
+    temporal params, which the proxy computes once the memory is known."""
+    feature_i: torch.Tensor = None     # [1,192,H8,W8]
+    feature_p: torch.Tensor = None     # [1,256,H8,W8]
+    memory: torch.Tensor = None
+    ctx: torch.Tensor = None
+    temporal: torch.Tensor = None      # [1,256,H16,W16], before the q_feature multiply
+    memory_has_value: bool = False
+
+
+@dataclass
+class LDResult:
+    sym16: np.ndarray = None
+    sym16_dense: np.ndarray = None
+    cond_dense: np.ndarray = None
+    z_int8: np.ndarray = None
+    y: torch.Tensor = None             # y * (1 / q_dec)
+    y_hat: torch.Tensor = None
+    common_params: torch.Tensor = None
+    x_hat: torch.Tensor = None
+
+
+def ld_feature_adaptor_i(sd, f):
+    """FeatureAdaptorI, LD: 4 dcb2 blocks (video_model_ld.py:62-75)."""
+    return seq(sd, "feature_adaptor_i.conv.", 4, f)
+
+
+def ld_feature_adaptor_m(sd, memory, feature):
+    """FeatureAdaptorM, LD: 4 blocks over cat(memory, feature) (:78-89)."""
+    return seq(sd, "feature_adaptor_m.conv.", 4, torch.cat([memory, feature], 1))
+
+
+def ld_feature_extractor(sd, memory):
+    """FeatureExtractor, LD: 5 blocks (:92-106)."""
+    return seq(sd, "feature_extractor.conv.", 5, memory)
+
+
+def ld_recon_head(sd, feature):
+    """ReconHead, LD (:151-166): 3 blocks, 1x1 to 192, pixel_shuffle(8) with
+    the proxy's clamp; also returns the head output (the reset feature)."""
+    out = seq(sd, "recon_head.conv.", 3, feature)
+    head = conv(sd, "recon_head.head.", out)
+    return torch.clamp(F.pixel_shuffle(head, 8), -0.5, 0.5), head
+
+
+def ld_add_ref_feature_from_frame(sd, st: LDState, frame, apply_adaptor=True):
+    """DMCLDProxy::add_ref_feature_from_frame (dmc_ld_proxy.cpp:407-418)."""
+    st.feature_i = F.pixel_unshuffle(frame, 8)
+    if apply_adaptor:
+        st.memory = ld_feature_adaptor_i(sd, st.feature_i)
+        st.ctx = ld_feature_extractor(sd, st.memory)
+        st.temporal = ld_temporal_prior(sd, st.memory)
+    st.memory_has_value = apply_adaptor
+    return st
+
+
+def ld_encoder(sd, x_padded, ctx, qp):
+    """Encoder, LD (video_model_ld.py:43-59): 2 blocks, the third with the
+    q_encoder multiply fused in its tail (dmc_ld_proxy.cpp:63-71), 3x3
+    stride-2 down conv."""
+    feat = torch.cat([F.pixel_unshuffle(x_padded, 8), ctx], 1)
+    feat = seq(sd, "encoder.conv1.", 2, feat)
+    feat = dcb(sd, "encoder.conv2.", feat, quant=qsel(sd, "q_encoder", qp))
+    return conv(sd, "encoder.down.", feat, stride=2, padding=1)
+
+
+def ld_hyper_encoder(sd, y_pad):
+    """HyperEncoder, LD (:124-134)."""
+    z = dcb(sd, "hyper_encoder.conv.0.", y_pad)
+    z = rbs2(sd, "hyper_encoder.conv.1.", z, False)
+    return rbs2(sd, "hyper_encoder.conv.2.", z, False)
+
+
+def ld_hyper_decoder(sd, z_hat):
+    """HyperDecoder, LD (:109-121)."""
+    hyper = rbu(sd, "hyper_decoder.conv.0.", z_hat, False)
+    hyper = rbu(sd, "hyper_decoder.conv.1.", hyper, False)
+    return dcb(sd, "hyper_decoder.conv.2.", hyper)
+
+
+def ld_temporal_prior(sd, memory):
+    """TemporalPriorEncoder, LD (:182-188): no quant on its input."""
+    return rbs2(sd, "temporal_prior_encoder.conv.", memory, False)
+
+
+def _ld_prior(sd, z_hat, temporal, qp):
+    """PriorFusion over cat(hyper, temporal * q_feature) (:137-148)."""
+    hyper = ld_hyper_decoder(sd, z_hat)
+    hyper = hyper[:, :, :temporal.shape[2], :temporal.shape[3]]
+    x = torch.cat([hyper, temporal * qsel(sd, "q_feature", qp).reshape(1, -1, 1, 1)], 1)
+    for i in range(3):
+        x = dcb(sd, f"y_prior_fusion.conv.{i}.", x)
+    return conv(sd, "y_prior_fusion.conv.3.", x)
+
+
+def ld_spatial_prior(sd, y_hat0, params):
+    """SpatialPrior, LD (:169-179): means for the second half from
+    cat(y_hat of step 0, common params)."""
+    x = torch.cat([y_hat0, params], 1)
+    x = dcb(sd, "y_spatial_prior.conv.0.", x)
+    x = dcb(sd, "y_spatial_prior.conv.1.", x)
+    return conv(sd, "y_spatial_prior.conv.2.", x)
+
+
+def _ld_decoder(sd, y_hat, ctx, qp):
+    """Decoder, LD (:24-40): 1x1 sub-pixel up conv, 3 blocks, 1x1 conv with
+    the q_decoder multiply fused (conv1x1_bias_with_quant, dmc_ld_proxy.cpp:27-35)."""
+    up = subpel(sd, "decoder.up.", y_hat)
+    feat = seq(sd, "decoder.conv1.", 3, torch.cat([up, ctx], 1))
+    return conv(sd, "decoder.conv2.", feat) * qsel(sd, "q_decoder", qp).reshape(1, -1, 1, 1)
+
+
+def ld_code_y(sd, y, params, skip_thres=0.0):
+    """The two checkerboard halves of DMCLDProxy::compress
+    (dmc_ld_proxy.cpp:432-455, forward_prior_2x common_model.py:212-232):
+    y / max(q_dec, .5); half 0 on the common scales / means; half 1 on the
+    same scales and the spatial prior's means; one symbol stream over the
+    whole of y; y_hat = (y_hat_0 + y_hat_1) * max(q_dec, .5)."""
+    q_dec, scales, means = params.chunk(3, 1)
+    q_dec = torch.clamp_min(q_dec, 0.5)
+    y = y * (1.0 / q_dec)
+    C, yH, yW = y.shape[1], y.shape[2], y.shape[3]
+    masks = mask_2x(C, yH, yW)
+    y_q0, yh0 = quantize_masked(y, means, scales, masks[0], skip_thres)
+    means1 = ld_spatial_prior(sd, yh0, params)
+    y_q1, yh1 = quantize_masked(y, means1, scales, masks[1], skip_thres)
+    sym16, _, cond = build_index(y_q0 + y_q1, scales, skip_thres)
+    res = LDResult(sym16=compact(sym16, cond), sym16_dense=sym16, cond_dense=cond, y=y, common_params=params)
+    res.y_hat = (yh0 + yh1) * q_dec
+    return res
+
+
+def ld_compress(sd, st: LDState, x, qp, reset_feature_memory, skip_thres=0.0, recon=False):
+    """DMCLDProxy::compress (dmc_ld_proxy.cpp:420-485); x: [1,3,H,W] one frame."""
+    y = ld_encoder(sd, pad_to(x, 16), st.ctx, qp)
+    z = ld_hyper_encoder(sd, pad_to(y, 4))
+    z_hat = round_z(z)
+    params = _ld_prior(sd, z_hat, st.temporal, qp)
+    res = ld_code_y(sd, y, params, skip_thres)
+    res.z_int8 = z_hat.permute(0, 2, 3, 1).reshape(-1).numpy().astype(np.int8)
+    # lambda_enc_1 (:466-470, 639-649)
+    st.feature_p = _ld_decoder(sd, res.y_hat, st.ctx, qp)
+    if recon:
+        res.x_hat, _ = ld_recon_head(sd, st.feature_p)
+    if reset_feature_memory:
+        _, head = ld_recon_head(sd, st.feature_p)
+        st.memory = ld_feature_adaptor_i(sd, head)
+    else:
+        st.memory = ld_feature_adaptor_m(sd, st.memory, st.feature_p)
+    st.ctx = ld_feature_extractor(sd, st.memory)
+    st.temporal = ld_temporal_prior(sd, st.memory)
+    st.memory_has_value = True
+    return res
+
+
+def ld_decompress(sd, st: LDState, z_int8, decode_y, qp, height, width, reset_feature_memory, skip_thres=0.0):
+    """DMCLDProxy::decompress (dmc_ld_proxy.cpp:487-593): one entropy round
+    trip.  ``decode_y(indexes_uint8) -> int8 symbols``."""
+    if st.memory_has_value:
+        st.memory = ld_feature_adaptor_m(sd, st.memory, st.feature_p)
+    else:
+        st.memory = ld_feature_adaptor_i(sd, st.feature_i)
+    st.temporal = ld_temporal_prior(sd, st.memory)
+    H16, W16 = (height + 15) // 16, (width + 15) // 16
+    H64, W64 = (H16 + 3) // 4, (W16 + 3) // 4
+    z_hat = torch.from_numpy(z_int8.astype(np.float32)).reshape(1, H64, W64, -1).permute(0, 3, 1, 2)
+    z_hat = z_hat.to(st.memory.dtype).contiguous()
+    params = _ld_prior(sd, z_hat, st.temporal, qp)
+    st.ctx = ld_feature_extractor(sd, st.memory)
+    q_dec, scales, means = params.chunk(3, 1)
+    q_dec = torch.clamp_min(q_dec, 0.5)
+    _, idx, cond = build_index(None, scales, skip_thres)
+    decoded = decode_y(compact(idx, cond))
+    C = scales.shape[1]
+    y_q = recover(decoded, cond, idx.size)
+    y_q = torch.from_numpy(y_q.astype(np.float32)).reshape(1, H16, W16, C).permute(0, 3, 1, 2)
+    y_q = y_q.to(params.dtype).contiguous()
+    masks = mask_2x(C, H16, W16)
+    yh0 = (y_q + means) * masks[0].to(y_q.dtype)
+    means1 = ld_spatial_prior(sd, yh0, params)
+    y_hat = (yh0 + (y_q + means1) * masks[1].to(y_q.dtype)) * q_dec
+    st.feature_p = _ld_decoder(sd, y_hat, st.ctx, qp)
+    x_hat, head = ld_recon_head(sd, st.feature_p)
+    st.feature_i = head
+    st.memory_has_value = not reset_feature_memory
+    return x_hat, y_hat
+
+
+# ---------------------------------------------------------------------------
 # per-op references used by the GPU kernel parity tests (float64)
 # ---------------------------------------------------------------------------
 
@@ -994,3 +1205,39 @@ def htl_quant_step_f32(y, scales, means, qdec, ysf_in, step, thr, C, H, W, n_img
 def htl_restore_step_f32(yq_quarter, means, qdec, ysf_in, step, C, H, W, n_img=1):
     q = np.maximum(qdec.astype(np.float32), f32(0.5))
     return intra_restore_step_f32(yq_quarter, means, ysf_in, q, step, C, H, W, n_img)
+
+
+def mask2_active(step, C, H, W, n_img=1):
+    """[n_img*H*W, C] bool: active positions of 2x checkerboard half `step`."""
+    m = mask_2x(C, H, W)[step][0].permute(1, 2, 0).reshape(H * W, C).numpy()
+    return np.tile(m, (n_img, 1))
+
+
+def ld_quant_step_f32(y, scales, means, qdec, yhat_in, step, thr, C, H, W, n_img=1):
+    """One LD encoder half (process_with_mask_no_scale(_add_and_multiply_inplace),
+    stream.cu:548-645) on [P, C] fp16 arrays -> (yhat fp16, sym, cond, active)."""
+    act = mask2_active(step, C, H, W, n_img)
+    q = np.maximum(qdec.astype(np.float32), f32(0.5))
+    ys = (y.astype(np.float32) * (f32(1.0) / q)).astype(np.float32)
+    mean = means.astype(np.float32)
+    sc = scales.astype(np.float32)
+    yq = _quant32(ys, mean, sc, thr)
+    yh = (yq.astype(np.float32) + mean).astype(np.float16)
+    sym = ((yq << 8) + scale_to_index(sc)).astype(np.int16)
+    cond = (sc > f32(thr)).astype(np.uint8)
+    old = np.zeros_like(yhat_in) if step == 0 else yhat_in
+    out = np.where(act, yh, old)
+    if step == 1:
+        out = (out.astype(np.float32) * q).astype(np.float16)
+    return out.astype(np.float16), sym, cond, act
+
+
+def ld_restore_step_f32(yq, means, qdec, yhat_in, step, C, H, W, n_img=1):
+    """restore_y / restore_y_and_add_multiply_inplace (stream.cu:685-754)."""
+    act = mask2_active(step, C, H, W, n_img)
+    yh = (yq.astype(np.float32) + means.astype(np.float32)).astype(np.float16)
+    old = np.zeros_like(yhat_in) if step == 0 else yhat_in
+    out = np.where(act, yh, old)
+    if step == 1:
+        out = (out.astype(np.float32) * np.maximum(qdec.astype(np.float32), f32(0.5))).astype(np.float16)
+    return out.astype(np.float16)

@@ -46,8 +46,8 @@ def load(model, sd):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--reference", default=os.environ.get("DCVC_REFERENCE", "/root/reference"))
-    ap.add_argument("--parts", default="base,htl",
-                    help="comma list: base (DMCI, HT-S, masks, CDFs), htl (HT-L modules)")
+    ap.add_argument("--parts", default="base,htl,ld",
+                    help="comma list: base (DMCI, HT-S, masks, CDFs), htl (HT-L modules), ld (low-delay modules)")
     args = ap.parse_args()
     sys.path.insert(0, args.reference)
     sys.path.insert(0, os.path.join(REPO, "oracle", "_ref"))
@@ -60,6 +60,9 @@ def main():
         save(base_vectors(DMCI, DMC, ModelStructure), "reference_modules.npz")
     if "htl" in parts:
         save(htl_vectors(DMC, ModelStructure), "reference_htl.npz")
+    if "ld" in parts:
+        from src.models.video_model_ld import DMC as DMCLD
+        save(ld_vectors(DMCLD), "reference_ld.npz")
 
 
 def base_vectors(DMCI, DMC, ModelStructure):
@@ -142,6 +145,46 @@ def base_vectors(DMCI, DMC, ModelStructure):
         out["cdf.hts_z.quantized_cdf"] = torch.from_numpy(zc)
         out["cdf.hts_z.cdf_length"] = torch.from_numpy(zl)
 
+    return out
+
+
+def ld_vectors(DMCLD):
+    """Every low-delay submodule the proxy runs (video_model_ld.py), on seeded
+    inputs at 64 x 64 pixels, plus the reference's 2x checkerboard masks."""
+    out = {}
+    with torch.no_grad():
+        sd = weights.round_to_half(weights.perturb_biases(weights.init_state_dict("ld", 3), 103))
+        m = load(DMCLD(), sd)
+        H8 = W8 = 8
+        qe = torch.from_numpy(sd["q_encoder"][13])[None, :, None, None]
+        qdd = torch.from_numpy(sd["q_decoder"][13])[None, :, None, None]
+        qf = torch.from_numpy(sd["q_feature"][13])[None, :, None, None]
+        fi = seeded((1, 192, H8, W8), 30, 0.3)
+        out["ld.fa_i.in"] = fi
+        out["ld.fa_i.out"] = mem = m.feature_adaptor_i(fi)
+        fp = seeded((1, 256, H8, W8), 31, 0.5)
+        out["ld.fa_m.in_feature"] = fp
+        out["ld.fa_m.out"] = m.feature_adaptor_m(mem, fp)
+        out["ld.fe.out"] = ctx = m.feature_extractor(mem)
+        x = seeded((1, 3, 64, 64), 32, 0.25)
+        out["ld.encoder.in"] = x
+        out["ld.encoder.out"] = y = m.encoder(x, ctx, qe)
+        out["ld.hyper_encoder.out"] = z = m.hyper_encoder(y)
+        z_hat = torch.round(z * 3)
+        out["ld.hyper_decoder.in"] = z_hat
+        out["ld.hyper_decoder.out"] = hp = m.hyper_decoder(z_hat)
+        out["ld.tpe.out"] = tp = m.temporal_prior_encoder(mem)
+        out["ld.prior_fusion.out"] = params = m.y_prior_fusion(hp, tp, qf)
+        yh = seeded((1, 128, 4, 4), 33)
+        out["ld.sp.in_yhat"] = yh
+        out["ld.sp.out"] = m.y_spatial_prior(yh, params)
+        yd = seeded((1, 128, 4, 4), 34)
+        out["ld.decoder.in"] = yd
+        out["ld.decoder.out"] = feat = m.decoder(yd, ctx, qdd)
+        out["ld.recon"] = m.recon_head(feat)
+        out["ld.recon.reset"] = m.recon_head(feat, for_reset=True)
+        for i, mk in enumerate(m.get_mask_2x(1, 8, 5, 6, "cpu")):
+            out[f"mask2x.{i}"] = mk.float()
     return out
 
 

@@ -1,0 +1,137 @@
+"""End-to-end GPU parity of the low-delay codec (DMCLDProxy): one frame per
+call, one entropy round trip, a 2x checkerboard.
+
+* The decoder rebuilds, from the bitstream alone, the encoder's y_hat and
+  temporal context bit for bit over a sequence of frames with a
+  feature-memory reset, several QPs, and a size that needs 1/64 padding.
+* Against the FP32 CPU oracle on the same fp16-rounded weights: symbols,
+  indexes, bitstream size and PSNR, end to end and conditioned on the GPU's
+  own y and z_hat.
+"""
+import numpy as np
+import pytest
+import torch
+
+from dcvc_amd import entropy, synthetic, weights
+from oracle import dcvc_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _state(model, seed):
+    sd = weights.perturb_biases(weights.init_state_dict(model, seed), seed + 100)
+    return entropy.add_cdf_to_state_dict(weights.to_torch(weights.round_to_half(sd)))
+
+
+@pytest.fixture(scope="module")
+def models(gpu):
+    return _state("dmci", 0), _state("ld", 3)
+
+
+def _to_gpu(arr, gpu):
+    return torch.from_numpy(arr)[None].half().to(gpu).contiguous(memory_format=torch.channels_last)
+
+
+def psnr(a, b):
+    mse = float(((a.double() - b.double()) ** 2).mean())
+    return 10 * np.log10(1.0 / max(mse, 1e-12))
+
+
+def _intra(sdi, frame, gpu, H, W, pad_b, pad_r):
+    from dcvc_amd.inference_extensions_cuda import DMCIProxy
+    e = DMCIProxy()
+    e.set_param(sdi, 0.0)
+    bits, x_hat, ec = e.compress(_to_gpu(frame, gpu), 32, pad_b, pad_r)
+    d = DMCIProxy()
+    d.set_param(sdi, 0.0)
+    return x_hat.clone(), d.decompress(bits, 32, H, W, ec).clone()
+
+
+@pytest.mark.parametrize("H,W", [(128, 192), (136, 200)])
+def test_ld_frames_encoder_decoder_bitexact(gpu, models, H, W):
+    from dcvc_amd.inference_extensions_cuda import DMCLDProxy
+    sdi, sdp = models
+    pad_r, pad_b = synthetic.get_padding_size(H, W, 16)
+    Hp, Wp = H + pad_b, W + pad_r
+    frames = synthetic.motion_sequence_yuv420(H, W, 6, seed=9)
+    x_hat_i, x_hat_id = _intra(sdi, synthetic.frame_to_input(frames[0]), gpu, H, W, pad_b, pad_r)
+    enc, dec = DMCLDProxy(), DMCLDProxy()
+    enc.set_param(sdp, 0.15)
+    dec.set_param(sdp, 0.15)
+    enc.add_ref_feature_from_frame(x_hat_i, True)
+    dec.add_ref_feature_from_frame(x_hat_id, False)
+    P16, P8 = (Hp // 16) * (Wp // 16), (Hp // 8) * (Wp // 8)
+    for t, (reset, qp) in enumerate([(False, 32), (False, 0), (True, 47), (False, 63), (False, 20)], start=1):
+        bits, ec = enc.compress(_to_gpu(synthetic.frame_to_input(frames[t]), gpu), qp, reset, pad_b, pad_r)
+        yhat_e = enc.debug_copy(0, P16 * 512 * 2, np.float16).reshape(P16, 512)[:, :128].copy()
+        feat_e = enc.debug_copy(3, P8 * 512 * 2, np.float16).reshape(P8, 512)[:, 256:].copy()
+        n_e = enc.last_symbol_count()
+        x_hat = dec.decompress(bits, qp, H, W, ec, reset)
+        torch.cuda.synchronize()
+        yhat_d = dec.debug_copy(0, P16 * 512 * 2, np.float16).reshape(P16, 512)[:, :128]
+        feat_d = dec.debug_copy(3, P8 * 512 * 2, np.float16).reshape(P8, 512)[:, 256:]
+        assert dec.last_symbol_count() == n_e and n_e > 0
+        assert np.array_equal(yhat_e.view(np.uint16), yhat_d.view(np.uint16)), f"frame {t}: y_hat differs"
+        assert np.array_equal(feat_e.view(np.uint16), feat_d.view(np.uint16)), f"frame {t}: feature differs"
+        assert x_hat.shape == (1, 3, Hp, Wp)
+        assert torch.isfinite(x_hat.float()).all() and float(x_hat.abs().max()) <= 0.5
+
+
+def test_ld_symbols_and_psnr_vs_oracle(gpu, models):
+    from dcvc_amd.inference_extensions_cuda import DMCLDProxy
+    sdi, sdp = models
+    H, W = 128, 192
+    frames = synthetic.motion_sequence_yuv420(H, W, 2, seed=10)
+    x_hat_i, _ = _intra(sdi, synthetic.frame_to_input(frames[0]), gpu, H, W, 0, 0)
+    enc, dec = DMCLDProxy(), DMCLDProxy()
+    enc.set_param(sdp, 0.0)
+    dec.set_param(sdp, 0.0)
+    enc.add_ref_feature_from_frame(x_hat_i, True)
+    dec.add_ref_feature_from_frame(x_hat_i, False)
+    frame = synthetic.frame_to_input(frames[1])
+    bits, ec = enc.compress(_to_gpu(frame, gpu), 32, False, 0, 0)
+    H16, W16 = H // 16, W // 16
+    P16, ny = H16 * W16, H16 * W16 * 128
+    sym_g = enc.debug_copy(5, ny * 2, np.int16).astype(np.int64)
+    y_g = enc.debug_copy(1, P16 * 128 * 2, np.float16).astype(np.float32)
+    z_g = enc.debug_copy(6, (H16 // 4) * (W16 // 4) * 128, np.int8)
+    x_hat = dec.decompress(bits, 32, H, W, ec, False).float().cpu()
+    sdo = {k: v.float() for k, v in sdp.items()}
+    st = O.LDState()
+    O.ld_add_ref_feature_from_frame(sdo, st, x_hat_i.float().cpu().contiguous(), True)
+    temporal = st.temporal.clone()
+    r = O.ld_compress(sdo, st, torch.from_numpy(frame)[None].half().float(), 32, False, recon=True)
+    o = r.sym16_dense.astype(np.int64)
+    sym_m = float(np.mean((sym_g >> 8) == (o >> 8)))
+    idx_m = float(np.mean((sym_g & 0xFF) == (o & 0xFF)))
+    y_o = (r.y * torch.clamp_min(r.common_params[:, :128], 0.5)).permute(0, 2, 3, 1).reshape(-1).numpy()
+    y_rel = float(np.linalg.norm(y_g - y_o) / np.linalg.norm(y_o))
+    z_match = float(np.mean(z_g == r.z_int8))
+    z_hat = torch.from_numpy(z_g.astype(np.float32)).reshape(1, H16 // 4, W16 // 4, 128).permute(0, 3, 1, 2)
+    params_c = O._ld_prior(sdo, z_hat.contiguous(), temporal, 32)
+    y_c = torch.from_numpy(y_g.reshape(1, H16, W16, 128)).permute(0, 3, 1, 2).contiguous()
+    rc = O.ld_code_y(sdo, y_c, params_c)
+    oc = rc.sym16_dense.astype(np.int64)
+    sym_c = float(np.mean((sym_g >> 8) == (oc >> 8)))
+    idx_c = float(np.mean((sym_g & 0xFF) == (oc & 0xFF)))
+    enc_o = entropy.RansEncoder()
+    enc_o.set_cdf(sdp["bit_estimator_z.quantized_cdf"].numpy(), sdp["bit_estimator_z.cdf_length"].numpy(), 0)
+    enc_o.set_cdf(sdp["gaussian_encoder.quantized_cdf"].numpy(), sdp["gaussian_encoder.cdf_length"].numpy(), 1)
+    enc_o.reset()
+    enc_o.set_entropy_coder_parallel(ec)
+    enc_o.encode_y(r.sym16)
+    enc_o.encode_z(r.z_int8, 32 * 128, 128)
+    enc_o.flush()
+    n_o = len(enc_o.get_encoded_stream())
+    src = torch.from_numpy(frame)[None]
+    pg, po = psnr(x_hat, src), psnr(r.x_hat, src)
+    print(f"LD vs FP32 oracle: y rel {y_rel:.2e}, z match {z_match:.4f}; symbols {sym_m:.5f} indexes {idx_m:.5f}; "
+          f"conditioned on the GPU's y and z_hat: symbols {sym_c:.5f} indexes {idx_c:.5f}; bytes gpu {len(bits)} "
+          f"oracle {n_o}; PSNR gpu {pg:.4f} oracle {po:.4f}")
+    # measured on MI355X: y rel 9.9e-4, z 99.87 %; end to end symbols 99.41 %,
+    # indexes 94.4 %; conditioned 99.91 % / 99.13 %; PSNR within 0.001 dB
+    assert y_rel < 4e-3 and z_match > 0.99
+    assert sym_c > 0.995 and idx_c > 0.95
+    assert sym_m > 0.98 and idx_m > 0.85
+    assert abs(len(bits) - n_o) / n_o < 0.01
+    assert abs(pg - po) < 0.05

@@ -446,6 +446,44 @@ def test_intra_quant_and_restore_steps(gpu, H, W, n_img, thr):
     assert np.array_equal(yout_d.cpu().numpy().view(np.uint16), yout_r.view(np.uint16))
 
 
+@pytest.mark.parametrize("H,W,n_img,thr", [(17, 30, 1, 0.0), (16, 24, 2, 0.15), (68, 120, 1, 0.15)])
+def test_ld_quant_and_restore_steps(gpu, H, W, n_img, thr):
+    """LD boundary: 2x checkerboard over channel halves, the common scales in
+    both halves, y_hat * max(q_dec, .5) after the second; bit-exact against the
+    float32 restatement, and the decoder's indexes and restore reproduce
+    y_hat from the encoder's symbols."""
+    from dcvc_amd import kernels
+    r = rng(H * 5 + n_img)
+    C, P = 128, n_img * H * W
+    y, scales, mean0, qdec = _rand_prior(r, P, C)
+    means_steps = [mean0, (r.standard_normal((P, C)) * 2).astype(np.float16)]
+    yhat_g = torch.full((P, C), 7.0, dtype=torch.float16, device=gpu)
+    sym_g = torch.zeros(P * C, dtype=torch.int16, device=gpu)
+    cond_g = torch.zeros(P * C, dtype=torch.uint8, device=gpu)
+    yhat_r = np.full((P, C), 7.0, np.float16)
+    sym_r = np.zeros((P, C), np.int16)
+    cond_r = np.zeros((P, C), np.uint8)
+    for step in range(2):
+        kernels.ld_quant_step(f16(y, gpu), n_img, H, W, f16(scales, gpu), f16(means_steps[step], gpu),
+                              f16(qdec, gpu), yhat_g, sym_g, cond_g, thr, step)
+        yhat_r, s, c, act = O.ld_quant_step_f32(y, scales, means_steps[step], qdec, yhat_r, step, thr, C, H, W, n_img)
+        sym_r[act] = s[act]
+        cond_r[act] = c[act]
+    torch.cuda.synchronize()
+    assert np.array_equal(yhat_g.cpu().numpy().view(np.uint16), yhat_r.view(np.uint16))
+    assert np.array_equal(sym_g.cpu().numpy(), sym_r.reshape(-1))
+    assert np.array_equal(cond_g.cpu().numpy(), cond_r.reshape(-1))
+    idx, cond = kernels.hts_build_index_dec(f16(scales, gpu), n_img, H, W, thr)
+    assert np.array_equal(cond.cpu().numpy(), cond_r.reshape(-1))
+    yq = torch.from_numpy((sym_r.reshape(-1).astype(np.int32) >> 8).astype(np.int8)).to(gpu)
+    yq = torch.where(cond.bool(), yq, torch.zeros_like(yq))
+    yh_dec = torch.full((P, C), -3.0, dtype=torch.float16, device=gpu)
+    for step in range(2):
+        kernels.ld_restore_step(yq, n_img, H, W, C, f16(means_steps[step], gpu), f16(qdec, gpu), yh_dec, step)
+    torch.cuda.synchronize()
+    assert np.array_equal(yh_dec.cpu().numpy().view(np.uint16), yhat_r.view(np.uint16))
+
+
 @pytest.mark.parametrize("H,W,n_img,thr", [(16, 16, 1, 0.0), (17, 30, 2, 0.15), (68, 120, 1, 0.15)])
 def test_htl_quant_and_restore_steps(gpu, H, W, n_img, thr):
     """HT-L boundary: per-step scales and means, y / max(q_dec, .5) quantised,

@@ -143,3 +143,32 @@ def test_htl_modules(sd_htl):
     _, reset_head = O.htl_recon_head(sd, gl("htl.decoder.out"), only_reset=True)
     close(reset_head, gl("htl.recon.reset"))
     close(head, gl("htl.recon.reset"))
+
+
+GD = np.load(os.path.join(os.path.dirname(__file__), "golden", "reference_ld.npz"))
+
+
+def gd(key):
+    return torch.from_numpy(GD[key].astype(np.float32))
+
+
+def test_ld_modules():
+    """Every low-delay submodule of the oracle against the reference modules
+    (video_model_ld.py) on the same weights, and the 2x checkerboard masks."""
+    sd = weights.to_torch(weights.round_to_half(weights.perturb_biases(weights.init_state_dict("ld", 3), 103)))
+    close(O.ld_feature_adaptor_i(sd, gd("ld.fa_i.in")), gd("ld.fa_i.out"))
+    close(O.ld_feature_adaptor_m(sd, gd("ld.fa_i.out"), gd("ld.fa_m.in_feature")), gd("ld.fa_m.out"))
+    close(O.ld_feature_extractor(sd, gd("ld.fa_i.out")), gd("ld.fe.out"))
+    close(O.ld_encoder(sd, gd("ld.encoder.in"), gd("ld.fe.out"), 13), gd("ld.encoder.out"))
+    close(O.ld_hyper_encoder(sd, gd("ld.encoder.out")), gd("ld.hyper_encoder.out"))
+    close(O.ld_hyper_decoder(sd, gd("ld.hyper_decoder.in")), gd("ld.hyper_decoder.out"))
+    close(O.ld_temporal_prior(sd, gd("ld.fa_i.out")), gd("ld.tpe.out"))
+    close(O._ld_prior(sd, gd("ld.hyper_decoder.in"), gd("ld.tpe.out"), 13), gd("ld.prior_fusion.out"))
+    close(O.ld_spatial_prior(sd, gd("ld.sp.in_yhat"), gd("ld.prior_fusion.out")), gd("ld.sp.out"))
+    close(O._ld_decoder(sd, gd("ld.decoder.in"), gd("ld.fe.out"), 13), gd("ld.decoder.out"))
+    x_hat, head = O.ld_recon_head(sd, gd("ld.decoder.out"))
+    close(x_hat, torch.clamp(gd("ld.recon"), -0.5, 0.5))
+    close(head, gd("ld.recon.reset"))
+    ours = O.mask_2x(8, 5, 6)
+    for i in range(2):
+        assert torch.equal(ours[i].float(), gd(f"mask2x.{i}"))

@@ -224,3 +224,67 @@ def test_htl_fp32_step_restatement_agrees_with_torch_steps():
         else:
             ref = ((yh + so_far) * to_nchw(q32)).permute(0, 2, 3, 1).reshape(P, C).numpy()
             np.testing.assert_allclose(yout.astype(np.float32), ref, rtol=2 ** -10, atol=2 ** -14)
+
+
+def test_oracle_ld_encode_decode_round_trip():
+    """Low delay: one entropy round trip over all of y.  The oracle decoder,
+    fed the encoder's compacted symbols, asks for exactly the encoder's
+    indexes and rebuilds y_hat, the reconstruction and the context (incl.
+    the temporal params) exactly, over two frames with a reset between."""
+    sd = {k: v.float() for k, v in weights.to_torch(weights.round_to_half(weights.init_state_dict("ld", 3))).items()}
+    H = W = 64
+    frames = synthetic.motion_sequence_yuv420(H, W, 3, seed=4)
+    ref = torch.from_numpy(synthetic.frame_to_input(frames[0]))[None]
+    st_e, st_d = O.LDState(), O.LDState()
+    with torch.inference_mode():
+        O.ld_add_ref_feature_from_frame(sd, st_e, ref, True)
+        O.ld_add_ref_feature_from_frame(sd, st_d, ref, False)
+        for t, reset in ((1, True), (2, False)):
+            x = torch.from_numpy(synthetic.frame_to_input(frames[t]))[None]
+            r = O.ld_compress(sd, st_e, x, 50, reset, skip_thres=0.15, recon=True)
+            seen = {}
+
+            def decode_y(idx):
+                seen["idx"] = idx
+                return (r.sym16.astype(np.int64) >> 8).astype(np.int8)
+
+            x_hat, y_hat = O.ld_decompress(sd, st_d, r.z_int8, decode_y, 50, H, W, reset, skip_thres=0.15)
+            assert np.array_equal(seen["idx"], (r.sym16.astype(np.int64) & 0xFF).astype(np.uint8))
+            assert torch.equal(y_hat, r.y_hat)
+            assert torch.equal(x_hat, r.x_hat)
+            assert torch.equal(st_d.feature_p, st_e.feature_p)
+    # the decoder's next-frame context equals the encoder's (applied lazily on the decoder side)
+    st_d.memory = O.ld_feature_adaptor_m(sd, st_d.memory, st_d.feature_p)
+    assert torch.equal(st_d.memory, st_e.memory)
+    assert torch.equal(O.ld_temporal_prior(sd, st_d.memory), st_e.temporal)
+
+
+def test_ld_fp32_step_restatement_agrees_with_torch_steps():
+    """LD form: the float32 two-half restatement and the torch NCHW masked
+    halves give the same symbols, conditions and y_hat * max(q, .5)."""
+    r = np.random.Generator(np.random.PCG64(12))
+    C, H, W = 8, 5, 6
+    P = H * W
+    y = (r.standard_normal((P, C)) * 3).astype(np.float16)
+    qd = (r.random((P, C)) * 1.5).astype(np.float16)
+    sc = np.exp(r.uniform(np.log(0.05), np.log(20), (P, C))).astype(np.float16)
+    thr = 0.15
+    to_nchw = lambda a: torch.from_numpy(a.astype(np.float32).reshape(1, H, W, C)).permute(0, 3, 1, 2)
+    masks = O.mask_2x(C, H, W)
+    q32 = np.maximum(qd.astype(np.float32), np.float32(0.5))
+    ys_t = to_nchw((y.astype(np.float32) * (np.float32(1) / q32)).astype(np.float32))
+    yh = np.zeros((P, C), np.float16)
+    sym_all = np.zeros((P, C), np.int16)
+    yq_t = torch.zeros(1, C, H, W)
+    yh_t = torch.zeros(1, C, H, W)
+    for step in range(2):
+        mn = r.standard_normal((P, C)).astype(np.float16)
+        yh, sym, cond, act = O.ld_quant_step_f32(y, sc, mn, qd, yh, step, thr, C, H, W)
+        sym_all[act] = sym[act]
+        q_t, h_t = O.quantize_masked(ys_t, to_nchw(mn), to_nchw(sc), masks[step], thr)
+        yq_t, yh_t = yq_t + q_t, yh_t + h_t
+    sym_t, _, cond_t = O.build_index(yq_t, to_nchw(sc), thr)
+    assert np.array_equal(sym_all.reshape(-1), sym_t)
+    assert np.array_equal(cond.reshape(-1).astype(bool), cond_t)
+    ref = (yh_t * to_nchw(q32)).permute(0, 2, 3, 1).reshape(P, C).numpy()
+    np.testing.assert_allclose(yh.astype(np.float32), ref, rtol=2 ** -10, atol=2 ** -14)
