@@ -11,7 +11,8 @@ followed by its decompress (decoder proxy), including the CPU entropy coder.
 `value` counts the 4 real frames per step.
 
 Also reported: p50/p99 step latency over the timed steps, the native
-8-frame rate, the HT-L chunk model on the same stream, the per-launch layer table of one encode + decode (algorithmic
+8-frame rate, the HT-L chunk model and the low-delay model on the same
+stream, the per-launch layer table of one encode + decode (algorithmic
 FLOPs / bytes, each launch timed with HIP events on its own stream), the
 roofline of the kernel class that takes the most time, and the FP32 CPU
 oracle timed on this host.
@@ -231,6 +232,39 @@ def htl_form(x_hat_e, x_hat_d, chunks, H, W, pad_b, pad_r, skip, qp, steps=40, w
     return out
 
 
+def ld_form(x_hat_e, x_hat_d, chunks, H, W, pad_b, pad_r, skip, qp, steps=60, warmup=4):
+    """The low-delay model (one frame per call, one entropy round trip) on
+    the same stream's frames: frames/s, p50 / p99 per-frame latency, and the
+    GEMM work of one encode + decode."""
+    import torch
+    from dcvc_amd import codec
+    _, sdl = codec.make_state_dicts(seed=0, video="ld")
+    e = codec.VideoCodecLD(sdl, skip)
+    d = codec.VideoCodecLD(share_from=e)
+    e.add_ref_feature_from_frame(x_hat_e)
+    d.add_ref_feature_from_frame(x_hat_d, apply_feature_adaptor=False)
+    frames = [c[:, 3 * f:3 * f + 3].contiguous(memory_format=torch.channels_last) for c in chunks for f in range(4)]
+    lat, nbytes = [], []
+    for i in range(warmup + steps):
+        t0 = time.perf_counter()
+        bits, ec = e.proxy.compress(frames[i % len(frames)], qp, False, pad_b, pad_r)
+        d.proxy.decompress(bits, qp, H, W, ec, False)
+        torch.cuda.synchronize()
+        if i >= warmup:
+            lat.append(time.perf_counter() - t0)
+            nbytes.append(len(bits))
+    total = float(np.sum(lat))
+    out = {"frames_per_s": round(steps / total, 3), "ms_per_frame": round(1000 * total / steps, 3),
+           "p50_ms": round(1000 * float(np.percentile(lat, 50)), 3),
+           "p99_ms": round(1000 * float(np.percentile(lat, 99)), 3), "steps": steps,
+           "bpp": round(8 * float(np.mean(nbytes)) / (H * W), 5), "precision": "fp16"}
+    rec_e, rec_d = layer_trace(e, d, frames[0], qp, pad_b, pad_r, H, W)
+    out["encode_gemm_tflop"] = round(float(rec_e[rec_e[:, 0] <= 2][:, 5].sum()) / 1e12, 4)
+    out["decode_gemm_tflop"] = round(float(rec_d[rec_d[:, 0] <= 2][:, 5].sum()) / 1e12, 4)
+    out["traced_kernel_ms"] = round(float(np.concatenate([rec_e, rec_d])[:, 7].sum()) * 1e-6, 3)
+    return out
+
+
 def batching_form(sdi, sdp, rounds=2):
     """16 concurrent streams of uneven resolution (4x 854x480, 6x 1280x720,
     4x 1920x1080, 2x 3840x2160), one 4-frame chunk each per round at a random
@@ -372,6 +406,7 @@ def main():
     ap.add_argument("--no-precision-forms", action="store_true")
     ap.add_argument("--no-batching", action="store_true")
     ap.add_argument("--no-htl", action="store_true", help="skip the HT-L chunk-model section")
+    ap.add_argument("--no-ld", action="store_true", help="skip the low-delay model section")
     ap.add_argument("--serving-seconds", type=float, default=10.0,
                     help="length of the Poisson serving trace (0 skips it)")
     args = ap.parse_args()
@@ -487,6 +522,9 @@ def main():
         log("HT-L form")
         out["htl_form"] = htl_form(enc_i["x_hat"], dec_i["x_hat"], chunks, H, W, pad_b, pad_r, args.skip_thres,
                                    args.qp)
+    if not args.no_ld:
+        log("LD form")
+        out["ld_form"] = ld_form(enc_i["x_hat"], dec_i["x_hat"], chunks, H, W, pad_b, pad_r, args.skip_thres, args.qp)
     if not args.no_batching:
         log("batching form")
         out["batching_form"] = batching_form(sdi, sdp)
