@@ -95,6 +95,9 @@ void DMCIProxy::set_param(const HostDict& d, float skip_thres) {
 }
 
 void DMCIProxy::reset_graphs() {
+    // a graph may still be running on the proxy stream (the encoder returns
+    // before its context update finishes)
+    if (m_stream) hipStreamSynchronize(m_stream);
     m_g_enc0.reset();
     m_g_enc1.reset();
     for (auto& g : m_g_dec) g.reset();

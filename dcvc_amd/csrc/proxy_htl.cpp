@@ -85,9 +85,12 @@ void HTLWeights::load(const HostDict& d, float skip) {
 HTLProxy::HTLProxy() {
     hip_check(hipStreamCreateWithFlags(&m_stream, hipStreamNonBlocking), "stream");
     hip_check(hipStreamCreateWithFlags(&m_copy, hipStreamNonBlocking), "stream");
+    hip_check(hipStreamCreateWithFlags(&m_side, hipStreamNonBlocking), "stream");
     hip_check(hipEventCreateWithFlags(&m_ev_in, hipEventDisableTiming), "event");
     hip_check(hipEventCreateWithFlags(&m_ev_out, hipEventDisableTiming), "event");
     hip_check(hipEventCreateWithFlags(&m_ev_sym, hipEventDisableTiming), "event");
+    hip_check(hipEventCreateWithFlags(&m_ev_fork, hipEventDisableTiming), "event");
+    hip_check(hipEventCreateWithFlags(&m_ev_join, hipEventDisableTiming), "event");
 }
 
 static void free_pinned(int*& counts, std::array<int16_t*, 4>& sym, uint8_t*& idx, int8_t*& dec, int8_t*& z) {
@@ -107,12 +110,16 @@ static void free_pinned(int*& counts, std::array<int16_t*, 4>& sym, uint8_t*& id
 
 HTLProxy::~HTLProxy() {
     if (m_stream) hipStreamSynchronize(m_stream);
+    if (m_side) hipStreamSynchronize(m_side);
     reset_graphs();
     free_pinned(m_h_counts, m_h_sym, m_h_idx, m_h_dec, m_h_z);
     m_buf.free_all();
     hipEventDestroy(m_ev_in);
     hipEventDestroy(m_ev_out);
     hipEventDestroy(m_ev_sym);
+    hipEventDestroy(m_ev_fork);
+    hipEventDestroy(m_ev_join);
+    hipStreamDestroy(m_side);
     hipStreamDestroy(m_copy);
     hipStreamDestroy(m_stream);
 }
@@ -133,11 +140,17 @@ void HTLProxy::share_weights(const std::shared_ptr<HTLWeights>& w) {
 }
 
 void HTLProxy::reset_graphs() {
+    // a graph may still be running on the proxy stream (the encoder returns
+    // before its context update finishes)
+    if (m_stream) hipStreamSynchronize(m_stream);
+    if (m_side) hipStreamSynchronize(m_side);
     m_g_enc0.reset();
     for (auto& g : m_g_enc1) g.reset();
     for (auto& g : m_g_dec0) g.reset();
     m_g_dec1.reset();
     for (auto& g : m_g_dec) g.reset();
+    m_g_fe.reset();
+    for (auto& g : m_g_rh) g.reset();
 }
 
 // pre_allocate_tensors (dmc_htl_proxy.cpp:1042-1178)
@@ -172,6 +185,12 @@ void HTLProxy::prepare(int Hp, int Wp, f16* const frames[kFrames], int ld_out) {
     m_scratch.so = M.alloc_n<f16>(m_scratch.halves);
     m_scratch.x1 = M.alloc_n<f16>(m_scratch.halves);
     m_scratch.x2 = M.alloc_n<f16>(m_scratch.halves);
+    m_scratch2.halves = m_scratch.halves;
+    m_scratch2.t1 = M.alloc_n<f16>(m_scratch2.halves);
+    m_scratch2.t2 = M.alloc_n<f16>(m_scratch2.halves);
+    m_scratch2.so = M.alloc_n<f16>(m_scratch2.halves);
+    m_scratch2.x1 = M.alloc_n<f16>(m_scratch2.halves);
+    m_scratch2.x2 = M.alloc_n<f16>(m_scratch2.halves);
     m_d_qp = M.alloc_n<int>(1);
     m_d_counts = M.alloc_n<int>(4);
     m_d_offs = M.alloc_n<int>(compact_blocks(m_nq) + 1);
@@ -197,6 +216,8 @@ void HTLProxy::prepare(int Hp, int Wp, f16* const frames[kFrames], int ld_out) {
     m_y_hat = M.alloc_n<f16>(P16 * 256);
     m_rh_a = M.alloc_n<f16>(P8 * 256);
     m_rh_head = M.alloc_n<f16>(P8 * 192);
+    m_rh_a2 = M.alloc_n<f16>(P8 * 256);
+    m_rh_head2 = M.alloc_n<f16>(P8 * 192);
     m_z_i8 = M.alloc_n<int8_t>(m_nz);
     m_yq = M.alloc_n<int8_t>(m_nq);
     m_dec_c = M.alloc_n<int8_t>(m_nq);
@@ -347,22 +368,49 @@ void HTLProxy::decoder_and_recon(bool recon_all, bool reset_for_encoder) {
     const Act tmp = make_act(m_scratch.x1, 1, m_H8, m_W8, 512, 512);
     QuantSel qd{w.q_decoder, 512, true};  // separate fp16 multiply after the last block (:40)
     w.dec.forward(e, decin, fp, tmp, &qd);
-    const Act ra = make_act(m_rh_a, 1, m_H8, m_W8, 256, 256);
-    const Act tmp256 = make_act(m_scratch.x2, 1, m_H8, m_W8, 256, 256);
-    const Act fi = make_act(m_feature_i, 1, m_H8, m_W8, 192, 192);
     if (recon_all) {
-        // DMCHTLReconHeadProxy::forward (:361-380); frame 7's head is also the reset feature
-        for (int i = 0; i < kFrames; ++i) {
-            w.rh[i].forward(e, fp, ra, tmp256);
-            const Act head = (i == kFrames - 1) ? fi : make_act(m_rh_head, 1, m_H8, m_W8, 192, 192);
-            w.rh_head[i].run(e, ra, head, EPI_BIAS);
-            hip_check(pixel_shuffle8_clamp(head.p, 192, 1, m_H8, m_W8, 3, m_frames[i], m_ld_out, m_stream), "shuffle");
-        }
+        recon_frames(e, 0, kFrames, m_rh_a, m_rh_head);
     } else if (reset_for_encoder) {
         // forward_reset (:347-359): frame 7's head only
-        w.rh[7].forward(e, fp, ra, tmp256);
-        w.rh_head[7].run(e, ra, fi, EPI_BIAS);
+        const Act ra = make_act(m_rh_a, 1, m_H8, m_W8, 256, 256);
+        w.rh[7].forward(e, fp, ra, make_act(e.sc->x2, 1, m_H8, m_W8, 256, 256));
+        w.rh_head[7].run(e, ra, make_act(m_feature_i, 1, m_H8, m_W8, 192, 192), EPI_BIAS);
     }
+}
+
+// DMCHTLReconHeadProxy::forward (:361-380) for frames [first, last); frame 7's
+// head is also the reset feature
+void HTLProxy::recon_frames(const Exec& e, int first, int last, f16* ra_buf, f16* head_buf) {
+    const HTLWeights& w = *m_w;
+    const Act fp = make_act(m_cat_fam + 512, 1, m_H8, m_W8, 512, 1024);
+    const Act ra = make_act(ra_buf, 1, m_H8, m_W8, 256, 256);
+    const Act tmp256 = make_act(e.sc->x2, 1, m_H8, m_W8, 256, 256);
+    const Act fi = make_act(m_feature_i, 1, m_H8, m_W8, 192, 192);
+    for (int i = first; i < last; ++i) {
+        w.rh[i].forward(e, fp, ra, tmp256);
+        const Act head = (i == kFrames - 1) ? fi : make_act(head_buf, 1, m_H8, m_W8, 192, 192);
+        w.rh_head[i].run(e, ra, head, EPI_BIAS);
+        hip_check(pixel_shuffle8_clamp(head.p, 192, 1, m_H8, m_W8, 3, m_frames[i], m_ld_out, e.stream), "shuffle");
+    }
+}
+
+void HTLProxy::fork_side() {
+    hip_check(hipEventRecord(m_ev_fork, m_stream), "event record");
+    hip_check(hipStreamWaitEvent(m_side, m_ev_fork, 0), "stream wait");
+}
+
+void HTLProxy::join_side() {
+    hip_check(hipEventRecord(m_ev_join, m_side), "event record");
+    hip_check(hipStreamWaitEvent(m_stream, m_ev_join, 0), "stream wait");
+}
+
+// the decoder's ctx from the new memory (lambda_dec_1's feature extractor,
+// :775), on the side stream: only the decoder at the last step reads it
+void HTLProxy::fe_side() {
+    Exec e = exec_side();
+    const Act mem = make_act(m_cat_fam, 1, m_H8, m_W8, 512, 1024);
+    const Act tmp = make_act(m_scratch2.x1, 1, m_H8, m_W8, 512, 512);
+    m_w->fe.forward(e, mem, make_act(m_cat_enc + 1536, 1, m_H8, m_W8, 512, 2048), tmp);
 }
 
 // lambda_enc_1 (dmc_htl_proxy.cpp:701-708, 969-979)
@@ -389,8 +437,11 @@ int HTLProxy::compress(const f16* x, int ldx, int H, int W, int qp, bool reset, 
     hip_check(pad_unshuffle(x, ldx, 1, H, W, 24, 8, pad_b, pad_r, m_cat_enc, 2048, m_stream), "unshuffle");
     m_g_enc0.run(m_stream, m_use_graphs, [&] { enc_stage0(); });
     hip_check(hipEventRecord(m_ev_sym, m_stream), "event");
-    m_g_enc1[reset ? 1 : 0].run(m_stream, m_use_graphs, [&] { enc_stage1(reset); });
+    // the caller needs nothing from the context update (it only feeds this
+    // proxy's next call, ordered on the proxy stream): release the caller's
+    // stream now, so a decoder on it overlaps the update
     leave(s);
+    m_g_enc1[reset ? 1 : 0].run(m_stream, m_use_graphs, [&] { enc_stage1(reset); });
     // entropy coding on the host (steps 3..0, then z; :514-545) while the GPU
     // runs the decoder and the context update
     hip_check(hipStreamWaitEvent(m_copy, m_ev_sym, 0), "wait");
@@ -438,14 +489,10 @@ void HTLProxy::dec_index(int step) {
     hip_check(compact(m_idx, 1, m_cond, m_nq, m_idxc, m_d_offs, m_d_counts, m_stream), "compact");
 }
 
-// lambda_dec_1 (dmc_htl_proxy.cpp:765-785)
+// lambda_dec_1 (dmc_htl_proxy.cpp:765-785) without the feature extractor (fe_side)
 void HTLProxy::dec_stage1() {
-    Exec e = exec();
     hip_check(i8_to_f16(m_z_i8, (long)m_H64 * m_W64, 128, m_z_hat, 128, m_stream), "i8");
     hyper_and_prior();
-    const Act mem = make_act(m_cat_fam, 1, m_H8, m_W8, 512, 1024);
-    const Act tmp = make_act(m_scratch.x1, 1, m_H8, m_W8, 512, 512);
-    m_w->fe.forward(e, mem, make_act(m_cat_enc + 1536, 1, m_H8, m_W8, 512, 2048), tmp);
     dec_index(0);
 }
 
@@ -484,6 +531,9 @@ void HTLProxy::decompress(const uint8_t* bits, long nbytes, int qp, int height, 
     enter(s);
     set_device_int(m_d_qp, qp, m_stream);
     m_g_dec0[m_memory_has_value ? 1 : 0].run(m_stream, m_use_graphs, [&] { dec_stage0(m_memory_has_value); });
+    fork_side();
+    m_g_fe.run(m_side, m_use_graphs, [&] { fe_side(); });
+    hip_check(hipEventRecord(m_ev_join, m_side), "event record");
     // z decode on the host, concurrent with dec_0 on the GPU
     m_dec.set_parallel(ec_parallel);
     m_dec.set_stream(bits, nbytes);
@@ -492,16 +542,22 @@ void HTLProxy::decompress(const uint8_t* bits, long nbytes, int qp, int height, 
     m_g_dec1.run(m_stream, m_use_graphs, [&] { dec_stage1(); });
     for (int step = 0; step < 4; ++step) {
         decode_step_on_host(step);
+        if (step == 3) hip_check(hipStreamWaitEvent(m_stream, m_ev_join, 0), "stream wait");  // ctx
         m_g_dec[step].run(m_stream, m_use_graphs, [&] {
             dec_restore(step);
             if (step < 3) {
                 spatial_prior(step + 1);
                 dec_index(step + 1);
             } else {
-                decoder_and_recon(true, false);
+                decoder_and_recon(false, false);
             }
         });
     }
+    // recon head: frames 4-7 on the side stream, 0-3 here
+    fork_side();
+    m_g_rh[1].run(m_side, m_use_graphs, [&] { recon_frames(exec_side(), kFrames / 2, kFrames, m_rh_a2, m_rh_head2); });
+    m_g_rh[0].run(m_stream, m_use_graphs, [&] { recon_frames(exec(), 0, kFrames / 2, m_rh_a, m_rh_head); });
+    join_side();
     leave(s);
     m_memory_has_value = !reset;  // host state, outside the graphs (:912)
     m_memory_ready = false;

@@ -71,8 +71,16 @@ private:
     void hyper_and_prior();
     void spatial_prior(int step);
     void decoder_and_recon(bool recon_all, bool reset_for_encoder);
+    // recon head frames [first, last) on `e`'s stream with the given buffers
+    void recon_frames(const Exec& e, int first, int last, f16* ra, f16* head);
+    void fe_side();
+    void fork_side();
+    void join_side();
     void feature_adaptor(bool use_m);
     Exec exec() { return Exec{m_stream, m_d_qp, &m_scratch, m_prec}; }
+    // the decoder's side stream: feature extraction during the prior and
+    // entropy round trips, half of the recon head (own scratch)
+    Exec exec_side() { return Exec{m_side, m_d_qp, &m_scratch2, m_prec}; }
     // scales / means of checkerboard step k: the common params for step 0,
     // the spatial prior output after
     const f16* step_scales(int step) const { return step == 0 ? m_common + 256 : m_sp_o; }
@@ -82,11 +90,11 @@ private:
     std::shared_ptr<HTLWeights> m_w;
     RansEncoder m_enc;
     RansDecoder m_dec;
-    hipStream_t m_stream = nullptr, m_copy = nullptr;
-    hipEvent_t m_ev_in = nullptr, m_ev_out = nullptr, m_ev_sym = nullptr;
+    hipStream_t m_stream = nullptr, m_copy = nullptr, m_side = nullptr;
+    hipEvent_t m_ev_in = nullptr, m_ev_out = nullptr, m_ev_sym = nullptr, m_ev_fork = nullptr, m_ev_join = nullptr;
     bool m_use_graphs = true;
     int m_prec = PREC_F16;
-    Graph m_g_enc0, m_g_enc1[2], m_g_dec0[2], m_g_dec1, m_g_dec[4];
+    Graph m_g_enc0, m_g_enc1[2], m_g_dec0[2], m_g_dec1, m_g_dec[4], m_g_fe, m_g_rh[2];
 
     // geometry
     int m_Hp = 0, m_Wp = 0, m_H8 = 0, m_W8 = 0, m_H16 = 0, m_W16 = 0, m_H16p = 0, m_W16p = 0;
@@ -96,7 +104,7 @@ private:
 
     // device buffers
     DevMem m_buf;
-    Scratch m_scratch;
+    Scratch m_scratch, m_scratch2;
     int* m_d_qp = nullptr;
     int* m_d_counts = nullptr;  // 4 step counts (encoder) / 1 (decoder)
     int* m_d_offs = nullptr;
@@ -104,7 +112,7 @@ private:
     f16 *m_y = nullptr, *m_y_pad = nullptr, *m_he_a = nullptr, *m_he_b = nullptr, *m_z = nullptr, *m_z_hat = nullptr;
     f16 *m_hd_a = nullptr, *m_hd_b = nullptr, *m_cat_pf = nullptr, *m_pf_a = nullptr, *m_common = nullptr;
     f16 *m_cat_sp = nullptr, *m_sp_a = nullptr, *m_sp_o = nullptr, *m_y_hat = nullptr;
-    f16 *m_rh_a = nullptr, *m_rh_head = nullptr;
+    f16 *m_rh_a = nullptr, *m_rh_head = nullptr, *m_rh_a2 = nullptr, *m_rh_head2 = nullptr;
     int8_t *m_z_i8 = nullptr, *m_yq = nullptr, *m_dec_c = nullptr;
     std::array<int16_t*, 4> m_sym{}, m_symc{};
     std::array<uint8_t*, 4> m_condq{};

@@ -79,13 +79,17 @@ void HTSWeights::load(const HostDict& d, float skip) {
 HTSProxy::HTSProxy() {
     hip_check(hipStreamCreateWithFlags(&m_stream, hipStreamNonBlocking), "stream");
     hip_check(hipStreamCreateWithFlags(&m_copy, hipStreamNonBlocking), "stream");
+    hip_check(hipStreamCreateWithFlags(&m_side, hipStreamNonBlocking), "stream");
     hip_check(hipEventCreateWithFlags(&m_ev_in, hipEventDisableTiming), "event");
     hip_check(hipEventCreateWithFlags(&m_ev_out, hipEventDisableTiming), "event");
     hip_check(hipEventCreateWithFlags(&m_ev_sym, hipEventDisableTiming), "event");
+    hip_check(hipEventCreateWithFlags(&m_ev_fork, hipEventDisableTiming), "event");
+    hip_check(hipEventCreateWithFlags(&m_ev_join, hipEventDisableTiming), "event");
 }
 
 HTSProxy::~HTSProxy() {
     if (m_stream) hipStreamSynchronize(m_stream);
+    if (m_side) hipStreamSynchronize(m_side);
     reset_graphs();
     hipHostFree(m_h_count);
     hipHostFree(m_h_sym);
@@ -96,6 +100,9 @@ HTSProxy::~HTSProxy() {
     hipEventDestroy(m_ev_in);
     hipEventDestroy(m_ev_out);
     hipEventDestroy(m_ev_sym);
+    hipEventDestroy(m_ev_fork);
+    hipEventDestroy(m_ev_join);
+    hipStreamDestroy(m_side);
     hipStreamDestroy(m_copy);
     hipStreamDestroy(m_stream);
 }
@@ -118,12 +125,17 @@ void HTSProxy::share_weights(const std::shared_ptr<HTSWeights>& w) {
 }
 
 void HTSProxy::reset_graphs() {
+    // a graph may still be running on the proxy stream (the encoder returns
+    // before its context update finishes)
+    if (m_stream) hipStreamSynchronize(m_stream);
+    if (m_side) hipStreamSynchronize(m_side);
     m_g_enc0.reset();
     for (auto& g : m_g_enc1) g.reset();
     for (auto& g : m_g_dec0) g.reset();
     m_g_dec1.reset();
     m_g_dec2.reset();
     m_g_dec3.reset();
+    for (auto& g : m_g_rh) g.reset();
 }
 
 void HTSProxy::prepare(int Hp, int Wp, f16* const frames[kFrames], int ld_out) {
@@ -157,6 +169,12 @@ void HTSProxy::prepare(int Hp, int Wp, f16* const frames[kFrames], int ld_out) {
     m_scratch.so = M.alloc_n<f16>(m_scratch.halves);
     m_scratch.x1 = M.alloc_n<f16>(m_scratch.halves);
     m_scratch.x2 = M.alloc_n<f16>(m_scratch.halves);
+    m_scratch2.halves = m_scratch.halves;
+    m_scratch2.t1 = M.alloc_n<f16>(m_scratch2.halves);
+    m_scratch2.t2 = M.alloc_n<f16>(m_scratch2.halves);
+    m_scratch2.so = M.alloc_n<f16>(m_scratch2.halves);
+    m_scratch2.x1 = M.alloc_n<f16>(m_scratch2.halves);
+    m_scratch2.x2 = M.alloc_n<f16>(m_scratch2.halves);
     m_d_qp = M.alloc_n<int>(1);
     m_d_count = M.alloc_n<int>(1);
     m_d_offs = M.alloc_n<int>(compact_blocks(m_ny) + 1);
@@ -182,6 +200,9 @@ void HTSProxy::prepare(int Hp, int Wp, f16* const frames[kFrames], int ld_out) {
     m_rh_common = M.alloc_n<f16>(P8 * 512);
     m_rh_a = M.alloc_n<f16>(P8 * 256);
     m_rh_head = M.alloc_n<f16>(P8 * 192);
+    m_rh_common2 = M.alloc_n<f16>(P8 * 512);
+    m_rh_a2 = M.alloc_n<f16>(P8 * 256);
+    m_rh_head2 = M.alloc_n<f16>(P8 * 192);
     m_z_i8 = M.alloc_n<int8_t>(m_nz);
     m_yq = M.alloc_n<int8_t>(m_ny);
     m_dec_c = M.alloc_n<int8_t>(m_ny);
@@ -324,27 +345,48 @@ void HTSProxy::decoder_and_recon(bool recon_all, bool reset_for_encoder) {
     const Act tmp = make_act(m_scratch.x1, 1, m_H8, m_W8, 512, 512);
     QuantSel qd{w.q_decoder, 512};
     w.dec.forward(e, decin, fp, tmp, &qd);
-    const Act common = make_act(m_rh_common, 1, m_H8, m_W8, 512, 512);
-    const Act ra = make_act(m_rh_a, 1, m_H8, m_W8, 256, 256);
-    const Act fi = make_act(m_feature_i, 1, m_H8, m_W8, 192, 192);
     if (recon_all) {
-        for (int i = 0; i < kFrames; ++i) {
-            if (i % 2 == 0) w.rh1[i / 2].forward(e, fp, common);
-            w.rh2[i][0].forward(e, common, ra);
-            w.rh2[i][1].forward(e, ra, ra);
-            w.rh2[i][2].forward(e, ra, ra);
-            const Act head = (i == kFrames - 1) ? fi : make_act(m_rh_head, 1, m_H8, m_W8, 192, 192);
-            w.rh_head[i].run(e, ra, head, EPI_BIAS);
-            hip_check(pixel_shuffle8_clamp(head.p, 192, 1, m_H8, m_W8, 3, m_frames[i], m_ld_out, m_stream), "shuffle");
-        }
+        recon_frames(e, 0, kFrames, m_rh_common, m_rh_a, m_rh_head);
     } else if (reset_for_encoder) {
         // ReconHead(for_reset=True): conv1[-1] -> conv2[-1] (video_model_ht.py:254-256)
+        const Act common = make_act(m_rh_common, 1, m_H8, m_W8, 512, 512);
+        const Act ra = make_act(m_rh_a, 1, m_H8, m_W8, 256, 256);
         w.rh1[3].forward(e, fp, common);
         w.rh2[7][0].forward(e, common, ra);
         w.rh2[7][1].forward(e, ra, ra);
         w.rh2[7][2].forward(e, ra, ra);
-        w.rh_head[7].run(e, ra, fi, EPI_BIAS);
+        w.rh_head[7].run(e, ra, make_act(m_feature_i, 1, m_H8, m_W8, 192, 192), EPI_BIAS);
     }
+}
+
+// ReconHead frames [first, last) (video_model_ht.py:258-265): conv1[i/2] once
+// per frame pair, conv2[i], pixel_shuffle(8) + clamp into the frame output;
+// frame 7's head output is also the reset feature.
+void HTSProxy::recon_frames(const Exec& e, int first, int last, f16* common_buf, f16* ra_buf, f16* head_buf) {
+    const HTSWeights& w = *m_w;
+    const Act fp = make_act(m_cat_fam + 512, 1, m_H8, m_W8, 512, 1024);
+    const Act common = make_act(common_buf, 1, m_H8, m_W8, 512, 512);
+    const Act ra = make_act(ra_buf, 1, m_H8, m_W8, 256, 256);
+    const Act fi = make_act(m_feature_i, 1, m_H8, m_W8, 192, 192);
+    for (int i = first; i < last; ++i) {
+        if (i == first || i % 2 == 0) w.rh1[i / 2].forward(e, fp, common);
+        w.rh2[i][0].forward(e, common, ra);
+        w.rh2[i][1].forward(e, ra, ra);
+        w.rh2[i][2].forward(e, ra, ra);
+        const Act head = (i == kFrames - 1) ? fi : make_act(head_buf, 1, m_H8, m_W8, 192, 192);
+        w.rh_head[i].run(e, ra, head, EPI_BIAS);
+        hip_check(pixel_shuffle8_clamp(head.p, 192, 1, m_H8, m_W8, 3, m_frames[i], m_ld_out, e.stream), "shuffle");
+    }
+}
+
+void HTSProxy::fork_side() {
+    hip_check(hipEventRecord(m_ev_fork, m_stream), "event record");
+    hip_check(hipStreamWaitEvent(m_side, m_ev_fork, 0), "stream wait");
+}
+
+void HTSProxy::join_side() {
+    hip_check(hipEventRecord(m_ev_join, m_side), "event record");
+    hip_check(hipStreamWaitEvent(m_stream, m_ev_join, 0), "stream wait");
 }
 
 // lambda_enc_1 (dmc_hts_proxy.cpp:571-574, 832-842)
@@ -376,8 +418,11 @@ int HTSProxy::compress(const f16* x, int ldx, int H, int W, int qp, bool reset, 
     hip_check(pad_unshuffle(x, ldx, 1, H, W, 24, 8, pad_b, pad_r, m_cat_enc, 2048, m_stream), "unshuffle");
     m_g_enc0.run(m_stream, m_use_graphs, [&] { enc_stage0(); });
     hip_check(hipEventRecord(m_ev_sym, m_stream), "event");
-    m_g_enc1[reset ? 1 : 0].run(m_stream, m_use_graphs, [&] { enc_stage1(reset); });
+    // the caller needs nothing from the context update (it only feeds this
+    // proxy's next call, ordered on the proxy stream): release the caller's
+    // stream now, so a decoder on it overlaps the update
     leave(s);
+    m_g_enc1[reset ? 1 : 0].run(m_stream, m_use_graphs, [&] { enc_stage1(reset); });
     // entropy coding on the host while the GPU runs the context update
     hip_check(hipStreamWaitEvent(m_copy, m_ev_sym, 0), "wait");
     hip_check(hipMemcpyAsync(m_h_count, m_d_count, sizeof(int), hipMemcpyDeviceToHost, m_copy), "d2h");
@@ -425,11 +470,12 @@ void HTSProxy::dec_stage1() {
     hip_check(compact(m_idx, 1, m_cond, m_ny, m_idxc, m_d_offs, m_d_count, m_stream), "compact");
 }
 
-// lambda_dec_2 (dmc_hts_proxy.cpp:660)
+// lambda_dec_2 (dmc_hts_proxy.cpp:660): on the side stream, concurrent with
+// the prior (it needs only the memory, and only the decoder reads its ctx)
 void HTSProxy::dec_stage2() {
-    Exec e = exec();
+    Exec e = exec_side();
     const Act mem = make_act(m_cat_fam, 1, m_H8, m_W8, 512, 1024);
-    const Act tmp = make_act(m_scratch.x1, 1, m_H8, m_W8, 512, 512);
+    const Act tmp = make_act(m_scratch2.x1, 1, m_H8, m_W8, 512, 512);
     m_w->fe.forward(e, mem, make_act(m_cat_enc + 1536, 1, m_H8, m_W8, 512, 2048), tmp);
 }
 
@@ -446,7 +492,7 @@ void HTSProxy::dec_stage3() {
         spatial_means(step);
         hip_check(hts_dec_step(g, m_yq, m_means, 256, m_common, 768, m_cat_sp, 512, step, m_stream), "restore");
     }
-    decoder_and_recon(true, false);
+    decoder_and_recon(false, false);
 }
 
 void HTSProxy::decompress(const uint8_t* bits, long nbytes, int qp, int height, int width, int ec_parallel,
@@ -461,6 +507,10 @@ void HTSProxy::decompress(const uint8_t* bits, long nbytes, int qp, int height, 
     enter(s);
     set_device_int(m_d_qp, qp, m_stream);
     m_g_dec0[m_memory_has_value ? 1 : 0].run(m_stream, m_use_graphs, [&] { dec_stage0(m_memory_has_value); });
+    // the context from the new memory, on the side stream during dec_1
+    fork_side();
+    m_g_dec2.run(m_side, m_use_graphs, [&] { dec_stage2(); });
+    hip_check(hipEventRecord(m_ev_join, m_side), "event record");
     // z decode on the host, concurrent with dec_0 on the GPU
     m_dec.set_parallel(ec_parallel);
     m_dec.set_stream(bits, nbytes);
@@ -468,7 +518,6 @@ void HTSProxy::decompress(const uint8_t* bits, long nbytes, int qp, int height, 
     hip_check(hipMemcpyAsync(m_z_i8, m_h_z, m_nz, hipMemcpyHostToDevice, m_stream), "h2d");
     m_g_dec1.run(m_stream, m_use_graphs, [&] { dec_stage1(); });
     hip_check(hipEventRecord(m_ev_sym, m_stream), "event");
-    m_g_dec2.run(m_stream, m_use_graphs, [&] { dec_stage2(); });
     hip_check(hipStreamWaitEvent(m_copy, m_ev_sym, 0), "wait");
     hip_check(hipMemcpyAsync(m_h_count, m_d_count, sizeof(int), hipMemcpyDeviceToHost, m_copy), "d2h");
     hip_check(hipStreamSynchronize(m_copy), "sync");
@@ -479,7 +528,14 @@ void HTSProxy::decompress(const uint8_t* bits, long nbytes, int qp, int height, 
     m_last_count = n;
     m_dec.decode_y(m_h_idx, n, m_h_dec);
     hip_check(hipMemcpyAsync(m_dec_c, m_h_dec, n, hipMemcpyHostToDevice, m_stream), "h2d");
+    hip_check(hipStreamWaitEvent(m_stream, m_ev_join, 0), "stream wait");  // ctx
     m_g_dec3.run(m_stream, m_use_graphs, [&] { dec_stage3(); });
+    // recon head: frames 4-7 on the side stream, 0-3 here
+    fork_side();
+    m_g_rh[1].run(m_side, m_use_graphs,
+                  [&] { recon_frames(exec_side(), kFrames / 2, kFrames, m_rh_common2, m_rh_a2, m_rh_head2); });
+    m_g_rh[0].run(m_stream, m_use_graphs, [&] { recon_frames(exec(), 0, kFrames / 2, m_rh_common, m_rh_a, m_rh_head); });
+    join_side();
     leave(s);
     // pinned staging buffers are reused by the next call: finish the uploads
     hip_check(hipStreamSynchronize(m_stream), "sync");

@@ -71,18 +71,25 @@ private:
     void dec_stage2();
     void dec_stage3();
     void hyper_and_prior();
+    // recon head frames [first, last) on `e`'s stream with the given buffers
+    void recon_frames(const Exec& e, int first, int last, f16* common, f16* ra, f16* head);
     void spatial_means(int step);
     void decoder_and_recon(bool recon_all, bool reset_for_encoder);
     Exec exec() { return Exec{m_stream, m_d_qp, &m_scratch, m_prec}; }
+    // the decoder's side stream: feature extraction during the prior, and
+    // half of the recon head (independent work, own scratch)
+    Exec exec_side() { return Exec{m_side, m_d_qp, &m_scratch2, m_prec}; }
+    void fork_side();
+    void join_side();
 
     std::shared_ptr<HTSWeights> m_w;
     RansEncoder m_enc;
     RansDecoder m_dec;
-    hipStream_t m_stream = nullptr, m_copy = nullptr;
-    hipEvent_t m_ev_in = nullptr, m_ev_out = nullptr, m_ev_sym = nullptr;
+    hipStream_t m_stream = nullptr, m_copy = nullptr, m_side = nullptr;
+    hipEvent_t m_ev_in = nullptr, m_ev_out = nullptr, m_ev_sym = nullptr, m_ev_fork = nullptr, m_ev_join = nullptr;
     bool m_use_graphs = true;
     int m_prec = PREC_F16;
-    Graph m_g_enc0, m_g_enc1[2], m_g_dec0[2], m_g_dec1, m_g_dec2, m_g_dec3;
+    Graph m_g_enc0, m_g_enc1[2], m_g_dec0[2], m_g_dec1, m_g_dec2, m_g_dec3, m_g_rh[2];
 
     // geometry
     int m_Hp = 0, m_Wp = 0, m_H8 = 0, m_W8 = 0, m_H16 = 0, m_W16 = 0, m_H16p = 0, m_W16p = 0;
@@ -92,7 +99,7 @@ private:
 
     // device buffers
     DevMem m_buf;
-    Scratch m_scratch;
+    Scratch m_scratch, m_scratch2;
     int* m_d_qp = nullptr;
     int* m_d_count = nullptr;
     int* m_d_offs = nullptr;
@@ -101,6 +108,7 @@ private:
     f16 *m_hd_a = nullptr, *m_hd_b = nullptr, *m_cat_pf = nullptr, *m_pf_a = nullptr, *m_common = nullptr;
     f16 *m_cat_sp = nullptr, *m_sp_a = nullptr, *m_means = nullptr;
     f16 *m_rh_common = nullptr, *m_rh_a = nullptr, *m_rh_head = nullptr;
+    f16 *m_rh_common2 = nullptr, *m_rh_a2 = nullptr, *m_rh_head2 = nullptr;
     int8_t *m_z_i8 = nullptr, *m_yq = nullptr, *m_dec_c = nullptr;
     int16_t *m_sym = nullptr, *m_symc = nullptr;
     uint8_t *m_cond = nullptr, *m_idx = nullptr, *m_idxc = nullptr;

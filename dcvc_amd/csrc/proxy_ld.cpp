@@ -110,6 +110,9 @@ void LDProxy::share_weights(const std::shared_ptr<LDWeights>& w) {
 }
 
 void LDProxy::reset_graphs() {
+    // a graph may still be running on the proxy stream (the encoder returns
+    // before its context update finishes)
+    if (m_stream) hipStreamSynchronize(m_stream);
     m_g_enc0.reset();
     for (auto& g : m_g_enc1) g.reset();
     for (auto& g : m_g_dec0) g.reset();
@@ -345,8 +348,11 @@ int LDProxy::compress(const f16* x, int ldx, int H, int W, int qp, bool reset, i
     hip_check(pad_unshuffle(x, ldx, 1, H, W, 3, 8, pad_b, pad_r, m_cat_dec + 64, 512, m_stream), "unshuffle");
     m_g_enc0.run(m_stream, m_use_graphs, [&] { enc_stage0(); });
     hip_check(hipEventRecord(m_ev_sym, m_stream), "event");
-    m_g_enc1[reset ? 1 : 0].run(m_stream, m_use_graphs, [&] { enc_stage1(reset); });
+    // the caller needs nothing from the context update (it only feeds this
+    // proxy's next call, ordered on the proxy stream): release the caller's
+    // stream now, so a decoder on it overlaps the update
     leave(s);
+    m_g_enc1[reset ? 1 : 0].run(m_stream, m_use_graphs, [&] { enc_stage1(reset); });
     // entropy coding on the host (worker, :807-820) while the GPU updates the context
     hip_check(hipStreamWaitEvent(m_copy, m_ev_sym, 0), "wait");
     hip_check(hipMemcpyAsync(m_h_count, m_d_count, sizeof(int), hipMemcpyDeviceToHost, m_copy), "d2h");
