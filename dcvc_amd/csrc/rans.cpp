@@ -24,13 +24,16 @@ struct Sink {
     inline void push(uint8_t b) { v->push_back(b); }
 };
 
-inline void enc_put(uint32_t& r, Sink& out, uint32_t start, uint32_t freq) {
+// One coding step, r' = (r / freq) * 2^16 + r % freq + start, with the
+// division done by the row's precomputed reciprocal (CdfTable::rcp).
+inline void enc_put(uint32_t& r, Sink& out, uint32_t freq, uint32_t rcp, int shift, uint32_t bias, uint32_t cmpl) {
     const uint32_t r_max = freq << kRenormShift;
     while (r >= r_max) {
         out.push(static_cast<uint8_t>(r));
         r >>= 8;
     }
-    r = ((r / freq) << kProbBits) + (r % freq) + start;
+    const uint32_t q = (uint32_t)(((uint64_t)r * rcp) >> 32) >> shift;
+    r = r + bias + q * cmpl;
 }
 
 inline void enc_put_bits(uint32_t& r, Sink& out, uint32_t val) {
@@ -42,8 +45,7 @@ inline void enc_put_bits(uint32_t& r, Sink& out, uint32_t val) {
     r = (r << kBypassBits) | val;
 }
 
-inline void enc_symbol(uint32_t& r, Sink& out, int32_t s, int32_t max_value, const uint32_t* start,
-                       const uint32_t* freq) {
+inline void enc_symbol(uint32_t& r, Sink& out, int32_t s, int32_t max_value, const CdfTable& tab, size_t row) {
     int32_t value = (s < 0 ? -s : s) * 2 - (s > 0 ? 1 : 0);
     if (value >= max_value) {
         const uint32_t raw = static_cast<uint32_t>(value - max_value);
@@ -62,7 +64,8 @@ inline void enc_symbol(uint32_t& r, Sink& out, int32_t s, int32_t max_value, con
         // LIFO: the decoder reads bins in the order they were listed
         for (int j = nb - 1; j >= 0; --j) enc_put_bits(r, out, bins[j]);
     }
-    enc_put(r, out, start[value], freq[value]);
+    const size_t e = row + (size_t)value;
+    enc_put(r, out, tab.freq[e], tab.rcp[e], tab.rcp_shift[e], tab.bias[e], tab.cmpl[e]);
 }
 
 struct Reader {
@@ -131,6 +134,28 @@ void build_table(CdfTable& t, const int32_t* cdfs, int n_cdf, int row_len, const
         for (int j = 0; j + 1 < row_len; ++j) {
             t.start[(size_t)i * row_len + j] = (uint32_t)c[j] & 0xFFFF;
             t.freq[(size_t)i * row_len + j] = (uint32_t)(c[j + 1] - c[j]) & 0xFFFF;
+        }
+    }
+    const size_t n = (size_t)n_cdf * row_len;
+    t.rcp.assign(n, 0);
+    t.bias.assign(n, 0);
+    t.cmpl.assign(n, 0);
+    t.rcp_shift.assign(n, 0);
+    for (size_t e = 0; e < n; ++e) {
+        const uint32_t f = t.freq[e];
+        if (f == 0) continue;  // never coded (padding or an unused row tail)
+        t.cmpl[e] = (1u << kProbBits) - f;
+        if (f == 1) {
+            // q = r - 1 (r >= 1): r' = r * 2^16 + start
+            t.rcp[e] = ~0u;
+            t.rcp_shift[e] = 0;
+            t.bias[e] = t.start[e] + (1u << kProbBits) - 1;
+        } else {
+            uint32_t sh = 0;
+            while (f > (1u << sh)) ++sh;
+            t.rcp[e] = (uint32_t)(((1ull << (sh + 31)) + f - 1) / f);
+            t.rcp_shift[e] = (uint8_t)(sh - 1);
+            t.bias[e] = t.start[e];
         }
     }
 }
@@ -306,14 +331,14 @@ void RansEncoder::flush() {
                         if (idx >= tab.n_cdf) throw std::invalid_argument("y cdf index out of range");
                         const int32_t s = static_cast<int8_t>(c >> 8);
                         const size_t row = (size_t)idx * tab.row_len;
-                        enc_symbol(r, out, s, tab.max_value[idx], &tab.start[row], &tab.freq[row]);
+                        enc_symbol(r, out, s, tab.max_value[idx], tab, row);
                     }
                 } else {
                     const CdfTable& tab = m_tab[0];
                     for (long k = t.begin + t.count - 1; k >= t.begin; --k) {
                         const int idx = (int)(k % t.ch) + t.cdf_offset;
                         const size_t row = (size_t)idx * tab.row_len;
-                        enc_symbol(r, out, t.z[k], tab.max_value[idx], &tab.start[row], &tab.freq[row]);
+                        enc_symbol(r, out, t.z[k], tab.max_value[idx], tab, row);
                     }
                 }
             }

@@ -53,6 +53,11 @@ struct CdfTable {
     std::vector<int32_t> max_value;
     std::vector<uint32_t> start;   // n_cdf x row_len (encoder symbol starts)
     std::vector<uint32_t> freq;    // n_cdf x row_len
+    // encoder division by freq as a multiply (Alverson's integer reciprocal):
+    // x / freq == (x * rcp >> 32) >> rcp_shift for x < 2^31, and the coding
+    // step x' = (x / f) * 2^16 + x % f + start == x + bias + (x / f) * cmpl
+    std::vector<uint32_t> rcp, bias, cmpl;
+    std::vector<uint8_t> rcp_shift;
     int row_len = 0;
     int n_cdf = 0;
     bool valid() const { return n_cdf > 0; }
