@@ -38,7 +38,7 @@ METRIC = "1080p encode+decode frames/s on 1 MI355X, BF16/FP8 vs FP16 plain; p99 
 PEAK_FP16_TFLOPS = 2500.0   # MI355X dense FP16/BF16 MFMA
 PEAK_HBM_GBS = 8000.0
 EPI_NAMES = {0: "bias", 1: "wsilu", 2: "shortcut", 3: "shortcut2", 4: "shortcut_quant", 5: "quant", 6: "quad"}
-KIND_NAMES = {0: "conv1x1", 1: "conv_s2", 2: "subpel", 3: "d3x3"}
+KIND_NAMES = {0: "conv1x1", 1: "conv_s2", 2: "subpel", 3: "d3x3", 4: "ffn"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 

@@ -43,59 +43,11 @@
 //   [+ S2 | * q].
 #include "common.h"
 #include "gemm.h"
+#include "gemm_dev.h"
 
 namespace dcvc {
 
 namespace {
-
-constexpr int kBK = 32;                 // K tile (halves): one 64-byte LDS row
-constexpr int kRowBytes = kBK * 2;
-constexpr uint32_t kOOB = 0x80000000u;  // buffer offset beyond every range: reads zero
-constexpr int kRsrcWord3 = 0x00020000;  // gfx9 raw buffer: 32-bit data format
-
-DCVC_DEV __amdgpu_buffer_rsrc_t make_rsrc(const void* p, uint32_t bytes) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, kRsrcWord3);
-}
-
-typedef __attribute__((address_space(3))) void* lds_ptr_t;
-
-// 16 bytes per lane from buffer offset `off` straight into LDS at
-// dst + 16 * lane (dst wave-uniform).
-DCVC_DEV void dma16(__amdgpu_buffer_rsrc_t r, char* dst, uint32_t off) {
-    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (lds_ptr_t)dst, 16, off, 0, 0, 0);
-}
-
-// XOR swizzle of the 16-byte K chunk within a 64-byte LDS row.
-DCVC_DEV int swz(int row) { return ((row >> 2) ^ (row >> 1)) & 3; }
-
-// s_waitcnt vmcnt(VM) [lgkmcnt(0) if LGKM] (gfx9 encoding), then a workgroup
-// barrier.  The compiler fences keep LDS accesses on their side of the
-// barrier; no memory-model fence is used because it would also wait for the
-// K tiles still in flight.
-template <int VM, bool LGKM>
-DCVC_DEV void wait_tile_and_barrier() {
-    static_assert(VM >= 0 && VM < 64, "vmcnt range");
-    constexpr int imm = (VM & 15) | ((VM >> 4) << 14) | (7 << 4) | ((LGKM ? 0 : 15) << 8);
-    asm volatile("" ::: "memory");
-    __builtin_amdgcn_s_waitcnt(imm);
-    __builtin_amdgcn_s_barrier();
-    asm volatile("" ::: "memory");
-}
-
-// Scheduling pattern for one k16 step: NDS fragment reads, NMF MFMAs and NVM
-// DMA issues spread evenly, so the matrix pipe never waits behind a burst of
-// memory instructions.
-template <int NDS, int NMF, int NVM, int R = 0>
-DCVC_DEV void interleave() {
-    if constexpr (R < NDS) {
-        constexpr int vm = (NVM * (R + 1)) / NDS - (NVM * R) / NDS;
-        constexpr int mf = (NMF * (R + 1)) / NDS - (NMF * R) / NDS;
-        if constexpr (vm > 0) __builtin_amdgcn_sched_group_barrier(0x020, vm, 0);  // VMEM read
-        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);                        // DS read
-        if constexpr (mf > 0) __builtin_amdgcn_sched_group_barrier(0x008, mf, 0);  // MFMA
-        interleave<NDS, NMF, NVM, R + 1>();
-    }
-}
 
 int g_tile_policy = 0;  // 0 automatic, otherwise a forced configuration id
 
@@ -155,7 +107,9 @@ struct Tile {
     static constexpr bool kWaitLds = !kFp8 && (kAhead == STAGES_ - 1);
     // configurations meant to run 2 blocks per CU keep within 256 registers
     // per lane (architectural + accumulation) so two waves share each SIMD
-    static constexpr int kWavesPerSimd = ((STAGES_ == 3 && PREC_ != PREC_FP8) || (BM_ * BN_ <= 128 * 128)) ? 2 : 1;
+    // (and 8-wave configurations run two waves on every SIMD of their CU)
+    static constexpr int kWavesPerSimd =
+        (WM_ * WN_ >= 8 || (STAGES_ == 3 && PREC_ != PREC_FP8) || (BM_ * BN_ <= 128 * 128)) ? 2 : 1;
     static constexpr int EPI = EPI_, AMODE = AMODE_, DMODE = DMODE_;
     static constexpr int kWaves = WM * WN;
     static constexpr int kThreads = kWaves * 64;
@@ -579,9 +533,12 @@ __global__ __launch_bounds__(T::kThreads, T::kWavesPerSimd) void gemm_kernel(Gem
 //   3: 128 x 128, wave 64 x 64, 5 stages (2 blocks / CU)
 //   4: 256 x 128, wave 128 x 64, 3 stages (2 blocks / CU)
 //   5: 128 x 256, wave 64 x 128, 3 stages (2 blocks / CU)
+//   6: 256 x 256, 8 waves (4 x 2, wave 64 x 128), 4 stages (2 waves / SIMD)
+//   7: 256 x 256, 8 waves, 3 stages
 //   quad (B rows; 4 B rows per output): 1: 256 x 256 5 st, 2: 128 x 256 5 st,
-//   3: 128 x 128 waves 4 x 1, 4: 128 x 256 3 st, 5: 256 x 256 4 st
-constexpr int kPolicies = 5;
+//   3: 128 x 128 waves 4 x 1, 4: 128 x 256 3 st, 5: 256 x 256 4 st,
+//   6 / 7: 256 x 256 8 waves, 4 / 3 st
+constexpr int kPolicies = 7;
 
 template <int BM, int BN, int WM, int WN, int ST, int EPI, int AMODE, int DMODE, int PREC = PREC_F16>
 static hipError_t launch_tile(GemmArgs g, hipStream_t stream) {
@@ -602,7 +559,11 @@ static hipError_t launch_tile(GemmArgs g, hipStream_t stream) {
 // large-M layers run 2 blocks per CU (one block's epilogue overlaps the
 // other's K loop), except long-K layers (K >= 1024), which keep the 256 x 256
 // tile; the 1/16- and 1/64-resolution layers take 128 x 128 tiles so the grid
-// still covers the 256 CUs.
+// still covers the 256 CUs.  The 8-wave 256 x 256 tiles (6, 7) are 2-10 %
+// faster on the 1/8-resolution 512-channel and long-K shapes in isolation,
+// but inside the codecs, where two streams run concurrently, one block per CU
+// cannot share a CU with the other stream's blocks and the layers took
+// 5-17 % longer, so they are only used when forced.
 static int pick_tile(const GemmArgs& g, bool quad) {
     if (g_tile_policy >= 1 && g_tile_policy <= kPolicies) return g_tile_policy;
     const long long m256 = ceil_div(g.M, 256), m128 = ceil_div(g.M, 128);
@@ -637,6 +598,8 @@ static hipError_t dispatch_tiles(const GemmArgs& g, hipStream_t stream) {
         case 2: return launch_tile<128, 256, 2, 2, 5, EPI, AMODE, DMODE, PREC>(g, stream);
         case 4: return launch_tile<128, 256, 2, 2, 3, EPI, AMODE, DMODE, PREC>(g, stream);
         case 5: return launch_tile<256, 256, 2, 2, 4, EPI, AMODE, DMODE, PREC>(g, stream);
+        case 6: return launch_tile<256, 256, 4, 2, 4, EPI, AMODE, DMODE, PREC>(g, stream);
+        case 7: return launch_tile<256, 256, 4, 2, 3, EPI, AMODE, DMODE, PREC>(g, stream);
         default: return launch_tile<128, 128, 4, 1, 5, EPI, AMODE, DMODE, PREC>(g, stream);
         }
     } else {
@@ -645,6 +608,8 @@ static hipError_t dispatch_tiles(const GemmArgs& g, hipStream_t stream) {
         case 2: return launch_tile<256, 128, 2, 2, 5, EPI, AMODE, DMODE, PREC>(g, stream);
         case 4: return launch_tile<256, 128, 2, 2, 3, EPI, AMODE, DMODE, PREC>(g, stream);
         case 5: return launch_tile<128, 256, 2, 2, 3, EPI, AMODE, DMODE, PREC>(g, stream);
+        case 6: return launch_tile<256, 256, 4, 2, 4, EPI, AMODE, DMODE, PREC>(g, stream);
+        case 7: return launch_tile<256, 256, 4, 2, 3, EPI, AMODE, DMODE, PREC>(g, stream);
         default: return launch_tile<128, 128, 2, 2, 5, EPI, AMODE, DMODE, PREC>(g, stream);
         }
     }

@@ -4,6 +4,7 @@
 #include "common.h"
 #include "dwconv.h"
 #include "elementwise.h"
+#include "ffn.h"
 #include "gemm.h"
 
 using namespace dcvc;
@@ -118,6 +119,34 @@ int dcvc_conv1x1_bias_wsilu_chunk_add(const void* A, int lda, int M, int K, cons
     GemmArgs g = plain_args(A, lda, M, K, W, N, bias, D, ldd);
     return run_gemm(g, EPI_QUAD, A_PLAIN, D_PLAIN, stream);
 }
+
+int dcvc_depth_conv_ffn(const void* A, int lda, int M, int C, int CM, const void* W0, const float* b0,
+                        const void* W2, const float* b2, int epi, const void* S2, int lds2, const void* q,
+                        int qstride, const int* qp, int q_round, void* D, int ldd, void* stream) {
+    FfnArgs f;
+    f.A = H(A);
+    f.lda = lda;
+    f.M = M;
+    f.C = C;
+    f.CM = CM;
+    f.W0 = H(W0);
+    f.b0 = b0;
+    f.W2 = H(W2);
+    f.b2 = b2;
+    f.S2 = H(S2);
+    f.lds2 = lds2;
+    f.q = H(q);
+    f.qstride = qstride;
+    f.qp = qp;
+    f.q_round = q_round;
+    f.D = Hm(D);
+    f.ldd = ldd;
+    const int v = ffn_validate(f, epi);
+    if (v != DCVC_OK) return v;
+    return st(ffn_launch(f, epi, S(stream)));
+}
+
+int dcvc_set_ffn_fusion(int mode) { return ffn_set_mode(mode); }
 
 int dcvc_weights_to_lowp(const void* W, int rows, int K, int prec, void* Wq, float* scale, void* stream) {
     if (!W || !Wq || rows <= 0 || K <= 0 || K % 2 != 0) return DCVC_ERR_ARG;

@@ -12,6 +12,7 @@
 #include "common.h"
 #include "dwconv.h"
 #include "elementwise.h"
+#include "ffn.h"
 #include "gemm.h"
 
 namespace dcvc {
@@ -227,8 +228,9 @@ void set_device_int(int* dst, int v, hipStream_t s);
 // layer classes while tracing is on (used to build the per-chunk layer table).
 // A timed trace brackets each launch with HIP events on its stream.
 struct TraceRecord {
-    int kind;   // 0 1x1 GEMM, 1 stride-2 conv, 2 sub-pixel conv, 3 depthwise 3x3
-    int epi;    // GemmEpi for kinds 0-2
+    int kind;   // 0 1x1 GEMM, 1 stride-2 conv, 2 sub-pixel conv, 3 depthwise 3x3,
+                // 4 fused FFN (N = block channels, K = hidden channels)
+    int epi;    // GemmEpi for kinds 0-2 and 4 (the tail)
     int M, N, K;            // N = output channels (quad: 4N weight rows)
     long long flops;        // algorithmic multiply-adds x 2
     long long bytes;        // algorithmic HBM bytes: inputs + weights + outputs once

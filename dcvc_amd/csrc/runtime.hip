@@ -208,10 +208,49 @@ void DCB::forward(const Exec& e, const Act& in, const Act& out, const QuantSel* 
     dw.run(e, t1, t2);
     const Act o = shortcut ? make_act(sc.so, in.n_img, in.H, in.W, cout, cout) : out;
     dc3.run(e, t2, o, EPI_SHORTCUT, &x);
+    if (shortcut && q) throw std::logic_error("DCB: quant tail with block shortcut is not used by the models");
+    if (ffn_use((int)in.pixels(), cout, cmid, e.prec)) {
+        // ffn.0 and ffn.2 in one launch, the hidden activation kept on chip
+        FfnArgs f;
+        f.A = o.p;
+        f.lda = o.ld;
+        f.M = (int)in.pixels();
+        f.C = cout;
+        f.CM = cmid;
+        f.W0 = ffn0.w;
+        f.b0 = ffn0.b;
+        f.W2 = ffn2.w;
+        f.b2 = ffn2.b;
+        f.D = out.p;
+        f.ldd = out.ld;
+        int epi = EPI_SHORTCUT;
+        if (shortcut) {
+            epi = EPI_SHORTCUT2;
+            f.S2 = x.p;
+            f.lds2 = x.ld;
+        } else if (q) {
+            epi = EPI_SHORTCUT_QUANT;
+            f.q = q->table;
+            f.qstride = q->stride;
+            f.qp = e.d_qp;
+            f.q_round = q->round_first ? 1 : 0;
+        }
+        const int v = ffn_validate(f, epi);
+        if (v != DCVC_OK) throw StatusError(v, "DCB: invalid fused FFN arguments");
+        if (trace_enabled()) {
+            const long long M = f.M;
+            const long long wb = 5LL * cmid * cout;  // ffn.0 (4 CM x C) + ffn.2 (C x CM)
+            trace_record(TraceRecord{4, epi, f.M, cout, cmid, 2LL * M * wb,
+                                     2LL * (M * cout * (shortcut ? 3 : 2) + wb), -1},
+                         e.stream);
+        }
+        hip_check(ffn_launch(f, epi, e.stream), "fused ffn launch");
+        trace_launched(e.stream);
+        return;
+    }
     ffn0.run(e, o, t1, EPI_QUAD);
     if (shortcut) {
         ffn2.run(e, t1, out, EPI_SHORTCUT2, &o, &x);
-        if (q) throw std::logic_error("DCB: quant tail with block shortcut is not used by the models");
     } else if (q) {
         ffn2.run(e, t1, out, EPI_SHORTCUT_QUANT, &o, nullptr, q);
     } else {

@@ -131,6 +131,27 @@ def conv1x1(a, w, bias, epi="bias", s1=None, s2=None, q=None, qp=None, out=None)
     return out
 
 
+FFN_TAILS = {"shortcut": 2, "shortcut2": 3, "shortcut_quant": 4}
+
+
+def depth_conv_ffn(a, w0, b0, w2, b2, tail="shortcut", s2=None, q=None, qp=None, q_round=False, out=None):
+    """The ffn half of DepthConvBlock in one launch: a [M,C] fp16, w0 [4CM,C]
+    (ffn.0, reference channel order), w2 [C,CM] (ffn.2), fp32 biases.
+    out = tail(quad(a w0^T + b0) w2^T + b2 + a); out may be a or s2."""
+    _check(a)
+    M, C = a.shape
+    CM = w2.shape[1]
+    if out is None:
+        out = torch.empty((M, C), dtype=torch.float16, device=a.device)
+    w0, w2 = w0.contiguous(), w2.contiguous()
+    b0, b2 = b0.float().contiguous(), b2.float().contiguous()
+    qs = 0 if q is None else (q.stride(0) if q.dim() == 2 else q.numel())
+    _lib_call("dcvc_depth_conv_ffn", _p(a), _ld(a), M, C, CM, _p(w0), _p(b0), _p(w2), _p(b2), FFN_TAILS[tail],
+              _p(s2), _ld(s2) if s2 is not None else 0, _p(q), qs, _p(qp), 1 if q_round else 0, _p(out),
+              _ld(out), _stream())
+    return out
+
+
 def conv_bias(x, n_img, H, W, wt_ohwi, bias, kernel, stride, out=None):
     """x [n_img*H*W, Cin]; wt_ohwi [N, k*k*Cin] fp16 tap-major."""
     Cin = wt_ohwi.shape[1] // (kernel * kernel)

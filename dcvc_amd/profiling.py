@@ -22,8 +22,15 @@ EPI_NAMES = {0: "bias", 1: "wsilu", 2: "shortcut", 3: "shortcut2", 4: "shortcut_
 _TILE = re.compile(r"Tile<(\d+), (\d+), (\d+), (\d+), (\d+), (\d+), (\d+), (\d+)>")
 
 
+# FfnCfg<C, CM, EPI>: the fused DepthConvBlock FFN
+_FFN = re.compile(r"FfnCfg<(\d+), (\d+), (\d+)>")
+
+
 def gemm_class(kernel_name: str) -> str | None:
     """Kernel class of a GEMM instantiation, named as in the launch trace."""
+    f = _FFN.search(kernel_name)
+    if f:
+        return f"ffn_{EPI_NAMES[int(f.group(3))]}"
     m = _TILE.search(kernel_name)
     if not m:
         return None
@@ -36,6 +43,9 @@ def short_name(kernel_name: str) -> str:
     m = _TILE.search(kernel_name)
     if m:
         return f"gemm_kernel<{','.join(m.groups())}> [{gemm_class(kernel_name)}]"
+    f = _FFN.search(kernel_name)
+    if f:
+        return f"ffn_kernel<{','.join(f.groups())}> [{gemm_class(kernel_name)}]"
     return kernel_name.split("(")[0][:90]
 
 

@@ -80,6 +80,26 @@ int dcvc_conv1x1_bias_with_quant(const void* A, int lda, int M, int K, const voi
  * reference ffn.0 order; D[m, n] = sum_{j<4} wsilu(A W[4n+j]^T + b[4n+j]). */
 int dcvc_conv1x1_bias_wsilu_chunk_add(const void* A, int lda, int M, int K, const void* W, int N,
                                       const float* bias, void* D, int ldd, void* stream);
+/* The ffn half of DepthConvBlock, which the reference's proxy runs as
+ * conv1x1_bias_wsilu_chunk_add followed by conv1x1_bias_shortcut /
+ * _shortcut2 / _shortcut_with_quant (layers_proxy.cpp:84-97), in one launch:
+ *   T = quad(A W0^T + b0)            (W0 [4 CM x C], b0 [4 CM], as above)
+ *   D = tail(T W2^T + b2 + A)        (W2 [C x CM], b2 [C])
+ * tail by epi: 2 nothing, 3 + S2, 4 * q[n] (after rounding to fp16 first
+ * when q_round is nonzero, the HT-L order).  A is [M x C] (row stride lda);
+ * D may be A or S2 itself with the same row stride.  Supported: CM = 256,
+ * C in {256, 512}.  Bit-identical to the two entry points run in sequence. */
+int dcvc_depth_conv_ffn(const void* A, int lda, int M, int C, int CM, const void* W0, const float* b0,
+                        const void* W2, const float* b2, int epi, const void* S2, int lds2, const void* q,
+                        int qstride, const int* qp, int q_round, void* D, int ldd, void* stream);
+/* DepthConvBlock's use of dcvc_depth_conv_ffn inside the codecs (no
+ * reference counterpart): 0 never (default: with the decoder's two
+ * concurrent streams the one-workgroup-per-CU fused launch is slower than
+ * the two launches); 1 for fp16 blocks with C = 256 whose grid covers at
+ * least 192 of the 256 CUs (faster when the GPU runs nothing else); 2
+ * whenever supported.  Graphs captured before a change keep their
+ * launches. */
+int dcvc_set_ffn_fusion(int mode);
 
 /* Low-precision 1x1-conv product (precision form; no reference counterpart).
  * dcvc_weights_to_lowp converts fp16 weights [rows x K] on the device:

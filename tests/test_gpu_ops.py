@@ -171,7 +171,7 @@ def test_gemm_tile_configs_bit_identical(gpu, epi):
 
     outs = []
     try:
-        for policy in (1, 2, 3, 4, 5):
+        for policy in (1, 2, 3, 4, 5, 6, 7):
             assert lib.dcvc_gemm_set_tile_policy(policy) == 0
             r0 = rng(5)
             outs.append(run())
