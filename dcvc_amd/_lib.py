@@ -31,6 +31,7 @@ SIGNATURES = {
     "dcvc_conv1x1_bias_wsilu_chunk_add": [P, I, I, I, P, I, P, P, I, P],
     "dcvc_depth_conv_ffn": [P, I, I, I, I, P, P, P, P, I, P, I, P, I, P, I, P, I, P],
     "dcvc_set_ffn_fusion": [I],
+    "dcvc_conv1x1_grouped": [P, I, L, I, I, I, P, L, I, P, L, I, P, I, L, P, I, L, P, I, L, I, P],
     "dcvc_conv_bias": [P, I, I, I, I, I, P, I, P, I, I, P, I, P],
     "dcvc_weights_to_lowp": [P, I, I, I, P, P, P],
     "dcvc_gemm_lowp": [P, I, I, I, P, I, P, I, I, P, P, I, P, I, P, I, P, P, I, P],

@@ -59,6 +59,12 @@ struct GemmArgs {
     int ldd = 0;
     int prec = PREC_F16;             // GemmPrec; B is bf16 / fp8 bytes accordingly
     const float* wscale = nullptr;   // PREC_FP8: per B row dequantisation scale
+    // `batch` independent problems of this shape in one launch (A_PLAIN /
+    // D_PLAIN only): problem b reads A + (b / a_div) * sA (a_div > 1 shares
+    // one input among consecutive problems), B at b * sB weight elements,
+    // bias / wscale at b * sBias, S1 / S2 / D at b * their stride (elements)
+    int batch = 1, a_div = 1;
+    long sA = 0, sB = 0, sBias = 0, sS1 = 0, sS2 = 0, sD = 0;
     // filled by gemm_launch: byte extents of A and B for the buffer range checks
     uint32_t a_bytes = 0, b_bytes = 0;
 };

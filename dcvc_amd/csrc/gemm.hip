@@ -142,6 +142,17 @@ struct Tile {
 template <class T>
 __global__ __launch_bounds__(T::kThreads, T::kWavesPerSimd) void gemm_kernel(GemmArgs g, int tiles_n) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
+    if (g.batch > 1) {
+        // problem blockIdx.y of a batched launch (uniform over the block)
+        const long pb = (long)blockIdx.y;
+        g.A += (pb / g.a_div) * g.sA;
+        g.B = reinterpret_cast<const f16*>(reinterpret_cast<const char*>(g.B) + pb * g.sB * (T::kFp8 ? 1 : 2));
+        g.bias += pb * g.sBias;
+        if constexpr (T::kFp8) g.wscale += pb * g.sBias;
+        if (g.S1) g.S1 += pb * g.sS1;
+        if (g.S2) g.S2 += pb * g.sS2;
+        g.D += pb * g.sD;
+    }
     constexpr int BM = T::BM, BN = T::BN, SM = T::SM, SN = T::SN, NT = T::kThreads;
     constexpr int AMODE = T::AMODE, EPI = T::EPI;
     const int tid = threadIdx.x;
@@ -551,7 +562,7 @@ static hipError_t launch_tile(GemmArgs g, hipStream_t stream) {
     static const hipError_t attr = hipFuncSetAttribute((const void*)kern,
                                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)T::kLdsBytes);
     if (attr != hipSuccess) return attr;
-    hipLaunchKernelGGL(kern, dim3(tiles_m * tiles_n), dim3(T::kThreads), T::kLdsBytes, stream, g, tiles_n);
+    hipLaunchKernelGGL(kern, dim3(tiles_m * tiles_n, g.batch), dim3(T::kThreads), T::kLdsBytes, stream, g, tiles_n);
     return hipGetLastError();
 }
 
@@ -704,6 +715,13 @@ int gemm_validate(const GemmArgs& g, int epi, int amode, int dmode) {
     }
     if (epi == EPI_SHORTCUT2 && (!g.S2 || g.lds2 < g.N || g.lds2 % 8 != 0 || !aligned16(g.S2))) return DCVC_ERR_ARG;
     if ((epi == EPI_SHORTCUT_QUANT || epi == EPI_QUANT) && !g.q) return DCVC_ERR_ARG;
+    if (g.batch != 1) {
+        // every problem's operands keep the 16-byte alignment of the first
+        if (g.batch < 1 || g.batch > 65535 || g.a_div < 1 || amode != A_PLAIN || dmode != D_PLAIN) return DCVC_ERR_ARG;
+        if (g.sA < 0 || g.sB < 0 || g.sBias < 0 || g.sS1 < 0 || g.sS2 < 0 || g.sD < 0) return DCVC_ERR_ARG;
+        const long wb = (g.prec == PREC_FP8) ? 1 : 2;
+        if (g.sA % 8 || g.sD % 8 || g.sS1 % 8 || g.sS2 % 8 || (g.sB * wb) % 16) return DCVC_ERR_ARG;
+    }
     return DCVC_OK;
 }
 

@@ -148,6 +148,27 @@ int dcvc_depth_conv_ffn(const void* A, int lda, int M, int C, int CM, const void
 
 int dcvc_set_ffn_fusion(int mode) { return ffn_set_mode(mode); }
 
+int dcvc_conv1x1_grouped(const void* A, int lda, long sA, int a_div, int M, int K, const void* W, long sW, int N,
+                         const float* bias, long sBias, int epi, const void* S1, int lds1, long sS1, const void* S2,
+                         int lds2, long sS2, void* D, int ldd, long sD, int batch, void* stream) {
+    if (epi != EPI_BIAS && epi != EPI_WSILU && epi != EPI_SHORTCUT && epi != EPI_SHORTCUT2 && epi != EPI_QUAD)
+        return DCVC_ERR_ARG;
+    GemmArgs g = plain_args(A, lda, M, K, W, N, bias, D, ldd);
+    g.S1 = H(S1);
+    g.lds1 = lds1;
+    g.S2 = H(S2);
+    g.lds2 = lds2;
+    g.batch = batch;
+    g.a_div = a_div;
+    g.sA = sA;
+    g.sB = sW;
+    g.sBias = sBias;
+    g.sS1 = sS1;
+    g.sS2 = sS2;
+    g.sD = sD;
+    return run_gemm(g, epi, A_PLAIN, D_PLAIN, stream);
+}
+
 int dcvc_weights_to_lowp(const void* W, int rows, int K, int prec, void* Wq, float* scale, void* stream) {
     if (!W || !Wq || rows <= 0 || K <= 0 || K % 2 != 0) return DCVC_ERR_ARG;
     if (prec == PREC_BF16) return st(weights_to_bf16(H(W), (long)rows * K, Wq, S(stream)));

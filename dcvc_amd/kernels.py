@@ -131,6 +131,30 @@ def conv1x1(a, w, bias, epi="bias", s1=None, s2=None, q=None, qp=None, out=None)
     return out
 
 
+GROUPED_EPIS = {"bias": 0, "wsilu": 1, "shortcut": 2, "shortcut2": 3, "quad": 6}
+
+
+def conv1x1_grouped(a, w, bias, epi="bias", s1=None, s2=None, a_div=1, out=None):
+    """batch problems in one launch: a [batch // a_div, M, K] (problem b reads
+    a[b // a_div]), w [batch, rows, K], bias [batch, rows] (rows = 4N for
+    'quad'), s1 / s2 / out [batch, M, N]."""
+    nb = w.shape[0]
+    M, K = a.shape[1], a.shape[2]
+    rows = w.shape[1]
+    N = rows // 4 if epi == "quad" else rows
+    if a.shape[0] * a_div != nb:
+        raise ValueError("a must hold batch / a_div problems")
+    if out is None:
+        out = torch.empty((nb, M, N), dtype=torch.float16, device=a.device)
+    w, bias = w.contiguous(), bias.float().contiguous()
+    for t in (a, out) + tuple(x for x in (s1, s2) if x is not None):
+        assert t.is_contiguous(), "expected contiguous [batch, M, C] tensors"
+    _lib_call("dcvc_conv1x1_grouped", _p(a), K, M * K, a_div, M, K, _p(w), rows * K, N, _p(bias), rows,
+              GROUPED_EPIS[epi], _p(s1), N, M * N if s1 is not None else 0, _p(s2), N,
+              M * N if s2 is not None else 0, _p(out), N, M * N, nb, _stream())
+    return out
+
+
 FFN_TAILS = {"shortcut": 2, "shortcut2": 3, "shortcut_quant": 4}
 
 

@@ -101,6 +101,18 @@ int dcvc_depth_conv_ffn(const void* A, int lda, int M, int C, int CM, const void
  * launches. */
 int dcvc_set_ffn_fusion(int mode);
 
+/* Grouped 1x1-conv product (no reference counterpart: the reference's recon
+ * heads launch one conv per frame branch): `batch` problems of one shape in
+ * one launch.  Problem b reads A + (b / a_div) * sA (a_div > 1: consecutive
+ * problems share one input), W + b * sW, bias + b * sBias, S1 / S2 / D + b *
+ * their stride (all strides in elements, multiples of 8 except sBias).  epi:
+ * 0 bias, 1 wsilu, 2 shortcut, 3 shortcut2, 6 quad (W rows 4N per problem,
+ * sW = 4 N K, sBias = 4 N).  Each problem's result equals the single-problem
+ * entry point's bit for bit. */
+int dcvc_conv1x1_grouped(const void* A, int lda, long sA, int a_div, int M, int K, const void* W, long sW, int N,
+                         const float* bias, long sBias, int epi, const void* S1, int lds1, long sS1, const void* S2,
+                         int lds2, long sS2, void* D, int ldd, long sD, int batch, void* stream);
+
 /* Low-precision 1x1-conv product (precision form; no reference counterpart).
  * dcvc_weights_to_lowp converts fp16 weights [rows x K] on the device:
  * prec 1 -> bf16 (Wq: rows*K*2 bytes, scale unused), prec 2 -> fp8 e4m3

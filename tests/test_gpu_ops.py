@@ -552,3 +552,24 @@ def test_conditional_index_and_recover(gpu, N, p):
     rec = kernels.conditional_recover(torch.from_numpy(dec).to(gpu), torch.from_numpy(cond).to(gpu), offs8)
     torch.cuda.synchronize()
     assert np.array_equal(rec.cpu().numpy(), O.recover(dec[:n], cond, N))
+
+
+@pytest.mark.parametrize("epi,a_div", [("bias", 1), ("wsilu", 2), ("shortcut", 1), ("shortcut2", 8), ("quad", 1),
+                                       ("quad", 4)])
+def test_conv1x1_grouped_equals_per_problem(gpu, epi, a_div):
+    """Grouped launches (the recon heads' per-frame branches) give every
+    problem the bits of its own single-problem launch, with shared inputs."""
+    from dcvc_amd import kernels
+    r = rng(31 + a_div)
+    nb, M, K, N = 8, 3001, 256, 192
+    rows = 4 * N if epi == "quad" else N
+    a = f16(r.standard_normal((nb // a_div, M, K)), gpu)
+    w = f16(r.standard_normal((nb, rows, K)) / 16, gpu)
+    b = torch.from_numpy((r.standard_normal((nb, rows)) * 0.1).astype(np.float32)).to(gpu)
+    s1 = f16(r.standard_normal((nb, M, N)), gpu)
+    s2 = f16(r.standard_normal((nb, M, N)), gpu)
+    got = kernels.conv1x1_grouped(a, w, b, epi=epi, s1=s1, s2=s2, a_div=a_div)
+    for i in range(nb):
+        ref = kernels.conv1x1(a[i // a_div], w[i], b[i], epi=epi, s1=s1[i], s2=s2[i])
+        torch.cuda.synchronize()
+        assert torch.equal(got[i].view(torch.int16), ref.view(torch.int16)), f"problem {i}"
