@@ -37,15 +37,19 @@ class RansEncoder:
     """Same methods as MLCodec_extensions_cpp.RansEncoder."""
 
     def __init__(self):
-        self._h = _lib.load().dcvc_rans_encoder_create()
+        # the library handle is kept on the instance: at interpreter exit the
+        # module globals may already be gone when __del__ runs
+        self._lib = _lib.load()
+        self._h = self._lib.dcvc_rans_encoder_create()
         if not self._h:
             raise _lib.DcvcError("dcvc_rans_encoder_create", 2)
         self._keep = []
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
-            _lib.load().dcvc_rans_encoder_destroy(h)
+        lib = getattr(self, "_lib", None)
+        if h and lib is not None:
+            lib.dcvc_rans_encoder_destroy(h)
             self._h = None
 
     def set_cdf(self, cdfs, cdf_sizes, index):
@@ -87,15 +91,17 @@ class RansDecoder:
     """Same methods as MLCodec_extensions_cpp.RansDecoder (decode returns the array)."""
 
     def __init__(self):
-        self._h = _lib.load().dcvc_rans_decoder_create()
+        self._lib = _lib.load()
+        self._h = self._lib.dcvc_rans_decoder_create()
         if not self._h:
             raise _lib.DcvcError("dcvc_rans_decoder_create", 2)
         self._last = np.zeros(0, np.int8)
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
-            _lib.load().dcvc_rans_decoder_destroy(h)
+        lib = getattr(self, "_lib", None)
+        if h and lib is not None:
+            lib.dcvc_rans_decoder_destroy(h)
             self._h = None
 
     def set_cdf(self, cdfs, cdf_sizes, index):
