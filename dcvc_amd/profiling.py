@@ -18,8 +18,8 @@ import sys
 from collections import defaultdict
 
 EPI_NAMES = {0: "bias", 1: "wsilu", 2: "shortcut", 3: "shortcut2", 4: "shortcut_quant", 5: "quant", 6: "quad"}
-# Tile<BM, BN, WM, WN, STAGES, EPI, AMODE, DMODE>
-_TILE = re.compile(r"Tile<(\d+), (\d+), (\d+), (\d+), (\d+), (\d+), (\d+), (\d+)>")
+# Tile<BM, BN, WM, WN, STAGES, EPI, AMODE, DMODE[, PREC]>
+_TILE = re.compile(r"Tile<(\d+), (\d+), (\d+), (\d+), (\d+), (\d+), (\d+), (\d+)(?:, (\d+))?>")
 
 
 # FfnCfg<C, CM, EPI>: the fused DepthConvBlock FFN
@@ -42,7 +42,7 @@ def gemm_class(kernel_name: str) -> str | None:
 def short_name(kernel_name: str) -> str:
     m = _TILE.search(kernel_name)
     if m:
-        return f"gemm_kernel<{','.join(m.groups())}> [{gemm_class(kernel_name)}]"
+        return f"gemm_kernel<{','.join(x for x in m.groups() if x is not None)}> [{gemm_class(kernel_name)}]"
     f = _FFN.search(kernel_name)
     if f:
         return f"ffn_kernel<{','.join(f.groups())}> [{gemm_class(kernel_name)}]"
