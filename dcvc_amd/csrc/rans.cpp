@@ -68,33 +68,46 @@ inline void enc_symbol(uint32_t& r, Sink& out, int32_t s, int32_t max_value, con
     enc_put(r, out, tab.freq[e], tab.rcp[e], tab.rcp_shift[e], tab.bias[e], tab.cmpl[e]);
 }
 
+// Sub-stream reader.  p[n] is a zero byte: reads past the end return 0
+// without a branch, and pos keeps counting, so pos > n marks a truncated or
+// corrupted stream.
 struct Reader {
     const uint8_t* p;
     size_t n;
     size_t pos;
-    bool overrun;
+    inline uint32_t peek() const { return p[pos < n ? pos : n]; }
     inline uint32_t next() {
-        if (pos < n) return p[pos++];
-        overrun = true;
-        return 0;
+        const uint32_t v = peek();
+        ++pos;
+        return v;
     }
+    bool overrun() const { return pos > n; }
 };
 
-inline int32_t dec_symbol(uint32_t& r, Reader& in, const int32_t* cdf, int row_len, int32_t max_value) {
+// One decoding step (py_rans.cpp:394-442 semantics).  The symbol search
+// starts at the row's bucket table entry; the state renormalisation takes at
+// most two bytes (the state stays >= 2^23 and loses at most 16 bits per
+// symbol), done without branches.
+inline int32_t dec_symbol(uint32_t& r, Reader& in, const int32_t* cdf, const uint8_t* lut, int32_t max_value,
+                          bool& bad) {
     const uint32_t cum = r & kProbMask;
-    int s = 1;
-    while (s < row_len - 1 && (uint32_t)cdf[s] <= cum) ++s;
-    --s;
+    int s = lut[cum >> 8];
+    while (s < max_value && (uint32_t)cdf[s + 1] <= cum) ++s;
     const uint32_t start = (uint32_t)cdf[s];
     const uint32_t freq = (uint32_t)cdf[s + 1] - start;
-    if (freq == 0 || freq > (1u << kProbBits)) {
-        in.overrun = true;  // corrupted table row or state
+    if (freq == 0 || freq > (1u << kProbBits) || start > cum || cum - start >= freq) {
+        bad = true;  // corrupted table row or state
         return 0;
     }
-    r = freq * (r >> kProbBits) + (r & kProbMask) - start;
-    while (r < kLow) r = (r << 8) | in.next();
+    r = freq * (r >> kProbBits) + cum - start;
+    const uint32_t b1 = r < kLow;
+    r = b1 ? (r << 8) | in.peek() : r;
+    in.pos += b1;
+    const uint32_t b2 = r < kLow;
+    r = b2 ? (r << 8) | in.peek() : r;
+    in.pos += b2;
     int32_t value = s;
-    if (value == max_value) {
+    if (__builtin_expect(value == max_value, 0)) {
         auto get_bits = [&]() {
             const uint32_t v = r & kBypassMax;
             r >>= kBypassBits;
@@ -107,7 +120,7 @@ inline int32_t dec_symbol(uint32_t& r, Reader& in, const int32_t* cdf, int row_l
             v = get_bits();
             groups += (int)v;
             if (groups > kMaxBypassGroups) {
-                in.overrun = true;
+                bad = true;
                 return 0;
             }
         }
@@ -127,13 +140,25 @@ void build_table(CdfTable& t, const int32_t* cdfs, int n_cdf, int row_len, const
     t.start.assign((size_t)n_cdf * row_len, 0);
     t.freq.assign((size_t)n_cdf * row_len, 0);
     for (int i = 0; i < n_cdf; ++i) {
-        const int mv = sizes[i] - 2;  // cdf_length - 2 (py_rans.cpp:344)
-        if (mv < 0 || mv + 2 > row_len) throw std::invalid_argument("bad cdf length");
+        // cdf_length - 2, held as int8 (py_rans.cpp:266, 377): lengths of 258..385
+        // wrap to 0..127 as there; lengths of 130..257 wrap negative, which the
+        // reference would use as a table index, so they are rejected
+        const int mv = (int8_t)(sizes[i] - 2);
+        if (sizes[i] < 2 || mv < 0 || mv + 2 > row_len) throw std::invalid_argument("bad cdf length");
         t.max_value[i] = mv;
         const int32_t* c = cdfs + (size_t)i * row_len;
         for (int j = 0; j + 1 < row_len; ++j) {
             t.start[(size_t)i * row_len + j] = (uint32_t)c[j] & 0xFFFF;
             t.freq[(size_t)i * row_len + j] = (uint32_t)(c[j + 1] - c[j]) & 0xFFFF;
+        }
+    }
+    t.lut.assign((size_t)n_cdf * 256, 0);
+    for (int i = 0; i < n_cdf; ++i) {
+        const int32_t* c = cdfs + (size_t)i * row_len;
+        int s = 0;
+        for (int b = 0; b < 256; ++b) {
+            while (s < t.max_value[i] && (uint32_t)c[s + 1] <= (uint32_t)b << 8) ++s;
+            t.lut[(size_t)i * 256 + b] = (uint8_t)s;  // s <= max_value <= 127
         }
     }
     const size_t n = (size_t)n_cdf * row_len;
@@ -409,7 +434,9 @@ void RansDecoder::set_stream(const uint8_t* data, long size) {
     // py_rans.cpp:490-570
     const int n = m_parallel;
     auto init = [](Sub& s) {
-        if (s.bytes.size() < 4) throw std::runtime_error("truncated entropy-coded stream");
+        s.size = s.bytes.size();
+        s.bytes.push_back(0);  // read past the end: zero, and pos > size flags it
+        if (s.size < 4) throw std::runtime_error("truncated entropy-coded stream");
         s.pos = 4;
         s.overrun = false;
         s.state = (uint32_t)s.bytes[0] | ((uint32_t)s.bytes[1] << 8) | ((uint32_t)s.bytes[2] << 16) |
@@ -469,15 +496,17 @@ void RansDecoder::decode_z(long n, int cdf_offset, int ch, int8_t* out) {
         long begin, count;
         part_range(n, m_parallel, i, begin, count);
         Sub& s = m_sub[i];
-        Reader rd{s.bytes.data(), s.bytes.size(), s.pos, s.overrun};
+        Reader rd{s.bytes.data(), s.size, s.pos};
         uint32_t r = s.state;
-        for (long k = begin; k < begin + count; ++k) {
+        bool corrupt = false;
+        for (long k = begin; k < begin + count && !corrupt; ++k) {
             const int idx = (int)(k % ch) + cdf_offset;
-            out[k] = (int8_t)dec_symbol(r, rd, &tab.cdf[(size_t)idx * tab.row_len], tab.row_len, tab.max_value[idx]);
+            out[k] = (int8_t)dec_symbol(r, rd, &tab.cdf[(size_t)idx * tab.row_len], &tab.lut[(size_t)idx * 256],
+                                        tab.max_value[idx], corrupt);
         }
         s.state = r;
         s.pos = rd.pos;
-        s.overrun = rd.overrun;
+        s.overrun = s.overrun || corrupt || rd.overrun();
     });
     for (int i = 0; i < m_parallel; ++i) bad |= m_sub[i].overrun;
     if (bad) throw std::runtime_error("entropy-coded stream is truncated or corrupted");
@@ -491,19 +520,21 @@ void RansDecoder::decode_y(const uint8_t* indexes, long n, int8_t* out) {
         long begin, count;
         part_range(n, m_parallel, i, begin, count);
         Sub& s = m_sub[i];
-        Reader rd{s.bytes.data(), s.bytes.size(), s.pos, s.overrun};
+        Reader rd{s.bytes.data(), s.size, s.pos};
         uint32_t r = s.state;
-        for (long k = begin; k < begin + count; ++k) {
+        bool corrupt = false;
+        for (long k = begin; k < begin + count && !corrupt; ++k) {
             const int idx = indexes[k];
             if (idx >= tab.n_cdf) {
                 bad_idx[i] = 1;
                 break;
             }
-            out[k] = (int8_t)dec_symbol(r, rd, &tab.cdf[(size_t)idx * tab.row_len], tab.row_len, tab.max_value[idx]);
+            out[k] = (int8_t)dec_symbol(r, rd, &tab.cdf[(size_t)idx * tab.row_len], &tab.lut[(size_t)idx * 256],
+                                        tab.max_value[idx], corrupt);
         }
         s.state = r;
         s.pos = rd.pos;
-        s.overrun = rd.overrun;
+        s.overrun = s.overrun || corrupt || rd.overrun();
     });
     for (int i = 0; i < m_parallel; ++i) {
         if (bad_idx[i]) throw std::invalid_argument("y cdf index out of range");

@@ -58,6 +58,10 @@ struct CdfTable {
     // step x' = (x / f) * 2^16 + x % f + start == x + bias + (x / f) * cmpl
     std::vector<uint32_t> rcp, bias, cmpl;
     std::vector<uint8_t> rcp_shift;
+    // decoder search start: lut[row * 256 + b] is the symbol whose interval
+    // holds cum = 256 b, so the search for cum walks forward from
+    // lut[cum >> 8] (usually zero steps) instead of from symbol 0
+    std::vector<uint8_t> lut;
     int row_len = 0;
     int n_cdf = 0;
     bool valid() const { return n_cdf > 0; }
@@ -107,7 +111,8 @@ public:
 
 private:
     struct Sub {
-        std::vector<uint8_t> bytes;
+        std::vector<uint8_t> bytes;  // the sub-stream, then one zero byte
+        size_t size = 0;             // sub-stream bytes
         size_t pos = 0;
         uint32_t state = 0;
         bool overrun = false;

@@ -203,3 +203,75 @@ def test_compute_ec_parallel():
     assert entropy.compute_ec_parallel(0) == 1
     assert entropy.compute_ec_parallel(3 * 32768 + 5) == 3
     assert entropy.compute_ec_parallel(2088960) == 8
+
+
+def wide_table(rows=4, half=159):
+    """Laplace-shaped rows of 2 * half + 1 symbols, cdf_length 2 * half + 3:
+    the reference holds cdf_length - 2 as int8 (py_rans.cpp:266), so 321
+    wraps to a maximum symbol of 63 and larger values take the escape path."""
+    n = 2 * half + 1
+    k = torch.arange(n, dtype=torch.float64) - half
+    pmf = torch.stack([torch.exp(-k.abs() / (2.0 + 8 * r)) for r in range(rows)])
+    pmf = (pmf / pmf.sum(1, keepdim=True) * 0.999).float()
+    tail = torch.full((rows, 1), 0.001)
+    lengths = torch.full((rows,), n, dtype=torch.int32)
+    return entropy.pmf_to_cdf(pmf, tail, lengths, n).numpy(), (lengths + 2).numpy().astype(np.int32)
+
+
+def test_wide_rows_round_trip_and_match_reference():
+    cdf, lens = wide_table()
+    assert int(lens.max()) - 2 > 256
+    for bad_len in (131, 257):  # negative int8 maximum: rejected
+        with pytest.raises(Exception):
+            entropy.RansDecoder().set_cdf(cdf, np.full_like(lens, bad_len), 1)
+    r = rng(11)
+    n = 30000
+    idx = r.integers(0, len(lens), size=n)
+    v = np.clip(np.round(r.laplace(0, 6 + 20 * idx)), -128, 127).astype(np.int64)
+    y = ((v << 8) + idx).astype(np.int16)
+    for parallel in (1, 3):
+        enc = entropy.RansEncoder()
+        enc.set_cdf(cdf, lens, 1)
+        enc.reset()
+        enc.set_entropy_coder_parallel(parallel)
+        enc.encode_y(y)
+        enc.flush()
+        s = enc.get_encoded_stream()
+        try:
+            m = ref_module()
+        except pytest.skip.Exception:
+            m = None
+        if m is not None:
+            ref = m.RansEncoder()
+            ref.set_cdf(cdf, lens, 1)
+            ref.reset()
+            ref.set_entropy_coder_parallel(parallel)
+            ref.encode_y(y)
+            ref.flush()
+            assert s.tobytes() == np.asarray(ref.get_encoded_stream()).tobytes()
+        dec = entropy.RansDecoder()
+        dec.set_cdf(cdf, lens, 1)
+        dec.set_entropy_coder_parallel(parallel)
+        dec.set_stream(s)
+        assert np.array_equal(dec.decode_y(idx.astype(np.uint8)), v.astype(np.int8))
+
+
+def test_garbage_streams_raise_or_decode_in_bounds(tables):
+    """Random bytes as a stream: the decoder either raises or returns symbols,
+    never reads outside the stream (the reader clamps at its end byte)."""
+    g_cdf, g_len, _, _ = tables
+    w_cdf, w_len = wide_table()
+    r = rng(12)
+    for trial in range(200):
+        cdf, lens = (g_cdf, g_len) if trial % 2 else (w_cdf, w_len)
+        dec = entropy.RansDecoder()
+        dec.set_cdf(cdf, lens, 1)
+        dec.set_entropy_coder_parallel(int(r.integers(1, 9)))
+        data = r.integers(0, 256, size=int(r.integers(0, 200)), dtype=np.uint8)
+        idx = r.integers(0, len(lens), size=int(r.integers(1, 400))).astype(np.uint8)
+        try:
+            dec.set_stream(data)
+            out = dec.decode_y(idx)
+        except Exception:
+            continue
+        assert out.shape == idx.shape
