@@ -19,21 +19,28 @@ constexpr uint32_t kBypassMax = (1u << kBypassBits) - 1;
 constexpr int kMaxBypassGroups = 16;
 
 // Emission-order byte sink: the encoder produces bytes last-to-first.
+// Output of one sub-stream's encoder: bytes in emission order (the stream is
+// their reverse).  `p` and `end` live in locals of the coding loop, so the
+// byte stores cannot force reloads of the table pointers.
 struct Sink {
-    std::vector<uint8_t>* v;
-    inline void push(uint8_t b) { v->push_back(b); }
+    uint8_t* p;
+    uint8_t* end;
+    inline void push(uint8_t b) { *p++ = b; }
 };
 
+// most bytes one symbol can emit: two from the coding step, one per bypass bin
+constexpr int kMaxSymbolBytes = 2 + 2 * kMaxBypassGroups + 4;
+
 // One coding step, r' = (r / freq) * 2^16 + r % freq + start, with the
-// division done by the row's precomputed reciprocal (CdfTable::rcp).
-inline void enc_put(uint32_t& r, Sink& out, uint32_t freq, uint32_t rcp, int shift, uint32_t bias, uint32_t cmpl) {
-    const uint32_t r_max = freq << kRenormShift;
+// division done by the entry's precomputed reciprocal.
+inline void enc_put(uint32_t& r, Sink& out, const CdfTable::EncEntry& e) {
+    const uint32_t r_max = e.freq << kRenormShift;
     while (r >= r_max) {
         out.push(static_cast<uint8_t>(r));
         r >>= 8;
     }
-    const uint32_t q = (uint32_t)(((uint64_t)r * rcp) >> 32) >> shift;
-    r = r + bias + q * cmpl;
+    const uint32_t q = (uint32_t)(((uint64_t)r * e.rcp) >> 32) >> e.shift;
+    r = r + e.bias + q * (uint32_t)e.cmpl;
 }
 
 inline void enc_put_bits(uint32_t& r, Sink& out, uint32_t val) {
@@ -45,9 +52,11 @@ inline void enc_put_bits(uint32_t& r, Sink& out, uint32_t val) {
     r = (r << kBypassBits) | val;
 }
 
-inline void enc_symbol(uint32_t& r, Sink& out, int32_t s, int32_t max_value, const CdfTable& tab, size_t row) {
+// Encode one symbol (py_rans rans.cpp:144-180 semantics): values at or
+// above max_value code max_value, then the excess in bypass bins.
+inline void enc_symbol(uint32_t& r, Sink& out, int32_t s, int32_t max_value, const CdfTable::EncEntry* row) {
     int32_t value = (s < 0 ? -s : s) * 2 - (s > 0 ? 1 : 0);
-    if (value >= max_value) {
+    if (__builtin_expect(value >= max_value, 0)) {
         const uint32_t raw = static_cast<uint32_t>(value - max_value);
         value = max_value;
         uint32_t bins[2 * kMaxBypassGroups + 4];
@@ -64,18 +73,23 @@ inline void enc_symbol(uint32_t& r, Sink& out, int32_t s, int32_t max_value, con
         // LIFO: the decoder reads bins in the order they were listed
         for (int j = nb - 1; j >= 0; --j) enc_put_bits(r, out, bins[j]);
     }
-    const size_t e = row + (size_t)value;
-    enc_put(r, out, tab.freq[e], tab.rcp[e], tab.rcp_shift[e], tab.bias[e], tab.cmpl[e]);
+    enc_put(r, out, row[value]);
 }
 
-// Sub-stream reader.  p[n] is a zero byte: reads past the end return 0
-// without a branch, and pos keeps counting, so pos > n marks a truncated or
-// corrupted stream.
+// Sub-stream reader.  p[n] and p[n + 1] are zero bytes: reads past the end
+// return 0 without a branch, and pos keeps counting, so pos > n marks a
+// truncated or corrupted stream.
 struct Reader {
     const uint8_t* p;
     size_t n;
     size_t pos;
     inline uint32_t peek() const { return p[pos < n ? pos : n]; }
+    // the next two bytes, as reads at pos and pos + 1 would return them
+    inline void peek2(uint32_t& b0, uint32_t& b1) const {
+        const size_t q = pos < n ? pos : n;
+        b0 = p[q];
+        b1 = p[q + 1];
+    }
     inline uint32_t next() {
         const uint32_t v = peek();
         ++pos;
@@ -87,7 +101,7 @@ struct Reader {
 // One decoding step (py_rans.cpp:394-442 semantics).  The symbol search
 // starts at the row's bucket table entry; the state renormalisation takes at
 // most two bytes (the state stays >= 2^23 and loses at most 16 bits per
-// symbol), done without branches.
+// symbol), done without branches on bytes loaded before the state is known.
 inline int32_t dec_symbol(uint32_t& r, Reader& in, const int32_t* cdf, const uint8_t* lut, int32_t max_value,
                           bool& bad) {
     const uint32_t cum = r & kProbMask;
@@ -99,13 +113,14 @@ inline int32_t dec_symbol(uint32_t& r, Reader& in, const int32_t* cdf, const uin
         bad = true;  // corrupted table row or state
         return 0;
     }
+    uint32_t n0, n1;
+    in.peek2(n0, n1);
     r = freq * (r >> kProbBits) + cum - start;
     const uint32_t b1 = r < kLow;
-    r = b1 ? (r << 8) | in.peek() : r;
-    in.pos += b1;
+    r = b1 ? (r << 8) | n0 : r;
     const uint32_t b2 = r < kLow;
-    r = b2 ? (r << 8) | in.peek() : r;
-    in.pos += b2;
+    r = b2 ? (r << 8) | (b1 ? n1 : n0) : r;
+    in.pos += b1 + b2;
     int32_t value = s;
     if (__builtin_expect(value == max_value, 0)) {
         auto get_bits = [&]() {
@@ -137,8 +152,6 @@ void build_table(CdfTable& t, const int32_t* cdfs, int n_cdf, int row_len, const
     t.row_len = row_len;
     t.cdf.assign(cdfs, cdfs + (size_t)n_cdf * row_len);
     t.max_value.resize(n_cdf);
-    t.start.assign((size_t)n_cdf * row_len, 0);
-    t.freq.assign((size_t)n_cdf * row_len, 0);
     for (int i = 0; i < n_cdf; ++i) {
         // cdf_length - 2, held as int8 (py_rans.cpp:266, 377): lengths of 258..385
         // wrap to 0..127 as there; lengths of 130..257 wrap negative, which the
@@ -146,11 +159,6 @@ void build_table(CdfTable& t, const int32_t* cdfs, int n_cdf, int row_len, const
         const int mv = (int8_t)(sizes[i] - 2);
         if (sizes[i] < 2 || mv < 0 || mv + 2 > row_len) throw std::invalid_argument("bad cdf length");
         t.max_value[i] = mv;
-        const int32_t* c = cdfs + (size_t)i * row_len;
-        for (int j = 0; j + 1 < row_len; ++j) {
-            t.start[(size_t)i * row_len + j] = (uint32_t)c[j] & 0xFFFF;
-            t.freq[(size_t)i * row_len + j] = (uint32_t)(c[j + 1] - c[j]) & 0xFFFF;
-        }
     }
     t.lut.assign((size_t)n_cdf * 256, 0);
     for (int i = 0; i < n_cdf; ++i) {
@@ -161,26 +169,28 @@ void build_table(CdfTable& t, const int32_t* cdfs, int n_cdf, int row_len, const
             t.lut[(size_t)i * 256 + b] = (uint8_t)s;  // s <= max_value <= 127
         }
     }
-    const size_t n = (size_t)n_cdf * row_len;
-    t.rcp.assign(n, 0);
-    t.bias.assign(n, 0);
-    t.cmpl.assign(n, 0);
-    t.rcp_shift.assign(n, 0);
-    for (size_t e = 0; e < n; ++e) {
-        const uint32_t f = t.freq[e];
-        if (f == 0) continue;  // never coded (padding or an unused row tail)
-        t.cmpl[e] = (1u << kProbBits) - f;
-        if (f == 1) {
-            // q = r - 1 (r >= 1): r' = r * 2^16 + start
-            t.rcp[e] = ~0u;
-            t.rcp_shift[e] = 0;
-            t.bias[e] = t.start[e] + (1u << kProbBits) - 1;
-        } else {
-            uint32_t sh = 0;
-            while (f > (1u << sh)) ++sh;
-            t.rcp[e] = (uint32_t)(((1ull << (sh + 31)) + f - 1) / f);
-            t.rcp_shift[e] = (uint8_t)(sh - 1);
-            t.bias[e] = t.start[e];
+    t.enc.assign((size_t)n_cdf * row_len, CdfTable::EncEntry{0, 0, 0, 0, 0, 0});
+    for (int i = 0; i < n_cdf; ++i) {
+        const int32_t* c = cdfs + (size_t)i * row_len;
+        for (int j = 0; j + 1 < row_len; ++j) {
+            CdfTable::EncEntry& e = t.enc[(size_t)i * row_len + j];
+            const uint32_t start = (uint32_t)c[j] & 0xFFFF;
+            const uint32_t f = (uint32_t)(c[j + 1] - c[j]) & 0xFFFF;
+            e.freq = f;
+            if (f == 0) continue;  // never coded (padding or an unused row tail)
+            e.cmpl = (uint16_t)((1u << kProbBits) - f);
+            if (f == 1) {
+                // q = r - 1 (r >= 1): r' = r * 2^16 + start
+                e.rcp = ~0u;
+                e.shift = 0;
+                e.bias = start + (1u << kProbBits) - 1;
+            } else {
+                uint32_t sh = 0;
+                while (f > (1u << sh)) ++sh;
+                e.rcp = (uint32_t)(((1ull << (sh + 31)) + f - 1) / f);
+                e.shift = (uint8_t)(sh - 1);
+                e.bias = start;
+            }
         }
     }
 }
@@ -341,39 +351,55 @@ void RansEncoder::flush() {
     std::vector<std::string> errors(n);
     m_pool.run(n, [&](int i) {
         try {
-            std::vector<uint8_t> emitted;
             long total = 0;
             for (auto& t : m_tasks[i]) total += t.count;
-            emitted.reserve((size_t)total * 2 + 64);
-            Sink out{&emitted};
+            // emitted bytes; grown when fewer than kMaxSymbolBytes are left
+            std::vector<uint8_t> emitted((size_t)total * 2 + 4 * kMaxSymbolBytes);
+            Sink out{emitted.data(), emitted.data() + emitted.size()};
+            auto room = [&]() {
+                if (out.end - out.p >= kMaxSymbolBytes) return;
+                const size_t used = (size_t)(out.p - emitted.data());
+                emitted.resize(emitted.size() * 2);
+                out = Sink{emitted.data() + used, emitted.data() + emitted.size()};
+            };
             uint32_t r = kLow;
             for (const Task& t : m_tasks[i]) {
                 if (t.y) {
                     const CdfTable& tab = m_tab[1];
+                    const CdfTable::EncEntry* ent = tab.enc.data();
+                    const int32_t* max_value = tab.max_value.data();
+                    const size_t row_len = (size_t)tab.row_len;
+                    const int n_cdf = tab.n_cdf;
                     for (long k = t.begin + t.count - 1; k >= t.begin; --k) {
                         const int16_t c = t.y[k];
                         const int idx = c & 0xFF;
-                        if (idx >= tab.n_cdf) throw std::invalid_argument("y cdf index out of range");
-                        const int32_t s = static_cast<int8_t>(c >> 8);
-                        const size_t row = (size_t)idx * tab.row_len;
-                        enc_symbol(r, out, s, tab.max_value[idx], tab, row);
+                        if (idx >= n_cdf) throw std::invalid_argument("y cdf index out of range");
+                        room();
+                        enc_symbol(r, out, static_cast<int8_t>(c >> 8), max_value[idx], ent + idx * row_len);
                     }
-                } else {
+                } else if (t.count > 0) {
                     const CdfTable& tab = m_tab[0];
+                    const CdfTable::EncEntry* ent = tab.enc.data();
+                    const int32_t* max_value = tab.max_value.data();
+                    const size_t row_len = (size_t)tab.row_len;
+                    // channel of the last symbol, then counting down
+                    int c = (int)((t.begin + t.count - 1) % t.ch);
                     for (long k = t.begin + t.count - 1; k >= t.begin; --k) {
-                        const int idx = (int)(k % t.ch) + t.cdf_offset;
-                        const size_t row = (size_t)idx * tab.row_len;
-                        enc_symbol(r, out, t.z[k], tab.max_value[idx], tab, row);
+                        const int idx = c + t.cdf_offset;
+                        room();
+                        enc_symbol(r, out, t.z[k], max_value[idx], ent + idx * row_len);
+                        c = (c == 0) ? t.ch - 1 : c - 1;
                     }
                 }
             }
+            const size_t used = (size_t)(out.p - emitted.data());
             std::vector<uint8_t>& s = m_sub[i];
-            s.resize(emitted.size() + 4);
+            s.resize(used + 4);
             s[0] = (uint8_t)r;
             s[1] = (uint8_t)(r >> 8);
             s[2] = (uint8_t)(r >> 16);
             s[3] = (uint8_t)(r >> 24);
-            std::reverse_copy(emitted.begin(), emitted.end(), s.begin() + 4);
+            std::reverse_copy(emitted.begin(), emitted.begin() + used, s.begin() + 4);
         } catch (const std::exception& e) {
             errors[i] = e.what();
         }
@@ -435,7 +461,7 @@ void RansDecoder::set_stream(const uint8_t* data, long size) {
     const int n = m_parallel;
     auto init = [](Sub& s) {
         s.size = s.bytes.size();
-        s.bytes.push_back(0);  // read past the end: zero, and pos > size flags it
+        s.bytes.resize(s.size + 2, 0);  // reads past the end: zero, and pos > size flags them
         if (s.size < 4) throw std::runtime_error("truncated entropy-coded stream");
         s.pos = 4;
         s.overrun = false;
@@ -487,27 +513,51 @@ void RansDecoder::set_stream(const uint8_t* data, long size) {
     }
 }
 
+// Decode out[begin, begin + count) from one sub-stream; row_of(k) gives the
+// cdf row of symbol k (< 0: index out of range, stop).  The table pointers
+// are held in locals: the int8 stores to out may alias anything, and would
+// otherwise force them to be reloaded from the table after every symbol.
+// Returns false on an out-of-range row.
+template <class RowOf>
+bool decode_run(RansDecoder::Sub& s, const CdfTable& tab, long begin, long count, RowOf row_of, int8_t* out) {
+    const int32_t* cdf = tab.cdf.data();
+    const uint8_t* lut = tab.lut.data();
+    const int32_t* max_value = tab.max_value.data();
+    const size_t row_len = (size_t)tab.row_len;
+    Reader rd{s.bytes.data(), s.size, s.pos};
+    uint32_t r = s.state;
+    bool corrupt = false, in_range = true;
+    for (long k = begin; k < begin + count && !corrupt; ++k) {
+        const int idx = row_of(k);
+        if (idx < 0) {
+            in_range = false;
+            break;
+        }
+        out[k] = (int8_t)dec_symbol(r, rd, cdf + idx * row_len, lut + (size_t)idx * 256, max_value[idx], corrupt);
+    }
+    s.state = r;
+    s.pos = rd.pos;
+    s.overrun = s.overrun || corrupt || rd.overrun();
+    return in_range;
+}
+
 void RansDecoder::decode_z(long n, int cdf_offset, int ch, int8_t* out) {
     const CdfTable& tab = m_tab[0];
     if (!tab.valid()) throw std::logic_error("z cdf table not set");
     if (ch <= 0 || cdf_offset < 0 || cdf_offset + ch > tab.n_cdf) throw std::invalid_argument("bad z cdf range");
-    bool bad = false;
     m_pool.run(m_parallel, [&](int i) {
         long begin, count;
         part_range(n, m_parallel, i, begin, count);
-        Sub& s = m_sub[i];
-        Reader rd{s.bytes.data(), s.size, s.pos};
-        uint32_t r = s.state;
-        bool corrupt = false;
-        for (long k = begin; k < begin + count && !corrupt; ++k) {
-            const int idx = (int)(k % ch) + cdf_offset;
-            out[k] = (int8_t)dec_symbol(r, rd, &tab.cdf[(size_t)idx * tab.row_len], &tab.lut[(size_t)idx * 256],
-                                        tab.max_value[idx], corrupt);
-        }
-        s.state = r;
-        s.pos = rd.pos;
-        s.overrun = s.overrun || corrupt || rd.overrun();
+        if (count <= 0) return;
+        int c = (int)(begin % ch);  // channel of symbol k, advanced once per symbol
+        auto row_of = [&](long) {
+            const int idx = c + cdf_offset;
+            c = (c + 1 == ch) ? 0 : c + 1;
+            return idx;
+        };
+        decode_run(m_sub[i], tab, begin, count, row_of, out);
     });
+    bool bad = false;
     for (int i = 0; i < m_parallel; ++i) bad |= m_sub[i].overrun;
     if (bad) throw std::runtime_error("entropy-coded stream is truncated or corrupted");
 }
@@ -516,25 +566,15 @@ void RansDecoder::decode_y(const uint8_t* indexes, long n, int8_t* out) {
     const CdfTable& tab = m_tab[1];
     if (!tab.valid()) throw std::logic_error("y cdf table not set");
     std::vector<char> bad_idx(m_parallel, 0);
+    const int n_cdf = tab.n_cdf;
     m_pool.run(m_parallel, [&](int i) {
         long begin, count;
         part_range(n, m_parallel, i, begin, count);
-        Sub& s = m_sub[i];
-        Reader rd{s.bytes.data(), s.size, s.pos};
-        uint32_t r = s.state;
-        bool corrupt = false;
-        for (long k = begin; k < begin + count && !corrupt; ++k) {
+        auto row_of = [&](long k) {
             const int idx = indexes[k];
-            if (idx >= tab.n_cdf) {
-                bad_idx[i] = 1;
-                break;
-            }
-            out[k] = (int8_t)dec_symbol(r, rd, &tab.cdf[(size_t)idx * tab.row_len], &tab.lut[(size_t)idx * 256],
-                                        tab.max_value[idx], corrupt);
-        }
-        s.state = r;
-        s.pos = rd.pos;
-        s.overrun = s.overrun || corrupt || rd.overrun();
+            return idx < n_cdf ? idx : -1;
+        };
+        if (!decode_run(m_sub[i], tab, begin, count, row_of, out)) bad_idx[i] = 1;
     });
     for (int i = 0; i < m_parallel; ++i) {
         if (bad_idx[i]) throw std::invalid_argument("y cdf index out of range");

@@ -51,13 +51,16 @@ struct CdfTable {
     // one row per distribution: cdf values and the escape symbol max_value
     std::vector<int32_t> cdf;      // n_cdf x row_len
     std::vector<int32_t> max_value;
-    std::vector<uint32_t> start;   // n_cdf x row_len (encoder symbol starts)
-    std::vector<uint32_t> freq;    // n_cdf x row_len
-    // encoder division by freq as a multiply (Alverson's integer reciprocal):
-    // x / freq == (x * rcp >> 32) >> rcp_shift for x < 2^31, and the coding
-    // step x' = (x / f) * 2^16 + x % f + start == x + bias + (x / f) * cmpl
-    std::vector<uint32_t> rcp, bias, cmpl;
-    std::vector<uint8_t> rcp_shift;
+    // encoder entry per (row, symbol), n_cdf x row_len: the division by freq
+    // as a multiply (Alverson's integer reciprocal), x / freq ==
+    // (x * rcp >> 32) >> shift for x < 2^31, and the coding step
+    // x' = (x / f) * 2^16 + x % f + start == x + bias + (x / f) * cmpl
+    struct EncEntry {
+        uint32_t rcp, bias, freq;
+        uint16_t cmpl;  // 2^16 - freq (freq >= 1 wherever it is used)
+        uint8_t shift, pad;
+    };
+    std::vector<EncEntry> enc;
     // decoder search start: lut[row * 256 + b] is the symbol whose interval
     // holds cum = 256 b, so the search for cum walks forward from
     // lut[cum >> 8] (usually zero steps) instead of from symbol 0
@@ -109,14 +112,16 @@ public:
     void decode_z(long n, int cdf_offset, int ch, int8_t* out);
     void decode_y(const uint8_t* indexes, long n, int8_t* out);
 
-private:
+    // one sub-stream being decoded
     struct Sub {
-        std::vector<uint8_t> bytes;  // the sub-stream, then one zero byte
+        std::vector<uint8_t> bytes;  // the sub-stream, then two zero bytes
         size_t size = 0;             // sub-stream bytes
         size_t pos = 0;
         uint32_t state = 0;
         bool overrun = false;
     };
+
+private:
     CdfTable m_tab[2];
     int m_parallel = 1;
     Sub m_sub[kMaxEcParallel];
