@@ -91,12 +91,21 @@ int dw3x3_validate(const DwArgs& a) {
     return DCVC_OK;
 }
 
+// Strips of 8 rows, or of 4 when 8-row strips give fewer than 480 blocks
+// (under two per CU): measured on MI355X, 4-row strips take 10-35 % less
+// time on the 68x120 and 34x60 grids and are slower on 136x240.  Each
+// output's arithmetic is the same for either strip length.
 hipError_t dw3x3_launch(const DwArgs& a, hipStream_t stream) {
-    constexpr int TH = 8;
-    const long strips = (a.H + TH - 1) / TH;
-    const long total = (long)(a.C / 8) * a.W * strips * a.n_img;
-    const int blocks = (int)((total + 255) / 256);
-    hipLaunchKernelGGL(dw3x3_kernel<TH>, dim3(blocks), dim3(256), 0, stream, a);
+    auto blocks_for = [&](int th) {
+        const long strips = (a.H + th - 1) / th;
+        const long total = (long)(a.C / 8) * a.W * strips * a.n_img;
+        return (long)((total + 255) / 256);
+    };
+    const long b8 = blocks_for(8);
+    if (b8 < 480)
+        hipLaunchKernelGGL(dw3x3_kernel<4>, dim3((unsigned)blocks_for(4)), dim3(256), 0, stream, a);
+    else
+        hipLaunchKernelGGL(dw3x3_kernel<8>, dim3((unsigned)b8), dim3(256), 0, stream, a);
     return hipGetLastError();
 }
 

@@ -306,6 +306,32 @@ def test_d3x3(gpu, H, W, C, n_img):
     assert_fp16_close(out, ref, acc, "d3x3")
 
 
+# 4-row strips (the first three: fewer than 480 blocks of 8-row strips) and
+# 8-row strips (the last two); ragged strip ends and odd widths
+@pytest.mark.parametrize("H,W,C,n_img", [(68, 120, 256, 1), (34, 60, 256, 2), (9, 5, 64, 3), (136, 240, 256, 1),
+                                         (135, 240, 512, 1)])
+def test_d3x3_bit_exact_sequential_fp32(gpu, H, W, C, n_img):
+    """Each output is the fp32 sum of its nine taps in (ky, kx) order,
+    starting from 0, rounded to fp16 once.  An fp16 x fp16 product is exact
+    in fp32, so numpy's float32 adds reproduce the kernel's fused
+    multiply-adds bit for bit, whichever strip length the launch uses."""
+    from dcvc_amd import kernels
+    r = rng(H * W + C)
+    x = r.standard_normal((n_img, H, W, C)).astype(np.float16)
+    w = (r.standard_normal((C, 3, 3)) * 0.3).astype(np.float16)
+    xp = np.pad(x.astype(np.float32), ((0, 0), (1, 1), (1, 1), (0, 0)))
+    s = np.zeros((n_img, H, W, C), np.float32)
+    for ky in range(3):
+        for kx in range(3):
+            s = s + xp[:, ky:ky + H, kx:kx + W, :] * w[:, ky, kx].astype(np.float32)
+    wt9 = np.ascontiguousarray(w.reshape(C, 9).T)
+    out = kernels.d3x3(f16(x.reshape(-1, C), gpu), n_img, H, W, f16(wt9, gpu))
+    torch.cuda.synchronize()
+    got = out.cpu().numpy().view(np.uint16)
+    want = s.astype(np.float16).reshape(-1, C).view(np.uint16)
+    assert np.array_equal(got, want), int((got != want).sum())
+
+
 # ------------------------------------------------------------------ layout (bit-exact)
 
 @pytest.mark.parametrize("H,W,cin,f,pb,pr", [(64, 96, 3, 8, 0, 0), (60, 90, 3, 8, 4, 6), (1080, 1920, 24, 8, 8, 0),
