@@ -275,3 +275,43 @@ def test_garbage_streams_raise_or_decode_in_bounds(tables):
         except Exception:
             continue
         assert out.shape == idx.shape
+
+
+def test_escape_only_stream_outgrows_the_first_buffer():
+    """Every symbol escapes through a near-zero tail frequency (16 bits) plus
+    up to 12 bits of bypass bins, about 3.5 bytes a symbol: more than the
+    encoder's first output buffer (2 bytes a symbol), so it must grow."""
+    row = entropy.pmf_to_quantized_cdf(np.array([0.7, 0.3 - 1e-7, 1e-7], np.float32))
+    assert row[3] - row[2] == 1  # the escape symbol (max_value 2) at the minimum frequency
+    cdf = np.array([row] * 4, np.int32)
+    lens = np.full(4, 4, np.int32)
+    r = rng(21)
+    n = 20000
+    v = r.choice(np.array([-128, -100, 90, 127]), size=n)
+    y = ((v.astype(np.int64) << 8) + r.integers(0, 4, size=n)).astype(np.int16)
+    for parallel in (1, 2):
+        enc = entropy.RansEncoder()
+        enc.set_cdf(cdf, lens, 1)
+        enc.reset()
+        enc.set_entropy_coder_parallel(parallel)
+        enc.encode_y(y)
+        enc.flush()
+        s = enc.get_encoded_stream()
+        assert s.size > 3 * n
+        try:
+            m = ref_module()
+        except pytest.skip.Exception:
+            m = None
+        if m is not None:
+            ref = m.RansEncoder()
+            ref.set_cdf(cdf, lens, 1)
+            ref.reset()
+            ref.set_entropy_coder_parallel(parallel)
+            ref.encode_y(y)
+            ref.flush()
+            assert s.tobytes() == np.asarray(ref.get_encoded_stream()).tobytes()
+        dec = entropy.RansDecoder()
+        dec.set_cdf(cdf, lens, 1)
+        dec.set_entropy_coder_parallel(parallel)
+        dec.set_stream(s)
+        assert np.array_equal(dec.decode_y((y.astype(np.int64) & 0xFF).astype(np.uint8)), v.astype(np.int8))
