@@ -232,6 +232,37 @@ def htl_form(x_hat_e, x_hat_d, chunks, H, W, pad_b, pad_r, skip, qp, steps=40, w
     return out
 
 
+def skip0_form(sdp, x_hat_e, x_hat_d, chunks, H, W, pad_b, pad_r, qp, steps=40, warmup=3):
+    """HT-S on the same stream with skip_thres 0, the reference's default:
+    every y symbol is coded, so the encoder hands all of y to the CPU coder
+    (8 sub-streams) and the decoder waits on its y decode."""
+    import torch
+    from dcvc_amd import codec
+    e = codec.VideoCodecHTS(sdp, 0.0)
+    d = codec.VideoCodecHTS(share_from=e)
+    e.add_ref_feature_from_frame(x_hat_e)
+    d.add_ref_feature_from_frame(x_hat_d, apply_feature_adaptor=False)
+    n = len(chunks)
+    lat, nbytes, nsym, ecs = [], [], [], []
+    for i in range(warmup + steps):
+        t0 = time.perf_counter()
+        bits, ec = e.proxy.compress(chunks[i % n], qp, False, pad_b, pad_r)
+        d.proxy.decompress(bits, qp, H, W, ec, False)
+        torch.cuda.synchronize()
+        if i >= warmup:
+            lat.append(time.perf_counter() - t0)
+            nbytes.append(len(bits))
+            nsym.append(e.proxy.last_symbol_count())
+            ecs.append(ec)
+    total = float(np.sum(lat))
+    return {"frames_per_s": round(4.0 * steps / total, 3), "ms_per_step": round(1000 * total / steps, 3),
+            "p50_ms": round(1000 * float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(1000 * float(np.percentile(lat, 99)), 3), "steps": steps,
+            "y_symbols_coded": int(np.mean(nsym)), "ec_parallel": int(np.median(ecs)),
+            "bytes_per_chunk": int(np.mean(nbytes)), "bpp": round(8 * float(np.mean(nbytes)) / (4 * H * W), 5),
+            "skip_thres": 0.0, "precision": "fp16"}
+
+
 def ld_form(x_hat_e, x_hat_d, chunks, H, W, pad_b, pad_r, skip, qp, steps=60, warmup=4):
     """The low-delay model (one frame per call, one entropy round trip) on
     the same stream's frames: frames/s, p50 / p99 per-frame latency, and the
@@ -407,6 +438,7 @@ def main():
     ap.add_argument("--no-batching", action="store_true")
     ap.add_argument("--no-htl", action="store_true", help="skip the HT-L chunk-model section")
     ap.add_argument("--no-ld", action="store_true", help="skip the low-delay model section")
+    ap.add_argument("--no-skip0", action="store_true", help="skip the HT-S skip_thres 0 section")
     ap.add_argument("--serving-seconds", type=float, default=10.0,
                     help="length of the Poisson serving trace (0 skips it)")
     args = ap.parse_args()
@@ -522,6 +554,9 @@ def main():
         log("HT-L form")
         out["htl_form"] = htl_form(enc_i["x_hat"], dec_i["x_hat"], chunks, H, W, pad_b, pad_r, args.skip_thres,
                                    args.qp)
+    if not args.no_skip0:
+        log("skip_thres 0 form")
+        out["skip0_form"] = skip0_form(sdp, enc_i["x_hat"], dec_i["x_hat"], chunks, H, W, pad_b, pad_r, args.qp)
     if not args.no_ld:
         log("LD form")
         out["ld_form"] = ld_form(enc_i["x_hat"], dec_i["x_hat"], chunks, H, W, pad_b, pad_r, args.skip_thres, args.qp)
