@@ -96,9 +96,62 @@ __global__ void pad_unshuffle_kernel(const f16* __restrict__ x, int ldx, int n_i
     *reinterpret_cast<f16x8*>(y + pix * ldy + cg * 8) = v;
 }
 
+// f = 8 with CIN channels packed (ldx == CIN): one thread per (image, output
+// pixel, dy) reads the 8 input pixels of its row segment as CIN 16-byte
+// loads (8 * CIN halves, contiguous) and writes the CIN output groups
+// co = ci * 64 + dy * 8 + [0, 8) as 16-byte stores.  Segments that reach
+// the right padding take the per-element path of the kernel above.
+template <int CIN>
+__global__ __launch_bounds__(256) void unshuffle8_rows_kernel(const f16* __restrict__ x, int n_img, int H, int W,
+                                                              int Ho, int Wo, f16* __restrict__ y, int ldy) {
+    const long item = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long total = (long)n_img * Ho * Wo * 8;
+    if (item >= total) return;
+    const int dy = (int)(item & 7);
+    const long pix = item >> 3;
+    const int wo = (int)(pix % Wo);
+    const int ho = (int)((pix / Wo) % Ho);
+    const int img = (int)(pix / ((long)Wo * Ho));
+    const int h = min(ho * 8 + dy, H - 1);
+    const f16* xr = x + ((size_t)img * H + h) * W * CIN;
+    f16* yp = y + pix * ldy + dy * 8;
+    if (wo * 8 + 7 < W) {
+        f16x8 v[CIN];
+        const f16x8* src = reinterpret_cast<const f16x8*>(xr + (size_t)wo * 8 * CIN);
+#pragma unroll
+        for (int j = 0; j < CIN; ++j) v[j] = src[j];
+#pragma unroll
+        for (int ci = 0; ci < CIN; ++ci) {
+            f16x8 o;
+#pragma unroll
+            for (int p = 0; p < 8; ++p) o[p] = v[(p * CIN + ci) >> 3][(p * CIN + ci) & 7];
+            *reinterpret_cast<f16x8*>(yp + ci * 64) = o;
+        }
+    } else {
+#pragma unroll
+        for (int ci = 0; ci < CIN; ++ci) {
+            f16x8 o;
+#pragma unroll
+            for (int p = 0; p < 8; ++p) o[p] = xr[(size_t)min(wo * 8 + p, W - 1) * CIN + ci];
+            *reinterpret_cast<f16x8*>(yp + ci * 64) = o;
+        }
+    }
+}
+
 hipError_t pad_unshuffle(const f16* x, int ldx, int n_img, int H, int W, int Cin, int f, int padB,
                          int padR, f16* y, int ldy, hipStream_t s) {
     const int Ho = (H + padB) / f, Wo = (W + padR) / f;
+    const bool aligned = ((uintptr_t)x % 16 == 0) && ((uintptr_t)y % 16 == 0) && ldy % 8 == 0 && (W * Cin) % 8 == 0;
+    if (f == 8 && ldx == Cin && aligned && (Cin == 3 || Cin == 24)) {
+        const long total = (long)n_img * Ho * Wo * 8;
+        if (Cin == 3)
+            hipLaunchKernelGGL(unshuffle8_rows_kernel<3>, grid1d(total), dim3(256), 0, s, x, n_img, H, W, Ho, Wo, y,
+                               ldy);
+        else
+            hipLaunchKernelGGL(unshuffle8_rows_kernel<24>, grid1d(total), dim3(256), 0, s, x, n_img, H, W, Ho, Wo,
+                               y, ldy);
+        return hipGetLastError();
+    }
     const long total = (long)n_img * Ho * Wo * (Cin * f * f / 8);
     hipLaunchKernelGGL(pad_unshuffle_kernel, grid1d(total), dim3(256), 0, s, x, ldx, n_img, H, W, Cin, f,
                        Ho, Wo, y, ldy);
@@ -122,8 +175,53 @@ __global__ void pixel_shuffle8_kernel(const f16* __restrict__ x, int ldx, int n_
     }
 }
 
+// C = 3, packed output (ldy == 3): a block takes up to 32 consecutive source
+// pixels of one 1/8-scale row. It loads their 192 channels contiguously into
+// LDS. It then writes the 8 output rows they expand to, 256 pixels x 3
+// channels each, as contiguous 16-byte stores. Output position q of a row
+// is pixel q / 3, channel q % 3: source pixel (q / 3) >> 3, channel
+// (q % 3) * 64 + dy * 8 + ((q / 3) & 7).
+constexpr int kShufPix = 32;
+__global__ __launch_bounds__(256) void pixel_shuffle8_c3_kernel(const f16* __restrict__ x, int ldx, int n_img,
+                                                                int H8, int W8, f16* __restrict__ y) {
+    __shared__ f16x8 tile[kShufPix * 24];
+    const int groups = (W8 + kShufPix - 1) / kShufPix;
+    const int g = blockIdx.x % groups;
+    const int row8 = blockIdx.x / groups;  // img * H8 + h8
+    const int sp0 = g * kShufPix;
+    const int n_sp = min(kShufPix, W8 - sp0);
+    const f16* src = x + ((size_t)row8 * W8 + sp0) * ldx;
+    for (int c = threadIdx.x; c < n_sp * 24; c += 256) {
+        const int sp = c / 24, j = c - sp * 24;
+        tile[c] = *reinterpret_cast<const f16x8*>(src + (size_t)sp * ldx + j * 8);
+    }
+    __syncthreads();
+    const f16* t = reinterpret_cast<const f16*>(tile);
+    const int img = row8 / H8, h8 = row8 - img * H8;
+    const int W = W8 * 8;
+    const int per_row = n_sp * 3;  // 16-byte chunks per output row segment
+    for (int c = threadIdx.x; c < 8 * per_row; c += 256) {
+        const int dy = c / per_row, k = c - dy * per_row;
+        f16x8 o;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int q = k * 8 + i;
+            const int px = q / 3, ch = q - px * 3;
+            const float v = (float)t[(px >> 3) * 192 + ch * 64 + dy * 8 + (px & 7)];
+            o[i] = (f16)fminf(fmaxf(v, -0.5f), 0.5f);
+        }
+        f16* dst = y + (((size_t)img * H8 * 8 + h8 * 8 + dy) * W + sp0 * 8) * 3;
+        reinterpret_cast<f16x8*>(dst)[k] = o;
+    }
+}
+
 hipError_t pixel_shuffle8_clamp(const f16* x, int ldx, int n_img, int H8, int W8, int C, f16* y, int ldy,
                                 hipStream_t s) {
+    if (C == 3 && ldy == 3 && ldx == 192 && (uintptr_t)x % 16 == 0 && (uintptr_t)y % 16 == 0) {
+        const int blocks = n_img * H8 * ((W8 + kShufPix - 1) / kShufPix);
+        hipLaunchKernelGGL(pixel_shuffle8_c3_kernel, dim3(blocks), dim3(256), 0, s, x, ldx, n_img, H8, W8, y);
+        return hipGetLastError();
+    }
     const long total = (long)n_img * H8 * 8 * W8 * 8;
     hipLaunchKernelGGL(pixel_shuffle8_kernel, grid1d(total), dim3(256), 0, s, x, ldx, n_img, H8, W8, C, y, ldy);
     return hipGetLastError();

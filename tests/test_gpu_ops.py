@@ -334,25 +334,30 @@ def test_d3x3_bit_exact_sequential_fp32(gpu, H, W, C, n_img):
 
 # ------------------------------------------------------------------ layout (bit-exact)
 
-@pytest.mark.parametrize("H,W,cin,f,pb,pr", [(64, 96, 3, 8, 0, 0), (60, 90, 3, 8, 4, 6), (1080, 1920, 24, 8, 8, 0),
-                                             (32, 48, 64, 2, 0, 0)])
-def test_pad_and_unshuffle(gpu, H, W, cin, f, pb, pr):
+# f = 8 with 3 or 24 packed channels and W * cin % 8 == 0 takes the row-segment
+# kernel (its right-padding segments included); the rest the per-element one
+@pytest.mark.parametrize("H,W,cin,f,pb,pr,n_img", [(64, 96, 3, 8, 0, 0, 1), (60, 90, 3, 8, 4, 6, 1),
+                                                   (1080, 1920, 24, 8, 8, 0, 1), (32, 48, 64, 2, 0, 0, 1),
+                                                   (60, 90, 24, 8, 4, 6, 2), (1080, 1920, 3, 8, 8, 0, 1),
+                                                   (20, 36, 3, 8, 4, 4, 3)])
+def test_pad_and_unshuffle(gpu, H, W, cin, f, pb, pr, n_img):
     from dcvc_amd import kernels
-    x = torch.from_numpy(rng(H).standard_normal((1, cin, H, W)).astype(np.float16))
+    x = torch.from_numpy(rng(H + n_img).standard_normal((n_img, cin, H, W)).astype(np.float16))
     ref = F.pixel_unshuffle(F.pad(x.float(), (0, pr, 0, pb), mode="replicate"), f).half()
     ref = ref.permute(0, 2, 3, 1).reshape(-1, cin * f * f)
     xg = x.permute(0, 2, 3, 1).contiguous().reshape(-1, cin).to(gpu)
-    out = kernels.pad_and_unshuffle(xg, 1, H, W, cin, f, pb, pr)
+    out = kernels.pad_and_unshuffle(xg, n_img, H, W, cin, f, pb, pr)
     torch.cuda.synchronize()
     assert torch.equal(out.cpu(), ref)
 
 
-def test_pixel_shuffle_8_clamp(gpu):
+# C = 3 with packed output takes the 8-pixel-group kernel; C = 4 the per-pixel one
+@pytest.mark.parametrize("H8,W8,C,n_img", [(17, 30, 3, 1), (136, 240, 3, 1), (5, 7, 3, 2), (17, 30, 4, 1)])
+def test_pixel_shuffle_8_clamp(gpu, H8, W8, C, n_img):
     from dcvc_amd import kernels
-    H8, W8, C = 17, 30, 3
-    x = torch.from_numpy(rng(2).standard_normal((1, 64 * C, H8, W8)).astype(np.float16))
+    x = torch.from_numpy(rng(2 + H8).standard_normal((n_img, 64 * C, H8, W8)).astype(np.float16))
     ref = torch.clamp(F.pixel_shuffle(x.float(), 8), -0.5, 0.5).half().permute(0, 2, 3, 1).reshape(-1, C)
-    out = kernels.pixel_shuffle_8(x.permute(0, 2, 3, 1).contiguous().reshape(-1, 64 * C).to(gpu), 1, H8, W8, C)
+    out = kernels.pixel_shuffle_8(x.permute(0, 2, 3, 1).contiguous().reshape(-1, 64 * C).to(gpu), n_img, H8, W8, C)
     torch.cuda.synchronize()
     assert torch.equal(out.cpu(), ref)
 
