@@ -28,52 +28,57 @@ struct Sink {
     inline void push(uint8_t b) { *p++ = b; }
 };
 
-// most bytes one symbol can emit: two from the coding step, one per bypass bin
-constexpr int kMaxSymbolBytes = 2 + 2 * kMaxBypassGroups + 4;
+// most bytes one symbol can emit: two from the coding step, one per bypass
+// bin, and room for the coding step's unconditional store
+constexpr int kMaxSymbolBytes = 2 + 2 * kMaxBypassGroups + 4 + 1;
 
 // One coding step, r' = (r / freq) * 2^16 + r % freq + start, with the
-// division done by the entry's precomputed reciprocal.
+// division done by the entry's precomputed reciprocal.  The renormalisation
+// emits at most two bytes (r < 2^31, freq << 15 >= 2^15), without branches:
+// each byte is stored, and the sink advances only when it is emitted (the
+// sink always has room for a symbol's bytes plus one).
 inline void enc_put(uint32_t& r, Sink& out, const CdfTable::EncEntry& e) {
     const uint32_t r_max = e.freq << kRenormShift;
-    while (r >= r_max) {
-        out.push(static_cast<uint8_t>(r));
-        r >>= 8;
-    }
+    uint32_t b = r >= r_max;
+    out.p[0] = static_cast<uint8_t>(r);
+    out.p += b;
+    r = b ? r >> 8 : r;
+    b = r >= r_max;
+    out.p[0] = static_cast<uint8_t>(r);
+    out.p += b;
+    r = b ? r >> 8 : r;
     const uint32_t q = (uint32_t)(((uint64_t)r * e.rcp) >> 32) >> e.shift;
     r = r + e.bias + q * (uint32_t)e.cmpl;
 }
 
-inline void enc_put_bits(uint32_t& r, Sink& out, uint32_t val) {
-    constexpr uint32_t x_max = (1u << (kProbBits - kBypassBits)) << kRenormShift;
-    while (r >= x_max) {
+// Push one bypass bin: emit a byte first if r >= 2^29 (at most one: r < 2^31).
+constexpr uint32_t kBinLimit = 1u << (kProbBits - kBypassBits + kRenormShift);  // 2^29
+inline void enc_put_bin(uint32_t& r, Sink& out, uint32_t bin) {
+    if (r >= kBinLimit) {
         out.push(static_cast<uint8_t>(r));
         r >>= 8;
     }
-    r = (r << kBypassBits) | val;
+    r = (r << kBypassBits) | bin;
 }
 
 // Encode one symbol (py_rans rans.cpp:144-180 semantics): values at or
-// above max_value code max_value, then the excess in bypass bins.
+// above max_value code max_value, then the excess in bypass bins.  The
+// decoder reads the bin count (kBypassMax bins, then the remainder) and the
+// excess low group first, so they are pushed in the reverse order.
 inline void enc_symbol(uint32_t& r, Sink& out, int32_t s, int32_t max_value, const CdfTable::EncEntry* row) {
     int32_t value = (s < 0 ? -s : s) * 2 - (s > 0 ? 1 : 0);
     if (__builtin_expect(value >= max_value, 0)) {
         const uint32_t raw = static_cast<uint32_t>(value - max_value);
         value = max_value;
-        uint32_t bins[2 * kMaxBypassGroups + 4];
-        int nb = 0;
-        int groups = 0;
-        while (groups < kMaxBypassGroups && (raw >> (groups * kBypassBits)) != 0) ++groups;
-        int cnt = groups;
-        while (cnt >= (int)kBypassMax) {
-            bins[nb++] = kBypassMax;
-            cnt -= kBypassMax;
-        }
-        bins[nb++] = static_cast<uint32_t>(cnt);
-        for (int j = 0; j < groups; ++j) bins[nb++] = (raw >> (j * kBypassBits)) & kBypassMax;
-        // LIFO: the decoder reads bins in the order they were listed
-        for (int j = nb - 1; j >= 0; --j) enc_put_bits(r, out, bins[j]);
+        const int groups = raw ? (32 - __builtin_clz(raw) + kBypassBits - 1) / kBypassBits : 0;
+        for (int j = groups - 1; j >= 0; --j) enc_put_bin(r, out, (raw >> (kBypassBits * j)) & kBypassMax);
+        const int q = groups / (int)kBypassMax;
+        enc_put_bin(r, out, (uint32_t)(groups - q * (int)kBypassMax));
+        for (int j = 0; j < q; ++j) enc_put_bin(r, out, kBypassMax);
     }
-    enc_put(r, out, row[value]);
+    const CdfTable::EncEntry& e = row[value];
+    if (__builtin_expect(e.freq == 0, 0)) throw std::invalid_argument("symbol has zero frequency in its cdf row");
+    enc_put(r, out, e);
 }
 
 // Sub-stream reader.  p[n] and p[n + 1] are zero bytes: reads past the end
