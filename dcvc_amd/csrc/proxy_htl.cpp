@@ -529,6 +529,8 @@ void HTLProxy::decompress(const uint8_t* bits, long nbytes, int qp, int height, 
         throw std::invalid_argument("picture size does not match the prepared resolution");
     m_qp = qp;
     enter(s);
+    // a call that threw may have left side-stream work reading the memory
+    hip_check(hipStreamWaitEvent(m_stream, m_ev_join, 0), "stream wait");
     set_device_int(m_d_qp, qp, m_stream);
     m_g_dec0[m_memory_has_value ? 1 : 0].run(m_stream, m_use_graphs, [&] { dec_stage0(m_memory_has_value); });
     fork_side();

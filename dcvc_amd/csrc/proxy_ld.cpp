@@ -77,19 +77,26 @@ void LDWeights::load(const HostDict& d, float skip) {
 LDProxy::LDProxy() {
     hip_check(hipStreamCreateWithFlags(&m_stream, hipStreamNonBlocking), "stream");
     hip_check(hipStreamCreateWithFlags(&m_copy, hipStreamNonBlocking), "stream");
+    hip_check(hipStreamCreateWithFlags(&m_side, hipStreamNonBlocking), "stream");
     hip_check(hipEventCreateWithFlags(&m_ev_in, hipEventDisableTiming), "event");
     hip_check(hipEventCreateWithFlags(&m_ev_out, hipEventDisableTiming), "event");
     hip_check(hipEventCreateWithFlags(&m_ev_sym, hipEventDisableTiming), "event");
+    hip_check(hipEventCreateWithFlags(&m_ev_fork, hipEventDisableTiming), "event");
+    hip_check(hipEventCreateWithFlags(&m_ev_join, hipEventDisableTiming), "event");
 }
 
 LDProxy::~LDProxy() {
     if (m_stream) hipStreamSynchronize(m_stream);
+    if (m_side) hipStreamSynchronize(m_side);
     reset_graphs();
     for (void* p : {(void*)m_h_count, (void*)m_h_sym, (void*)m_h_idx, (void*)m_h_dec, (void*)m_h_z}) hipHostFree(p);
     m_buf.free_all();
     hipEventDestroy(m_ev_in);
     hipEventDestroy(m_ev_out);
     hipEventDestroy(m_ev_sym);
+    hipEventDestroy(m_ev_fork);
+    hipEventDestroy(m_ev_join);
+    hipStreamDestroy(m_side);
     hipStreamDestroy(m_copy);
     hipStreamDestroy(m_stream);
 }
@@ -113,6 +120,7 @@ void LDProxy::reset_graphs() {
     // a graph may still be running on the proxy stream (the encoder returns
     // before its context update finishes)
     if (m_stream) hipStreamSynchronize(m_stream);
+    if (m_side) hipStreamSynchronize(m_side);
     m_g_enc0.reset();
     for (auto& g : m_g_enc1) g.reset();
     for (auto& g : m_g_dec0) g.reset();
@@ -151,6 +159,12 @@ void LDProxy::prepare(int Hp, int Wp, f16* x_hat, int ld_out) {
     m_scratch.so = M.alloc_n<f16>(m_scratch.halves);
     m_scratch.x1 = M.alloc_n<f16>(m_scratch.halves);
     m_scratch.x2 = M.alloc_n<f16>(m_scratch.halves);
+    m_scratch2.halves = m_scratch.halves;
+    m_scratch2.t1 = M.alloc_n<f16>(m_scratch2.halves);
+    m_scratch2.t2 = M.alloc_n<f16>(m_scratch2.halves);
+    m_scratch2.so = M.alloc_n<f16>(m_scratch2.halves);
+    m_scratch2.x1 = M.alloc_n<f16>(m_scratch2.halves);
+    m_scratch2.x2 = M.alloc_n<f16>(m_scratch2.halves);
     m_d_qp = M.alloc_n<int>(1);
     m_d_count = M.alloc_n<int>(1);
     m_d_offs = M.alloc_n<int>(compact_blocks(m_ny) + 1);
@@ -392,11 +406,13 @@ void LDProxy::dec_stage1() {
     hip_check(compact(m_idx, 1, m_cond, m_ny, m_idxc, m_d_offs, m_d_count, m_stream), "compact");
 }
 
-// lambda_dec_2 (dmc_ld_proxy.cpp:560): ctx, while the host decodes y
+// lambda_dec_2 (dmc_ld_proxy.cpp:560): ctx from the new memory.  It needs
+// only dec_0's output, so it runs on the side stream from dec_0 on, during
+// the host's z decode and dec_1 (the reference starts it after dec_1).
 void LDProxy::dec_stage2() {
-    Exec e = exec();
+    Exec e = exec_side();
     const Act mem = make_act(m_cat_fam, 1, m_H8, m_W8, 256, 512);
-    const Act tmp = make_act(m_scratch.x1, 1, m_H8, m_W8, 256, 256);
+    const Act tmp = make_act(m_scratch2.x1, 1, m_H8, m_W8, 256, 256);
     m_w->fe.forward(e, mem, make_act(m_cat_dec + 256, 1, m_H8, m_W8, 256, 512), tmp);
 }
 
@@ -418,15 +434,20 @@ void LDProxy::decompress(const uint8_t* bits, long nbytes, int qp, int height, i
         throw std::invalid_argument("picture size does not match the prepared resolution");
     m_qp = qp;
     enter(s);
+    // a call that threw may have left side-stream work reading the memory
+    hip_check(hipStreamWaitEvent(m_stream, m_ev_join, 0), "stream wait");
     set_device_int(m_d_qp, qp, m_stream);
     m_g_dec0[m_memory_has_value ? 1 : 0].run(m_stream, m_use_graphs, [&] { dec_stage0(m_memory_has_value); });
+    hip_check(hipEventRecord(m_ev_fork, m_stream), "event record");
+    hip_check(hipStreamWaitEvent(m_side, m_ev_fork, 0), "stream wait");
+    m_g_dec2.run(m_side, m_use_graphs, [&] { dec_stage2(); });
+    hip_check(hipEventRecord(m_ev_join, m_side), "event record");
     m_dec.set_parallel(ec_parallel);
     m_dec.set_stream(bits, nbytes);
     m_dec.decode_z(m_nz, qp * 128, 128, m_h_z);
     hip_check(hipMemcpyAsync(m_z_i8, m_h_z, m_nz, hipMemcpyHostToDevice, m_stream), "h2d");
     m_g_dec1.run(m_stream, m_use_graphs, [&] { dec_stage1(); });
     hip_check(hipEventRecord(m_ev_sym, m_stream), "event");
-    m_g_dec2.run(m_stream, m_use_graphs, [&] { dec_stage2(); });
     hip_check(hipStreamWaitEvent(m_copy, m_ev_sym, 0), "wait");
     hip_check(hipMemcpyAsync(m_h_count, m_d_count, sizeof(int), hipMemcpyDeviceToHost, m_copy), "d2h");
     hip_check(hipStreamSynchronize(m_copy), "sync");
@@ -437,6 +458,7 @@ void LDProxy::decompress(const uint8_t* bits, long nbytes, int qp, int height, i
     m_last_count = n;
     m_dec.decode_y(m_h_idx, n, m_h_dec);
     hip_check(hipMemcpyAsync(m_dec_c, m_h_dec, n, hipMemcpyHostToDevice, m_stream), "h2d");
+    hip_check(hipStreamWaitEvent(m_stream, m_ev_join, 0), "stream wait");  // ctx
     m_g_dec3.run(m_stream, m_use_graphs, [&] { dec_stage3(); });
     leave(s);
     hip_check(hipStreamSynchronize(m_stream), "sync");

@@ -67,12 +67,15 @@ private:
     void dec_stage2();
     void dec_stage3();
     Exec exec() { return Exec{m_stream, m_d_qp, &m_scratch, m_prec}; }
+    // the decoder's side stream: feature extraction while the host decodes
+    // z and the prior runs (independent work, own scratch)
+    Exec exec_side() { return Exec{m_side, m_d_qp, &m_scratch2, m_prec}; }
 
     std::shared_ptr<LDWeights> m_w;
     RansEncoder m_enc;
     RansDecoder m_dec;
-    hipStream_t m_stream = nullptr, m_copy = nullptr;
-    hipEvent_t m_ev_in = nullptr, m_ev_out = nullptr, m_ev_sym = nullptr;
+    hipStream_t m_stream = nullptr, m_copy = nullptr, m_side = nullptr;
+    hipEvent_t m_ev_in = nullptr, m_ev_out = nullptr, m_ev_sym = nullptr, m_ev_fork = nullptr, m_ev_join = nullptr;
     bool m_use_graphs = true;
     int m_prec = PREC_F16;
     Graph m_g_enc0, m_g_enc1[2], m_g_dec0[2], m_g_dec1, m_g_dec2, m_g_dec3;
@@ -83,7 +86,7 @@ private:
     long m_ny = 0, m_nz = 0;
 
     DevMem m_buf;
-    Scratch m_scratch;
+    Scratch m_scratch, m_scratch2;
     int* m_d_qp = nullptr;
     int* m_d_count = nullptr;
     int* m_d_offs = nullptr;
