@@ -173,13 +173,25 @@ class Server:
 
     @staticmethod
     def stats(requests, frames_per_request=4):
+        """Latency percentiles over all served requests, and over the warm ones:
+        each (stream, kind)'s first request also allocates that stream's
+        context and captures its graphs, a one-time cost of a new stream."""
         ok = [r for r in requests if not r.error and r.t_done >= 0]
         lat = np.array([r.latency for r in ok]) * 1e3
         span = max(r.t_done for r in ok) - min(r.t_arrive for r in ok) if ok else 0.0
+        seen, warm, first = set(), [], []
+        for r in sorted(ok, key=lambda r: r.t_start):
+            key = (r.stream, r.kind)
+            (warm if key in seen else first).append(r.latency * 1e3)
+            seen.add(key)
         return {"requests": len(requests), "served": len(ok), "errors": len(requests) - len(ok),
                 "p50_ms": round(float(np.percentile(lat, 50)), 3) if ok else None,
                 "p99_ms": round(float(np.percentile(lat, 99)), 3) if ok else None,
                 "mean_ms": round(float(lat.mean()), 3) if ok else None,
+                "warm_p50_ms": round(float(np.percentile(warm, 50)), 3) if warm else None,
+                "warm_p99_ms": round(float(np.percentile(warm, 99)), 3) if warm else None,
+                "first_requests": len(first),
+                "first_request_mean_ms": round(float(np.mean(first)), 3) if first else None,
                 "frames_per_s": round(frames_per_request * len(ok) / span, 2) if span > 0 else None}
 
 

@@ -67,3 +67,20 @@ def test_requests_arriving_mid_run_overlap_running_ones():
     # the second request started while the first was still running
     assert reqs[1].t_start < reqs[0].t_done
     assert srv.max_running == 2
+
+
+def test_stats_split_first_requests_from_warm_ones():
+    """Each (stream, kind)'s first request counts as a first request; the
+    warm percentiles cover the rest."""
+    from dcvc_amd.serving import Request, Server
+    reqs = []
+    t = 0.0
+    for i, (stream, kind, lat) in enumerate([(0, "encode", 0.100), (1, "decode", 0.080), (0, "encode", 0.005),
+                                             (0, "decode", 0.090), (1, "decode", 0.004), (0, "encode", 0.006)]):
+        reqs.append(Request(rid=i, t_arrive=t, stream=stream, kind=kind, qp=0, t_start=t, t_done=t + lat))
+        t += 0.01
+    st = Server.stats(reqs)
+    assert st["first_requests"] == 3
+    assert abs(st["first_request_mean_ms"] - 90.0) < 1e-6
+    assert st["warm_p99_ms"] < 6.1 and st["warm_p50_ms"] == 5.0
+    assert st["p99_ms"] > 90
