@@ -18,9 +18,8 @@ constexpr int kBypassBits = 2;
 constexpr uint32_t kBypassMax = (1u << kBypassBits) - 1;
 constexpr int kMaxBypassGroups = 16;
 
-// Emission-order byte sink: the encoder produces bytes last-to-first.
-// Output of one sub-stream's encoder: bytes in emission order (the stream is
-// their reverse).  `p` and `end` live in locals of the coding loop, so the
+// Output of one sub-stream's encoder: bytes in emission order (the encoder
+// produces them last to first, so the stream is their reverse).  `p` and `end` live in locals of the coding loop, so the
 // byte stores cannot force reloads of the table pointers.
 struct Sink {
     uint8_t* p;
