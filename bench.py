@@ -124,7 +124,8 @@ def precision_forms(sdp, p_enc_ref, x_hat_i, chunks, H, W, pad_b, pad_r, skip, d
     import torch
     from dcvc_amd import codec, kernels
     from oracle import dcvc_oracle as O
-    out = {"layers": {}, "fps": {}, "per_qp": {}}
+    out = {"layers": {}, "fps": {}, "fps_loop_y_symbols": {}, "fps_loop_bytes_per_chunk": {},
+           "traced_kernel_ms": {}, "per_qp": {}}
     n_chunks = len(chunks)
     P16 = ((H + 15) // 16) * ((W + 15) // 16)
 
@@ -144,11 +145,21 @@ def precision_forms(sdp, p_enc_ref, x_hat_i, chunks, H, W, pad_b, pad_r, skip, d
             d.proxy.decompress(bits, 32, H, W, ec, False)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
+        nsym, nbytes = [], []
         for i in range(steps):
             bits, ec = e.proxy.compress(chunks[i % n_chunks], 32, False, pad_b, pad_r)
             d.proxy.decompress(bits, 32, H, W, ec, False)
+            nsym.append(e.proxy.last_symbol_count())
+            nbytes.append(len(bits))
         torch.cuda.synchronize()
         out["fps"][prec] = round(4.0 * steps / (time.perf_counter() - t0), 2)
+        # the host coder's share of a step grows with the coded symbols, which
+        # differ by precision (the skip threshold acts on the computed scales)
+        out["fps_loop_y_symbols"][prec] = int(np.mean(nsym))
+        out["fps_loop_bytes_per_chunk"][prec] = int(np.mean(nbytes))
+        # GPU work of one encode + decode in this precision (launch trace)
+        rec_e, rec_d = layer_trace(e, d, chunks[0], 32, pad_b, pad_r, H, W)
+        out["traced_kernel_ms"][prec] = round(float(np.concatenate([rec_e, rec_d])[:, 7].sum()) * 1e-6, 3)
     for qp in (0, 21, 42, 63):
         res = {}
         ref_syms = None
