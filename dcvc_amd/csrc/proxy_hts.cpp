@@ -230,6 +230,9 @@ void HTSProxy::prepare(int Hp, int Wp, f16* const frames[kFrames], int ld_out) {
 void HTSProxy::enter(hipStream_t caller) {
     hip_check(hipEventRecord(m_ev_in, caller), "event record");
     hip_check(hipStreamWaitEvent(m_stream, m_ev_in, 0), "stream wait");
+    // side-stream work of an earlier call that threw before joining it may
+    // still read the memory: order this call after it
+    hip_check(hipStreamWaitEvent(m_stream, m_ev_join, 0), "stream wait");
 }
 
 void HTSProxy::leave(hipStream_t caller) {
@@ -505,8 +508,6 @@ void HTSProxy::decompress(const uint8_t* bits, long nbytes, int qp, int height, 
         throw std::logic_error("add_ref_feature_from_frame must precede decompress");
     m_qp = qp;
     enter(s);
-    // a call that threw may have left side-stream work reading the memory
-    hip_check(hipStreamWaitEvent(m_stream, m_ev_join, 0), "stream wait");
     set_device_int(m_d_qp, qp, m_stream);
     m_g_dec0[m_memory_has_value ? 1 : 0].run(m_stream, m_use_graphs, [&] { dec_stage0(m_memory_has_value); });
     // the context from the new memory, on the side stream during dec_1

@@ -211,6 +211,9 @@ void LDProxy::prepare(int Hp, int Wp, f16* x_hat, int ld_out) {
 void LDProxy::enter(hipStream_t caller) {
     hip_check(hipEventRecord(m_ev_in, caller), "event record");
     hip_check(hipStreamWaitEvent(m_stream, m_ev_in, 0), "stream wait");
+    // side-stream work of an earlier call that threw before joining it may
+    // still read the memory: order this call after it
+    hip_check(hipStreamWaitEvent(m_stream, m_ev_join, 0), "stream wait");
 }
 
 void LDProxy::leave(hipStream_t caller) {
@@ -434,8 +437,6 @@ void LDProxy::decompress(const uint8_t* bits, long nbytes, int qp, int height, i
         throw std::invalid_argument("picture size does not match the prepared resolution");
     m_qp = qp;
     enter(s);
-    // a call that threw may have left side-stream work reading the memory
-    hip_check(hipStreamWaitEvent(m_stream, m_ev_join, 0), "stream wait");
     set_device_int(m_d_qp, qp, m_stream);
     m_g_dec0[m_memory_has_value ? 1 : 0].run(m_stream, m_use_graphs, [&] { dec_stage0(m_memory_has_value); });
     hip_check(hipEventRecord(m_ev_fork, m_stream), "event record");

@@ -164,3 +164,37 @@ def test_hts_symbols_and_psnr_vs_oracle(gpu, models):
         pg, po = psnr(outs[i], src), psnr(r.x_hat[i], src)
         print(f"frame {i}: PSNR gpu {pg:.4f} oracle {po:.4f}")
         assert abs(pg - po) < 0.02
+
+
+def test_hts_decoder_recovers_after_a_corrupt_bitstream(gpu, models):
+    """As for LD: a truncated chunk bitstream raises in the host z decode
+    with the feature extraction already queued on the side stream; after both
+    sides re-anchor on the intra frame, the next chunk decodes bit for bit."""
+    from dcvc_amd.inference_extensions_cuda import DMCIProxy, DMCHTSProxy
+    sdi, sdp = models
+    frames = synthetic.motion_sequence_yuv420(H, W, 17, seed=4)
+    i_enc = DMCIProxy()
+    i_enc.set_param(sdi, 0.0)
+    bits_i, x_hat_i, ec_i = i_enc.compress(_to_gpu_nchw(synthetic.frame_to_input(frames[0]), gpu), 32, 0, 0)
+    i_dec = DMCIProxy()
+    i_dec.set_param(sdi, 0.0)
+    x_hat_id = i_dec.decompress(bits_i, 32, H, W, ec_i)
+    enc, dec = DMCHTSProxy(), DMCHTSProxy()
+    enc.set_param(sdp, 0.0)
+    dec.set_param(sdp, 0.0)
+    enc.add_ref_feature_from_frame(x_hat_i, True)
+    dec.add_ref_feature_from_frame(x_hat_id, False)
+    x = _to_gpu_nchw(synthetic.chunk_to_input(frames[1:9]), gpu)
+    bits, ec = enc.compress(x, 32, False, 0, 0)
+    with pytest.raises(Exception):
+        dec.decompress(np.ascontiguousarray(bits[:3]), 32, H, W, ec, False)
+    enc.add_ref_feature_from_frame(x_hat_i, True)
+    dec.add_ref_feature_from_frame(x_hat_id, False)
+    P16 = (H // 16) * (W // 16)
+    x = _to_gpu_nchw(synthetic.chunk_to_input(frames[9:17]), gpu)
+    bits, ec = enc.compress(x, 32, False, 0, 0)
+    yhat_e = enc.debug_copy(0, P16 * 512 * 2, np.float16).reshape(P16, 512)[:, :256].copy()
+    dec.decompress(bits, 32, H, W, ec, False)
+    torch.cuda.synchronize()
+    yhat_d = dec.debug_copy(0, P16 * 512 * 2, np.float16).reshape(P16, 512)[:, :256]
+    assert np.array_equal(yhat_e.view(np.uint16), yhat_d.view(np.uint16))

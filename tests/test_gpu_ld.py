@@ -135,3 +135,35 @@ def test_ld_symbols_and_psnr_vs_oracle(gpu, models):
     assert sym_m > 0.98 and idx_m > 0.85
     assert abs(len(bits) - n_o) / n_o < 0.01
     assert abs(pg - po) < 0.05
+
+
+def test_ld_decoder_recovers_after_a_corrupt_bitstream(gpu, models):
+    """A truncated bitstream raises in the host z decode, after the decoder
+    has started its context extraction on the side stream. Both sides then
+    re-anchor on the intra frame. The next frames decode bit for bit: every
+    call is ordered after side work a failed call left behind."""
+    from dcvc_amd.inference_extensions_cuda import DMCLDProxy
+    sdi, sdp = models
+    H, W = 64, 96
+    pad_r, pad_b = synthetic.get_padding_size(H, W, 16)
+    Hp, Wp = H + pad_b, W + pad_r
+    frames = synthetic.motion_sequence_yuv420(H, W, 4, seed=5)
+    x_hat_i, x_hat_id = _intra(sdi, synthetic.frame_to_input(frames[0]), gpu, H, W, pad_b, pad_r)
+    enc, dec = DMCLDProxy(), DMCLDProxy()
+    enc.set_param(sdp, 0.15)
+    dec.set_param(sdp, 0.15)
+    enc.add_ref_feature_from_frame(x_hat_i, True)
+    dec.add_ref_feature_from_frame(x_hat_id, False)
+    bits, ec = enc.compress(_to_gpu(synthetic.frame_to_input(frames[1]), gpu), 32, False, pad_b, pad_r)
+    with pytest.raises(Exception):
+        dec.decompress(np.ascontiguousarray(bits[:3]), 32, H, W, ec, False)
+    enc.add_ref_feature_from_frame(x_hat_i, True)
+    dec.add_ref_feature_from_frame(x_hat_id, False)
+    P16 = (Hp // 16) * (Wp // 16)
+    for t in (2, 3):
+        bits, ec = enc.compress(_to_gpu(synthetic.frame_to_input(frames[t]), gpu), 32, False, pad_b, pad_r)
+        yhat_e = enc.debug_copy(0, P16 * 512 * 2, np.float16).reshape(P16, 512)[:, :128].copy()
+        dec.decompress(bits, 32, H, W, ec, False)
+        torch.cuda.synchronize()
+        yhat_d = dec.debug_copy(0, P16 * 512 * 2, np.float16).reshape(P16, 512)[:, :128]
+        assert np.array_equal(yhat_e.view(np.uint16), yhat_d.view(np.uint16)), f"frame {t}"
