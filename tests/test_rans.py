@@ -315,3 +315,19 @@ def test_escape_only_stream_outgrows_the_first_buffer():
         dec.set_entropy_coder_parallel(parallel)
         dec.set_stream(s)
         assert np.array_equal(dec.decode_y((y.astype(np.int64) & 0xFF).astype(np.uint8)), v.astype(np.int8))
+
+
+def test_zero_frequency_symbol_raises():
+    """A symbol whose cdf row gives it no frequency cannot be coded: the
+    encoder raises instead of renormalising forever."""
+    cdf = np.array([[0, 65535, 65535, 65536]] * 2, np.int32)  # value 1 has no frequency
+    lens = np.full(2, 4, np.int32)
+    enc = entropy.RansEncoder()
+    enc.set_cdf(cdf, lens, 1)
+    enc.reset()
+    enc.encode_y(np.array([(0 << 8) + 1, (-5 << 8) + 0], np.int16))  # codable: 0, and -5 as an escape
+    enc.flush()
+    enc.reset()
+    enc.encode_y(np.array([(0 << 8) + 1, (1 << 8) + 0], np.int16))  # +1 codes value 1
+    with pytest.raises(Exception):
+        enc.flush()
