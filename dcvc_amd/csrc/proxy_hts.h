@@ -64,13 +64,17 @@ private:
     void reset_graphs();
     void enter(hipStream_t caller);
     void leave(hipStream_t caller);
-    void enc_stage0();
+    void enc_stage0();      // encoder, hyper encoder and decoder (main stream)
+    void enc_temporal();    // temporal prior from memory * q_feature (side stream)
+    void enc_stage0b();     // prior fusion, spatial prior steps, compaction
     void enc_stage1(bool reset);
     void dec_stage0(bool memory_has_value);
     void dec_stage1();
     void dec_stage2();
     void dec_stage3();
     void hyper_and_prior();
+    void hyper_decode();
+    void prior_fusion();
     // recon head frames [first, last) on `e`'s stream with the given buffers
     void recon_frames(const Exec& e, int first, int last, f16* common, f16* ra, f16* head);
     void spatial_means(int step);
@@ -89,7 +93,7 @@ private:
     hipEvent_t m_ev_in = nullptr, m_ev_out = nullptr, m_ev_sym = nullptr, m_ev_fork = nullptr, m_ev_join = nullptr;
     bool m_use_graphs = true;
     int m_prec = PREC_F16;
-    Graph m_g_enc0, m_g_enc1[2], m_g_dec0[2], m_g_dec1, m_g_dec2, m_g_dec3, m_g_rh[2];
+    Graph m_g_enc0, m_g_enc0b, m_g_enc_tp, m_g_enc1[2], m_g_dec0[2], m_g_dec1, m_g_dec2, m_g_dec3, m_g_rh[2];
 
     // geometry
     int m_Hp = 0, m_Wp = 0, m_H8 = 0, m_W8 = 0, m_H16 = 0, m_W16 = 0, m_H16p = 0, m_W16p = 0;
