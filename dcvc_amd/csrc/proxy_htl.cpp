@@ -145,6 +145,8 @@ void HTLProxy::reset_graphs() {
     if (m_stream) hipStreamSynchronize(m_stream);
     if (m_side) hipStreamSynchronize(m_side);
     m_g_enc0.reset();
+    m_g_enc0b.reset();
+    m_g_enc_tp.reset();
     for (auto& g : m_g_enc1) g.reset();
     for (auto& g : m_g_dec0) g.reset();
     m_g_dec1.reset();
@@ -290,6 +292,11 @@ void HTLProxy::add_ref_feature_from_frame(const f16* frame, int ld, int Hp, int 
 
 // hyper decoder -> crop -> prior fusion -> common params (shared by both sides)
 void HTLProxy::hyper_and_prior() {
+    hyper_decode();
+    prior_fusion();
+}
+
+void HTLProxy::hyper_decode() {
     const HTLWeights& w = *m_w;
     Exec e = exec();
     const Act zh = make_act(m_z_hat, 1, m_H64, m_W64, 128, 128);
@@ -304,6 +311,11 @@ void HTLProxy::hyper_and_prior() {
     } else {
         w.hd2.forward(e, hdb, hyper);
     }
+}
+
+void HTLProxy::prior_fusion() {
+    const HTLWeights& w = *m_w;
+    Exec e = exec();
     const Act catpf = make_act(m_cat_pf, 1, m_H16, m_W16, 768, 768);
     const Act pfa = make_act(m_pf_a, 1, m_H16, m_W16, 768, 768);
     const Act tmp = make_act(m_scratch.x1, 1, m_H16, m_W16, 768, 768);
@@ -345,10 +357,24 @@ void HTLProxy::enc_stage0() {
     w.he1.forward(e, hea, heb);
     w.he2.forward(e, heb, z);
     hip_check(round_z(m_z, 128, (long)m_H64 * m_W64, 128, m_z_hat, 128, m_z_i8, m_stream), "round_z");
-    hip_check(mul_channel(m_cat_fam, 1024, (long)m_H8 * m_W8, 512, w.q_feature, 512, m_d_qp, m_tin, 512, m_stream),
+    hyper_decode();
+}
+
+// temporal prior from memory * q_feature: needs only the memory and qp, so
+// it runs on the side stream while the encoder and hyper coder run; it
+// writes channels 256-767 of the prior-fusion input, the hyper decoder 0-255
+void HTLProxy::enc_temporal() {
+    const HTLWeights& w = *m_w;
+    Exec e = exec_side();
+    hip_check(mul_channel(m_cat_fam, 1024, (long)m_H8 * m_W8, 512, w.q_feature, 512, m_d_qp, m_tin, 512, m_side),
               "mul");
     w.tpe.forward(e, make_act(m_tin, 1, m_H8, m_W8, 512, 512), make_act(m_cat_pf + 256, 1, m_H16, m_W16, 512, 768));
-    hyper_and_prior();
+}
+
+void HTLProxy::enc_stage0b() {
+    const HTLWeights& w = *m_w;
+    prior_fusion();
+    const Act y = make_act(m_y, 1, m_H16, m_W16, 256, 256);
     for (int step = 0; step < 4; ++step) {
         if (step > 0) spatial_prior(step);
         const int ld = step_ld(step);
@@ -438,7 +464,12 @@ int HTLProxy::compress(const f16* x, int ldx, int H, int W, int qp, bool reset, 
     set_device_int(m_d_qp, qp, m_stream);
     // pad_and_unshuffle_8 is eager: x changes every call (dmc_htl_proxy.cpp:603-604)
     hip_check(pad_unshuffle(x, ldx, 1, H, W, 24, 8, pad_b, pad_r, m_cat_enc, 2048, m_stream), "unshuffle");
+    fork_side();
+    m_g_enc_tp.run(m_side, m_use_graphs, [&] { enc_temporal(); });
+    hip_check(hipEventRecord(m_ev_join, m_side), "event record");
     m_g_enc0.run(m_stream, m_use_graphs, [&] { enc_stage0(); });
+    hip_check(hipStreamWaitEvent(m_stream, m_ev_join, 0), "stream wait");
+    m_g_enc0b.run(m_stream, m_use_graphs, [&] { enc_stage0b(); });
     hip_check(hipEventRecord(m_ev_sym, m_stream), "event");
     // the caller needs nothing from the context update (it only feeds this
     // proxy's next call, ordered on the proxy stream): release the caller's

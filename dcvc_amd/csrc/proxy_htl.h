@@ -61,7 +61,9 @@ private:
     void reset_graphs();
     void enter(hipStream_t caller);
     void leave(hipStream_t caller);
-    void enc_stage0();
+    void enc_stage0();      // encoder, hyper encoder and decoder (main stream)
+    void enc_temporal();    // temporal prior from memory * q_feature (side stream)
+    void enc_stage0b();     // prior fusion, spatial prior steps, compaction
     void enc_stage1(bool reset);
     void dec_stage0(bool memory_has_value);
     void dec_stage1();
@@ -69,6 +71,8 @@ private:
     void dec_index(int step);
     void decode_step_on_host(int step);
     void hyper_and_prior();
+    void hyper_decode();
+    void prior_fusion();
     void spatial_prior(int step);
     void decoder_and_recon(bool recon_all, bool reset_for_encoder);
     // recon head frames [first, last) on `e`'s stream with the given buffers
@@ -94,7 +98,7 @@ private:
     hipEvent_t m_ev_in = nullptr, m_ev_out = nullptr, m_ev_sym = nullptr, m_ev_fork = nullptr, m_ev_join = nullptr;
     bool m_use_graphs = true;
     int m_prec = PREC_F16;
-    Graph m_g_enc0, m_g_enc1[2], m_g_dec0[2], m_g_dec1, m_g_dec[4], m_g_fe, m_g_rh[2];
+    Graph m_g_enc0, m_g_enc0b, m_g_enc_tp, m_g_enc1[2], m_g_dec0[2], m_g_dec1, m_g_dec[4], m_g_fe, m_g_rh[2];
 
     // geometry
     int m_Hp = 0, m_Wp = 0, m_H8 = 0, m_W8 = 0, m_H16 = 0, m_W16 = 0, m_H16p = 0, m_W16p = 0;
