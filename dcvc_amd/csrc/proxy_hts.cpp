@@ -134,6 +134,7 @@ void HTSProxy::reset_graphs() {
     m_g_enc_tp.reset();
     for (auto& g : m_g_enc1) g.reset();
     for (auto& g : m_g_dec0) g.reset();
+    m_g_dec0b.reset();
     m_g_dec1.reset();
     m_g_dec2.reset();
     m_g_dec3.reset();
@@ -489,6 +490,13 @@ void HTSProxy::dec_stage0(bool memory_has_value) {
         w.fa_m.forward(e, make_act(m_cat_fam, 1, m_H8, m_W8, 1024, 1024), mem, tmp);
     else
         w.fa_i.forward(e, make_act(m_feature_i, 1, m_H8, m_W8, 192, 192), mem, tmp);
+}
+
+// the rest of lambda_dec_0: temporal prior from memory * q_feature, on the
+// main stream while the side stream extracts features from the same memory
+void HTSProxy::dec_stage0b() {
+    const HTSWeights& w = *m_w;
+    Exec e = exec();
     hip_check(mul_channel(m_cat_fam, 1024, (long)m_H8 * m_W8, 512, w.q_feature, 512, m_d_qp, m_tin, 512, m_stream),
               "mul");
     w.tpe.forward(e, make_act(m_tin, 1, m_H8, m_W8, 512, 512), make_act(m_cat_pf + 256, 1, m_H16, m_W16, 512, 768));
@@ -541,10 +549,12 @@ void HTSProxy::decompress(const uint8_t* bits, long nbytes, int qp, int height, 
     enter(s);
     set_device_int(m_d_qp, qp, m_stream);
     m_g_dec0[m_memory_has_value ? 1 : 0].run(m_stream, m_use_graphs, [&] { dec_stage0(m_memory_has_value); });
-    // the context from the new memory, on the side stream during dec_1
+    // the context from the new memory, on the side stream during the
+    // temporal prior and dec_1
     fork_side();
     m_g_dec2.run(m_side, m_use_graphs, [&] { dec_stage2(); });
     hip_check(hipEventRecord(m_ev_join, m_side), "event record");
+    m_g_dec0b.run(m_stream, m_use_graphs, [&] { dec_stage0b(); });
     // z decode on the host, concurrent with dec_0 on the GPU
     m_dec.set_parallel(ec_parallel);
     m_dec.set_stream(bits, nbytes);

@@ -68,7 +68,8 @@ private:
     void enc_temporal();    // temporal prior from memory * q_feature (side stream)
     void enc_stage0b();     // prior fusion, spatial prior steps, compaction
     void enc_stage1(bool reset);
-    void dec_stage0(bool memory_has_value);
+    void dec_stage0(bool memory_has_value);  // feature adaptor -> memory
+    void dec_stage0b();                       // temporal prior from the memory
     void dec_stage1();
     void dec_stage2();
     void dec_stage3();
@@ -93,7 +94,7 @@ private:
     hipEvent_t m_ev_in = nullptr, m_ev_out = nullptr, m_ev_sym = nullptr, m_ev_fork = nullptr, m_ev_join = nullptr;
     bool m_use_graphs = true;
     int m_prec = PREC_F16;
-    Graph m_g_enc0, m_g_enc0b, m_g_enc_tp, m_g_enc1[2], m_g_dec0[2], m_g_dec1, m_g_dec2, m_g_dec3, m_g_rh[2];
+    Graph m_g_enc0, m_g_enc0b, m_g_enc_tp, m_g_enc1[2], m_g_dec0[2], m_g_dec0b, m_g_dec1, m_g_dec2, m_g_dec3, m_g_rh[2];
 
     // geometry
     int m_Hp = 0, m_Wp = 0, m_H8 = 0, m_W8 = 0, m_H16 = 0, m_W16 = 0, m_H16p = 0, m_W16p = 0;
