@@ -5,9 +5,11 @@ reference model classes (DMCI.compress/decompress, image_model.py:194-217;
 DMC.add_ref_feature_from_frame/compress/decompress, video_model_ht.py:
 413-450) over a state dict with the reference's keys.  ``encode_sequence``
 and ``decode_sequence`` follow the reference harness
-(test_video.py:166-399): an I frame, then chunks of up to 8 frames padded by
-repetition, feature-memory reset every ``reset_interval`` frames, the
-stream_helper container, PSNR on YUV420 with 6:1:1 weighting.
+(test_video.py:166-399): an I frame, then chunks of up to the codec's frame
+delay (8 for HT-S / HT-L, video_model_ht.py:16; 1 for LD,
+video_model_ld.py:16) padded by repetition, an I frame again every
+``intra_period`` frames, feature-memory reset every ``reset_interval``
+frames, the stream_helper container, PSNR on YUV420 with 6:1:1 weighting.
 """
 from __future__ import annotations
 
@@ -21,8 +23,6 @@ import torch.nn.functional as F
 
 from . import entropy, stream_helper, synthetic, weights
 from .inference_extensions_cuda import DMCHTLProxy, DMCHTSProxy, DMCIProxy, DMCLDProxy
-
-FRAME_DELAY = 8
 
 
 def make_state_dicts(seed=0, perturb_bias=False, video="hts"):
@@ -61,6 +61,7 @@ class IntraCodec:
 
 class VideoCodecHTS:
     PROXY = DMCHTSProxy
+    FRAME_DELAY = 8  # g_frame_delay, video_model_ht.py:16
 
     def __init__(self, state_dict=None, skip_thres=0.0, share_from: "VideoCodecHTS | None" = None):
         self.proxy = self.PROXY()
@@ -93,6 +94,7 @@ class VideoCodecHTL(VideoCodecHTS):
 class VideoCodecLD(VideoCodecHTS):
     """The low-delay DMC (video_model_ld.py:273-308): one frame per call."""
     PROXY = DMCLDProxy
+    FRAME_DELAY = 1  # g_frame_delay, video_model_ld.py:16
 
 
 def video_codec(structure: str):
@@ -126,6 +128,8 @@ class SequenceResult:
     psnr: list = field(default_factory=list)
     enc_time: list = field(default_factory=list)
     dec_time: list = field(default_factory=list)
+    frame_types: list = field(default_factory=list)
+    recon: list = field(default_factory=list)
     frame_num: int = 0
     height: int = 0
     width: int = 0
@@ -134,28 +138,51 @@ class SequenceResult:
         return sum(self.bits) / (self.frame_num * self.height * self.width)
 
 
-def encode_sequence(frames, i_codec, p_codec, qp_i, qp_p, reset_interval=32, device="cuda"):
-    """frames: list of uint8 YUV420 tuples.  Returns SequenceResult with the bitstream."""
+def _is_intra(frame_idx, intra_period, frame_delay):
+    """test_video.py:205-212: frame 0, every frame at intra_period 1, and
+    frames at intra_period * k + 1 (never frame 1) otherwise."""
+    if frame_idx == 0 or intra_period == 1:
+        return True
+    if intra_period > 1 and frame_idx != 1:
+        if intra_period % frame_delay != 0:
+            raise ValueError("intra_period must be a multiple of the frame delay")
+        return frame_idx % intra_period == 1
+    return False
+
+
+def _input_of(frames, frame_delay, device):
+    """get_src_frame (test_video.py:75-122): one frame on 3 channels, or a
+    chunk of frame_delay frames on 3 * frame_delay channels."""
+    if frame_delay == 1 and len(frames) == 1:
+        return to_input(synthetic.frame_to_input(frames[0]), device)
+    return to_input(synthetic.chunk_to_input(frames, frame_delay), device)
+
+
+def encode_sequence(frames, i_codec, p_codec, qp_i, qp_p, reset_interval=32, intra_period=-1, device="cuda"):
+    """frames: list of uint8 YUV420 tuples.  Returns SequenceResult with the
+    bitstream, per-frame bits (the chunk's bits on its first frame) and frame
+    types (0 I, 1 P), as test_video.py:243-250 accounts them."""
     h, w = frames[0][0].shape
     pad_r, pad_b = synthetic.get_padding_size(h, w, 16)
+    delay = p_codec.FRAME_DELAY
     res = SequenceResult(frame_num=len(frames), height=h, width=w)
     out = io.BytesIO()
     sps_helper = stream_helper.SPSHelper()
     idx = 0
     while idx < len(frames):
-        is_intra = idx == 0
-        n = 1 if is_intra else min(FRAME_DELAY, len(frames) - idx)
+        is_intra = _is_intra(idx, intra_period, delay)
+        n = 1 if is_intra else min(delay, len(frames) - idx)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         if is_intra:
-            x = to_input(synthetic.frame_to_input(frames[0]), device)
+            x = to_input(synthetic.frame_to_input(frames[idx]), device)
             enc = i_codec.compress(x, qp_i, pad_b, pad_r)
             p_codec.clear_dpb()
             p_codec.add_ref_feature_from_frame(enc["x_hat"])
             reset, qp = 0, qp_i
         else:
-            reset = 1 if (reset_interval > 0 and (idx + FRAME_DELAY) % reset_interval == 1) else 0
-            x = to_input(synthetic.chunk_to_input(frames[idx:idx + n], FRAME_DELAY), device)
+            reset = 1 if (reset_interval > 0 and (idx + delay) % reset_interval == 1) else 0
+            x = _input_of(frames[idx:idx + n], delay, device)
             enc = p_codec.compress(x, qp_p, reset, pad_b, pad_r)
             qp = qp_p
         sps = {"sps_id": -1, "height": h, "width": w}
@@ -166,15 +193,18 @@ def encode_sequence(frames, i_codec, p_codec, qp_i, qp_p, reset_interval=32, dev
         torch.cuda.synchronize()
         res.enc_time.append(time.perf_counter() - t0)
         res.bits += [nb * 8] + [0] * (n - 1)
+        res.frame_types += [0] if is_intra else [1] * n
         idx += n
     res.bitstream = out.getvalue()
     return res
 
 
-def decode_sequence(bitstream, frames, i_codec, p_codec, res: SequenceResult | None = None):
-    """Decodes the container; PSNR against the source frames.  Returns res."""
+def decode_sequence(bitstream, frames, i_codec, p_codec, res: SequenceResult | None = None, keep_recon=False):
+    """Decodes the container; PSNR against the source frames.  Returns res;
+    with keep_recon, res.recon holds a host copy of every decoded frame."""
     h, w = frames[0][0].shape
     res = res or SequenceResult(frame_num=len(frames), height=h, width=w)
+    delay = p_codec.FRAME_DELAY
     f = io.BytesIO(bitstream)
     sps_helper = stream_helper.SPSHelper()
     done = 0
@@ -191,14 +221,17 @@ def decode_sequence(bitstream, frames, i_codec, p_codec, res: SequenceResult | N
             x_hat = i_codec.decompress(payload, sps, qp, ec_part)["x_hat"]
             p_codec.clear_dpb()
             p_codec.add_ref_feature_from_frame(x_hat, apply_feature_adaptor=False)
-            recon = [x_hat]
             n = 1
         else:
-            recon = p_codec.decompress(payload, sps, qp, ec_part, reset)["x_hat"]
-            n = min(FRAME_DELAY, len(frames) - done)
+            x_hat = p_codec.decompress(payload, sps, qp, ec_part, reset)["x_hat"]
+            n = min(delay, len(frames) - done)
+        recon = x_hat if isinstance(x_hat, (list, tuple)) else [x_hat]
         torch.cuda.synchronize()
         res.dec_time.append(time.perf_counter() - t0)
         for i in range(n):
-            res.psnr.append(yuv420_psnr(recon[i][:, :, :h, :w], frames[done + i])[0])
+            r = recon[i][:, :, :h, :w]
+            res.psnr.append(yuv420_psnr(r, frames[done + i])[0])
+            if keep_recon:
+                res.recon.append(r.float().cpu())
         done += n
     return res

@@ -45,6 +45,10 @@ int guarded(Fn&& fn) {
     } catch (const std::exception& e) {
         set_last_error(e.what());
         return DCVC_ERR_HIP;
+    } catch (...) {
+        // Nothing may unwind through an extern "C" frame.
+        set_last_error("unknown exception");
+        return DCVC_ERR_HIP;
     }
 }
 

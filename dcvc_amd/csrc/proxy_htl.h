@@ -133,6 +133,7 @@ private:
 
     bool m_memory_ready = false;     // encoder side: memory/ctx hold the adaptor output
     bool m_memory_has_value = false; // decoder side (dmc_htl_proxy.cpp:591, 912)
+    bool m_has_ref = false;  // add_ref_feature_from_frame ran since prepare(): m_feature_i holds a frame
     long m_counts[4] = {0, 0, 0, 0};
     int m_qp = 0;
 };

@@ -228,6 +228,7 @@ void HTSProxy::prepare(int Hp, int Wp, f16* const frames[kFrames], int ld_out) {
     m_ld_out = ld_out;
     m_memory_ready = false;
     m_memory_has_value = false;
+    m_has_ref = false;
 }
 
 void HTSProxy::enter(hipStream_t caller) {
@@ -260,6 +261,7 @@ void HTSProxy::add_ref_feature_from_frame(const f16* frame, int ld, int Hp, int 
     }
     m_memory_ready = apply_adaptor;
     m_memory_has_value = apply_adaptor;
+    m_has_ref = true;
     leave(s);
 }
 
@@ -543,7 +545,7 @@ void HTSProxy::decompress(const uint8_t* bits, long nbytes, int qp, int height, 
     if (qp < 0 || qp >= 64) throw std::invalid_argument("qp must be in [0, 64)");
     if ((height + 15) / 16 * 16 != m_Hp || (width + 15) / 16 * 16 != m_Wp)
         throw std::invalid_argument("picture size does not match the prepared resolution");
-    if (!m_memory_ready && !m_memory_has_value && !m_feature_i)
+    if (!m_memory_has_value && !m_has_ref)
         throw std::logic_error("add_ref_feature_from_frame must precede decompress");
     m_qp = qp;
     enter(s);

@@ -129,6 +129,7 @@ private:
 
     bool m_memory_ready = false;     // encoder side: memory/ctx hold the adaptor output
     bool m_memory_has_value = false; // decoder side (dmc_hts_proxy.cpp:617, 707)
+    bool m_has_ref = false;  // add_ref_feature_from_frame ran since prepare(): m_feature_i holds a frame
     long m_last_count = 0;
     int m_qp = 0;
 };

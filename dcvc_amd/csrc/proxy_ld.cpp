@@ -206,6 +206,7 @@ void LDProxy::prepare(int Hp, int Wp, f16* x_hat, int ld_out) {
     m_ld_out = ld_out;
     m_memory_ready = false;
     m_memory_has_value = false;
+    m_has_ref = false;
 }
 
 void LDProxy::enter(hipStream_t caller) {
@@ -254,6 +255,7 @@ void LDProxy::add_ref_feature_from_frame(const f16* frame, int ld, int Hp, int W
     }
     m_memory_ready = apply_adaptor;
     m_memory_has_value = apply_adaptor;
+    m_has_ref = true;
     leave(s);
 }
 
@@ -435,6 +437,8 @@ void LDProxy::decompress(const uint8_t* bits, long nbytes, int qp, int height, i
     if (qp < 0 || qp >= 64) throw std::invalid_argument("qp must be in [0, 64)");
     if ((height + 15) / 16 * 16 != m_Hp || (width + 15) / 16 * 16 != m_Wp)
         throw std::invalid_argument("picture size does not match the prepared resolution");
+    if (!m_memory_has_value && !m_has_ref)
+        throw std::logic_error("add_ref_feature_from_frame must precede decompress");
     m_qp = qp;
     enter(s);
     set_device_int(m_d_qp, qp, m_stream);
