@@ -56,6 +56,7 @@ SIGNATURES = {
     "dcvc_htl_restore_step": [P, I, I, I, I, P, I, P, I, P, I, P, I, I, P],
     "dcvc_compact_blocks": [L],
     "dcvc_conditional_index": [P, I, P, L, P, P, P, P],
+    "dcvc_conditional_index_noalloc": [P, I, P, L, P, P, P],
     "dcvc_conditional_recover": [P, P, L, P, P, P],
     # entropy coder
     "dcvc_last_error": [],

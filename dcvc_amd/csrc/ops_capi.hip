@@ -1,6 +1,10 @@
 // extern "C" kernel-level entry points declared in include/dcvc_ops.h.
 #include "../../include/dcvc_ops.h"
 
+#include <mutex>
+#include <string>
+
+#include "capi_util.h"
 #include "common.h"
 #include "dwconv.h"
 #include "elementwise.h"
@@ -35,6 +39,41 @@ static GemmArgs plain_args(const void* A, int lda, int M, int K, const void* W, 
     g.outH = 1;
     g.outW = M;
     return g;
+}
+
+// Block offsets of dcvc_conditional_index_noalloc: one int per COMPACT_TILE
+// elements, in a file-scope device array so that the entry point allocates
+// nothing.  Each GPU holds its own copy for the life of the process; every
+// call on that GPU shares it.
+constexpr int kNoallocMaxBlocks = 16384;
+constexpr long kNoallocMaxElements = (long)kNoallocMaxBlocks * COMPACT_TILE;  // 67,108,864
+static __device__ __align__(256) int scratch[kNoallocMaxBlocks];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static int* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current.  The host-side name `scratch` is not a
+// device address and is never passed to a kernel.
+static int noalloc_scratch(int** out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        set_last_error("dcvc_conditional_index_noalloc: no current HIP device");
+        return DCVC_ERR_HIP;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_last_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            return DCVC_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = static_cast<int*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return DCVC_OK;
 }
 
 extern "C" {
@@ -420,6 +459,25 @@ int dcvc_conditional_index(const void* vals, int esize, const uint8_t* cond, lon
     if ((esize != 1 && esize != 2) || N < 0 || !count || (N > 0 && (!vals || !cond || !out || !block_offsets)))
         return DCVC_ERR_ARG;
     return st(compact(vals, esize, cond, N, out, block_offsets, count, S(stream)));
+}
+
+int dcvc_conditional_index_noalloc(const void* vals, int esize, const uint8_t* cond, long N, void* out, int* count,
+                                   void* stream) {
+    if ((esize != 1 && esize != 2) || N < 0 || !count || (N > 0 && (!vals || !cond || !out))) {
+        set_last_error("dcvc_conditional_index_noalloc: bad argument");
+        return DCVC_ERR_ARG;
+    }
+    if (N > kNoallocMaxElements) {
+        set_last_error("dcvc_conditional_index_noalloc: N = " + std::to_string(N) + " exceeds the maximum of " +
+                       std::to_string(kNoallocMaxElements) + " elements");
+        return DCVC_ERR_ARG;
+    }
+    int* offs = nullptr;
+    const int r = noalloc_scratch(&offs);
+    if (r != DCVC_OK) return r;
+    // compact() writes every block offset it later reads (count, then scan
+    // in place), so the scratch left by an earlier call needs no clearing.
+    return st(compact(vals, esize, cond, N, out, offs, count, S(stream)));
 }
 
 int dcvc_conditional_recover(const int8_t* dec, const uint8_t* cond, long N, const int* block_offsets, int8_t* out,

@@ -353,6 +353,16 @@ def conditional_index(vals, cond):
     return out[:n], n, offs
 
 
+def conditional_index_noalloc(vals, cond, out, count):
+    """dcvc_conditional_index_noalloc into caller-owned `out` (same size and
+    dtype as vals) and `count` (one int32): the entry point allocates
+    nothing.  Returns the compacted view and the count."""
+    _lib_call("dcvc_conditional_index_noalloc", _p(vals), vals.element_size(), _p(cond), vals.numel(), _p(out),
+              _p(count), _stream())
+    n = int(count.item())
+    return out[:n], n
+
+
 def conditional_recover(dec, cond, offs):
     N = cond.numel()
     out = torch.empty(N, dtype=torch.int8, device=cond.device)

@@ -247,6 +247,16 @@ int dcvc_htl_restore_step(const int8_t* yq, int n_img, int H, int W, int C, cons
 int dcvc_compact_blocks(long N);
 int dcvc_conditional_index(const void* vals, int esize, const uint8_t* cond, long N, void* out,
                            int* block_offsets, int* count, void* stream);
+/* dcvc_conditional_index without the block_offsets workspace: no reference
+ * counterpart.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The block offsets live in a fixed per-GPU device array of 16384 ints, so
+ * N is limited to 16384 * 4096 = 67,108,864 elements (larger N returns 1
+ * with a message in dcvc_last_error()).  Nothing is allocated.  All calls on
+ * one GPU share that array: calls that overlap on one GPU race, and the
+ * offsets of one call are not kept for dcvc_conditional_recover. */
+int dcvc_conditional_index_noalloc(const void* vals, int esize, const uint8_t* cond, long N, void* out,
+                                   int* count, void* stream);
 /* conditional_recover_with_type_conversion_cuda (stream.cu:359-420): dense
  * int8 from compacted decoded symbols, 0 where cond == 0; block_offsets as
  * left by dcvc_conditional_index over the same cond. */
