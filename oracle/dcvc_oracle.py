@@ -36,11 +36,17 @@ mistake; none of them retypes the oracle's formula):
   (encoder, hyper codec, priors, spatial priors, decoder, recon head,
   feature adaptors / extractor): the reference modules' own forward on the
   same seeded weights (golden vectors, test_oracle_golden.py).
-* dmci_compress / hts_compress / hts_decompress (step order, y_hat
-  accumulation, context update): composed of the pinned parts; the HT-S
-  decoder rebuilds the encoder's y_hat and reconstruction exactly from its
-  symbols (test_oracle_pins.py).  The DMCI composition is pinned only
-  through its parts and the GPU encoder/decoder agreement.
+* dmci_compress / dmci_decompress / hts_compress / hts_decompress (step
+  order, y_hat accumulation, context update): composed of the pinned parts;
+  each decoder, fed the encoder's compacted symbols, asks for exactly the
+  encoder's CDF indexes (DMCI: in each of its four steps, quarter view) and
+  rebuilds the encoder's y_hat and reconstruction exactly
+  (test_oracle_pins.py).
+* storage_mode(fp16=True): the same functions with fp16 rounding at the GPU
+  library's storage points.  With the rounding switched off it is the FP32
+  restatement bit for bit (test_oracle_pins.py); where each rounding sits is
+  pinned on the GPU, block by block, by bit-exact agreement of the GPU
+  DepthConvBlock with it on nearly every element (tests/test_gpu_parity.py).
 * hts/intra/htl *_f32 step restatements: agree with the torch masked steps
   on symbols, conditions and y_hat (test_oracle_pins.py, HT-S and HT-L
   forms; the HT-L form covers the DMCI quarter steps it is built on).
@@ -60,6 +66,7 @@ mistake; none of them retypes the oracle's formula):
 """
 from __future__ import annotations
 
+import contextlib
 import math
 from dataclasses import dataclass, field
 
@@ -252,12 +259,55 @@ def _nchw(x):
     return x.clone(memory_format=torch.contiguous_format).as_strided(x.shape, tuple(reversed(std)))
 
 
-def conv(sd, prefix, x, stride=1, padding=0, groups=1):
-    x = _nchw(x)
-    w = sd[prefix + "weight"]
-    b = sd.get(prefix + "bias")
+# ---------------------------------------------------------------------------
+# storage modes
+#
+# The default is the FP32 restatement.  Under ``storage_mode(fp16=True)`` the
+# oracle rounds to fp16 wherever the GPU library stores a kernel's output
+# (``st16``): every GEMM / conv / depthwise output after its fused epilogue,
+# each quantiser's y_hat, and the per-channel multiplies run as their own
+# kernels.  It also folds the depthwise bias into dc.3 in float64, as the
+# library does at load.  Arithmetic inside each kernel stays float32, so this
+# mode isolates what fp16 activation storage alone changes.  Under
+# ``acc64=True`` every convolution accumulates in float64 and rounds once to
+# float32: a second summation order, used to measure how far two
+# implementations that differ only in fp32 summation order drift apart.
+# ---------------------------------------------------------------------------
+
+_MODE = {"fp16": False, "acc64": False}
+
+
+@contextlib.contextmanager
+def storage_mode(fp16=False, acc64=False):
+    old = dict(_MODE)
+    _MODE.update(fp16=bool(fp16), acc64=bool(acc64))
+    try:
+        yield
+    finally:
+        _MODE.update(old)
+
+
+def st16(x):
+    """A stored activation: fp16-rounded under storage_mode(fp16=True)."""
+    return x.half().float() if _MODE["fp16"] else x
+
+
+def _conv2d(x, w, b, stride=1, padding=0, groups=1):
+    if _MODE["acc64"]:
+        return F.conv2d(x.double(), w.double(), None if b is None else b.double(),
+                        stride=stride, padding=padding, groups=groups).to(x.dtype)
     return F.conv2d(x, w.to(x.dtype), None if b is None else b.to(x.dtype),
                     stride=stride, padding=padding, groups=groups)
+
+
+def conv(sd, prefix, x, stride=1, padding=0, groups=1):
+    x = _nchw(x)
+    return _conv2d(x, sd[prefix + "weight"], sd.get(prefix + "bias"), stride=stride, padding=padding, groups=groups)
+
+
+def conv16(sd, prefix, x, stride=1, padding=0, groups=1):
+    """A convolution launched on its own: its output is stored."""
+    return st16(conv(sd, prefix, x, stride=stride, padding=padding, groups=groups))
 
 
 def wsilu(x):
@@ -272,6 +322,8 @@ def dcb(sd, p, x, shortcut=False, quant=None):
     applies in conv1x1_bias_shortcut_with_quant (layers_proxy.cpp:92-94); in
     the Python model it is the ``feature * quant_step`` right after the block.
     """
+    if _MODE["fp16"]:
+        return _dcb_fp16(sd, p, x, shortcut, quant)
     if (p + "adaptor.weight") in sd:
         x = conv(sd, p + "adaptor.", x)
     c = sd[p + "dc.2.weight"].shape[0]
@@ -288,15 +340,41 @@ def dcb(sd, p, x, shortcut=False, quant=None):
     return out
 
 
+def _dcb_fp16(sd, p, x, shortcut, quant):
+    """DepthConvBlock with the GPU library's storage points: the adaptor,
+    dc.0 + WSiLU, the bias-free depthwise conv, dc.3 with the folded bias and
+    the shortcut, the quad GEMM's WSiLU chunk sum, and ffn.2 with its tail
+    each store fp16 (layers_proxy.cpp:71-100, 160-206: the same launches)."""
+    if (p + "adaptor.weight") in sd:
+        x = conv16(sd, p + "adaptor.", x)
+    w_dw = sd[p + "dc.2.weight"]
+    t = st16(wsilu(conv(sd, p + "dc.0.", x)))
+    t = st16(_conv2d(_nchw(t), w_dw, None, padding=1, groups=w_dw.shape[0]))
+    w3 = sd[p + "dc.3.weight"]
+    b3, b_dw = sd.get(p + "dc.3.bias"), sd.get(p + "dc.2.bias")
+    fold = torch.zeros(w3.shape[0], dtype=torch.float64) if b3 is None else b3.double().clone()
+    if b_dw is not None:
+        fold = fold + w3.double()[:, :, 0, 0] @ b_dw.double()
+    out = st16(_conv2d(_nchw(t), w3, fold.float()) + x)
+    h = wsilu(conv(sd, p + "ffn.0.", out))
+    t = st16(h[:, 0::4] + h[:, 1::4] + h[:, 2::4] + h[:, 3::4])
+    o = conv(sd, p + "ffn.2.", t) + out
+    if shortcut:
+        o = o + x
+    if quant is not None:
+        o = o * quant.to(o.dtype).reshape(1, -1, 1, 1)
+    return st16(o)
+
+
 def rbs2(sd, p, x, shortcut):
     """ResidualBlockWithStride2 (layers.py:176-188)."""
-    out = conv(sd, p + "down.", F.pixel_unshuffle(x, 2))
+    out = conv16(sd, p + "down.", F.pixel_unshuffle(x, 2))
     return dcb(sd, p + "conv.", out, shortcut=shortcut)
 
 
 def subpel(sd, p, x):
     """SubpelConv2x with kernel 1 (layers.py:92-103)."""
-    return F.pixel_shuffle(conv(sd, p + "conv.0.", x), 2)
+    return F.pixel_shuffle(conv16(sd, p + "conv.0.", x), 2)
 
 
 def rbu(sd, p, x, shortcut):
@@ -326,11 +404,10 @@ def qsel(sd, key, qp):
 # ---------------------------------------------------------------------------
 
 def dmci_encoder(sd, x_unsh, q):
-    out = dcb(sd, "enc.enc_1.", x_unsh)
-    out = out * q.reshape(1, -1, 1, 1)
+    out = dcb(sd, "enc.enc_1.", x_unsh, quant=q)  # * q_scale_enc in the block tail (image_model.py:66-68)
     for i in range(6):
         out = dcb(sd, f"enc.enc_2.{i}.", out)
-    return conv(sd, "enc.enc_2.6.", out, stride=2, padding=1)
+    return conv16(sd, "enc.enc_2.6.", out, stride=2, padding=1)
 
 
 def dmci_hyper_enc(sd, y):
@@ -348,21 +425,20 @@ def dmci_hyper_dec(sd, z_hat):
 def dmci_prior_fusion(sd, x):
     for i in range(3):
         x = dcb(sd, f"y_prior_fusion.conv.{i}.", x)
-    return conv(sd, "y_prior_fusion.conv.3.", x)
+    return conv16(sd, "y_prior_fusion.conv.3.", x)
 
 
 def dmci_spatial_prior(sd, i, x):
     x = dcb(sd, f"y_spatial_prior_adaptor_{i}.", x)
     for j in range(3):
         x = dcb(sd, f"y_spatial_prior.conv.{j}.", x)
-    return conv(sd, "y_spatial_prior.conv.3.", x)
+    return conv16(sd, "y_spatial_prior.conv.3.", x)
 
 
 def dmci_decoder(sd, y_hat, q):
     out = rbu(sd, "dec.dec_1.0.", y_hat, True)
-    for i in range(1, 13):
-        out = dcb(sd, f"dec.dec_1.{i}.", out)
-    out = out * q.reshape(1, -1, 1, 1)
+    for i in range(1, 13):  # * q_scale_dec in the last block's tail (image_model.py:42-44)
+        out = dcb(sd, f"dec.dec_1.{i}.", out, quant=q if i == 12 else None)
     out = dcb(sd, "dec.dec_2.", out)
     return torch.clamp(F.pixel_shuffle(out, 8), -0.5, 0.5)
 
@@ -394,8 +470,8 @@ def dmci_compress(sd, x, qp, skip_thres=0.0):
     yH, yW = y.shape[2], y.shape[3]
     params = params[:, :, :yH, :yW]
     scales, means = params.chunk(2, 1)
-    y = y * qsel(sd, "q_scale_y_enc", qp).reshape(1, -1, 1, 1)
-    reduced = conv(sd, "y_spatial_prior_reduction.", params)
+    y = st16(y * qsel(sd, "q_scale_y_enc", qp).reshape(1, -1, 1, 1))
+    reduced = conv16(sd, "y_spatial_prior_reduction.", params)
     C = y.shape[1]
     masks = mask_4x(C, yH, yW)
     res = IntraResult(y=y)
@@ -405,6 +481,7 @@ def dmci_compress(sd, x, qp, skip_thres=0.0):
             sp = dmci_spatial_prior(sd, step, torch.cat([y_hat_so_far, reduced], 1))
             scales, means = sp.chunk(2, 1)
         y_q, y_hat = quantize_masked(y, means, scales, masks[step], skip_thres)
+        y_hat = st16(y_hat)
         m = masks[step].to(y.dtype)
         s_hat = scales * m
         q = C // 4
@@ -416,11 +493,54 @@ def dmci_compress(sd, x, qp, skip_thres=0.0):
         res.steps_cond.append(cond)
         res.scales_steps.append(s_w)
         y_hat_so_far = y_hat if y_hat_so_far is None else y_hat_so_far + y_hat
-    y_hat = y_hat_so_far * qsel(sd, "q_scale_y_dec", qp).reshape(1, -1, 1, 1)
+    y_hat = st16(y_hat_so_far * qsel(sd, "q_scale_y_dec", qp).reshape(1, -1, 1, 1))
     res.y_hat = y_hat
     res.z_int8 = z_hat.permute(0, 2, 3, 1).reshape(-1).numpy().astype(np.int8)
     res.x_hat = dmci_decoder(sd, y_hat, qsel(sd, "q_scale_dec", qp))
     return res
+
+
+def dmci_decompress(sd, z_int8, decode_y, qp, height, width, skip_thres=0.0):
+    """DMCIProxy::decompress semantics (dmci_proxy.cpp:423-602).
+
+    ``decode_y(step, indexes_uint8) -> int8 quarter symbols`` is the entropy
+    decoder callback; the encoder wrote steps 3..0 after z, so the decoder
+    reads z, then steps 0..3.  Step k: CDF indexes of the active channel
+    group's scales (single_part_for_reading_4x + build_index_dec, :667-676),
+    the entropy round trip, recover + restore_y_4x(_and_add_inplace)
+    (stream.cu:756-860) into y_hat_so_far, then the spatial prior for the
+    next step.  y_hat = y_hat_so_far * q_scale_y_dec, then the decoder.
+    Returns (x_hat, y_hat, indexes per step).
+    """
+    H16, W16 = (height + 15) // 16, (width + 15) // 16
+    H64, W64 = (H16 + 3) // 4, (W16 + 3) // 4
+    z_hat = torch.from_numpy(z_int8.astype(np.float32)).reshape(1, H64, W64, -1).permute(0, 3, 1, 2).contiguous()
+    params = dmci_prior_fusion(sd, dmci_hyper_dec(sd, z_hat))
+    params = params[:, :, :H16, :W16]
+    scales, means = params.chunk(2, 1)
+    reduced = conv16(sd, "y_spatial_prior_reduction.", params)
+    C = scales.shape[1]
+    q = C // 4
+    masks = mask_4x(C, H16, W16)
+    y_hat_so_far = None
+    seen = []
+    for step in range(4):
+        if step > 0:
+            sp = dmci_spatial_prior(sd, step, torch.cat([y_hat_so_far, reduced], 1))
+            scales, means = sp.chunk(2, 1)
+        m = masks[step].to(scales.dtype)
+        s_hat = scales * m
+        s_w = s_hat[:, :q] + s_hat[:, q:2 * q] + s_hat[:, 2 * q:3 * q] + s_hat[:, 3 * q:]
+        _, idx, cond = build_index(None, s_w, skip_thres)
+        seen.append(compact(idx, cond))
+        decoded = decode_y(step, seen[-1])
+        yq = recover(decoded, cond, idx.size)
+        yq = torch.from_numpy(yq.astype(np.float32)).reshape(1, H16, W16, q).permute(0, 3, 1, 2)
+        yq = yq.to(scales.dtype).contiguous().repeat(1, 4, 1, 1)
+        y_hat = st16((yq + means) * m)
+        y_hat_so_far = y_hat if y_hat_so_far is None else y_hat_so_far + y_hat
+    y_hat = st16(y_hat_so_far * qsel(sd, "q_scale_y_dec", qp).reshape(1, -1, 1, 1))
+    return dmci_decoder(sd, y_hat, qsel(sd, "q_scale_dec", qp)), y_hat, seen
 
 
 # ---------------------------------------------------------------------------
@@ -457,7 +577,7 @@ def hts_recon_head(sd, feature, only_reset=False):
         out = common
         for j in range(3):
             out = dcb(sd, f"recon_head.conv2.{i}.{j}.", out)
-        out = conv(sd, f"recon_head.conv2.{i}.3.", out)
+        out = conv16(sd, f"recon_head.conv2.{i}.3.", out)
         heads.append(out)
         outs.append(torch.clamp(F.pixel_shuffle(out, 8), -0.5, 0.5))
     return outs, heads[-1]
@@ -501,7 +621,7 @@ def hts_encoder(sd, x_padded, ctx, qp):
     """Encoder (video_model_ht.py:63-92) with the quant tail fused in the last block."""
     feat = torch.cat([F.pixel_unshuffle(x_padded, 8), ctx], 1)
     feat = seq(sd, "encoder.conv1.", 6, feat, quant_last=qsel(sd, "q_encoder", qp))
-    return conv(sd, "encoder.down.", feat, stride=2, padding=1)
+    return conv16(sd, "encoder.down.", feat, stride=2, padding=1)
 
 
 def hts_hyper_encoder(sd, y_pad):
@@ -520,7 +640,7 @@ def hts_hyper_decoder(sd, z_hat):
 
 def hts_temporal_prior(sd, memory, qp):
     """TemporalPriorEncoder (video_model_ht.py:307-317)."""
-    return rbs2(sd, "temporal_prior_encoder.conv.", memory * qsel(sd, "q_feature", qp).reshape(1, -1, 1, 1), False)
+    return rbs2(sd, "temporal_prior_encoder.conv.", st16(memory * qsel(sd, "q_feature", qp).reshape(1, -1, 1, 1)), False)
 
 
 def _hts_prior(sd, y, z_hat, memory, qp):
@@ -530,18 +650,18 @@ def _hts_prior(sd, y, z_hat, memory, qp):
     x = torch.cat([hyper, temporal], 1)
     for i in range(3):
         x = dcb(sd, f"y_prior_fusion.conv.{i}.", x)
-    return conv(sd, "y_prior_fusion.conv.3.", x)
+    return conv16(sd, "y_prior_fusion.conv.3.", x)
 
 
 def _hts_spatial_prior(sd, i, y_hat, reduced):
     x = dcb(sd, f"y_spatial_prior_adaptor_{i}.", torch.cat([y_hat, reduced], 1))
     for j in range(3):
         x = dcb(sd, f"y_spatial_prior.conv.{j}.", x)
-    return conv(sd, "y_spatial_prior.conv.3.", x)
+    return conv16(sd, "y_spatial_prior.conv.3.", x)
 
 
 def _hts_decoder(sd, y_hat, ctx, qp):
-    up = F.pixel_shuffle(conv(sd, "decoder.up.conv.0.", y_hat), 2)
+    up = F.pixel_shuffle(conv16(sd, "decoder.up.conv.0.", y_hat), 2)
     return seq(sd, "decoder.conv1.", 7, torch.cat([up, ctx], 1), quant_last=qsel(sd, "q_decoder", qp))
 
 
@@ -566,7 +686,7 @@ def hts_compress(sd, st: HTSState, x, qp, reset_feature_memory, skip_thres=0.0, 
     q_dec, scales, means = params.chunk(3, 1)
     q_dec = torch.clamp_min(q_dec, 0.5)
     y = y * (1.0 / q_dec)
-    reduced = conv(sd, "y_spatial_prior_reduction.", params)
+    reduced = conv16(sd, "y_spatial_prior_reduction.", params)
     C, yH, yW = y.shape[1], y.shape[2], y.shape[3]
     masks = mask_4x(C, yH, yW)
     y_q_tot = torch.zeros_like(y)
@@ -576,8 +696,8 @@ def hts_compress(sd, st: HTSState, x, qp, reset_feature_memory, skip_thres=0.0, 
             means = _hts_spatial_prior(sd, step, y_hat, reduced)
         y_q, yh = quantize_masked(y, means, scales, masks[step], skip_thres)
         y_q_tot = y_q_tot + y_q
-        y_hat = y_hat + yh
-    y_hat = y_hat * q_dec
+        y_hat = y_hat + st16(yh)
+    y_hat = st16(y_hat * q_dec)
     sym16, _, cond = build_index(y_q_tot, scales, skip_thres)
     res = HTSResult(sym16=compact(sym16, cond), y=y, y_q=y_q_tot, y_hat=y_hat, scales=scales,
                     common_params=params,
@@ -621,14 +741,14 @@ def hts_decompress(sd, st: HTSState, z_int8, decode_y, qp, height, width, reset_
     y_q = recover(decoded, cond, idx.size)
     y_q = torch.from_numpy(y_q.astype(np.float32)).reshape(1, H16, W16, C).permute(0, 3, 1, 2)
     y_q = y_q.to(params.dtype).contiguous()
-    reduced = conv(sd, "y_spatial_prior_reduction.", params)
+    reduced = conv16(sd, "y_spatial_prior_reduction.", params)
     masks = mask_4x(C, H16, W16)
     y_hat = torch.zeros_like(y_q)
     for step in range(4):
         if step > 0:
             means = _hts_spatial_prior(sd, step, y_hat, reduced)
-        y_hat = y_hat + (y_q + means) * masks[step].to(y_q.dtype)
-    y_hat = y_hat * q_dec
+        y_hat = y_hat + st16((y_q + means) * masks[step].to(y_q.dtype))
+    y_hat = st16(y_hat * q_dec)
     st.feature_p = _hts_decoder(sd, y_hat, st.ctx, qp)
     x_hat, head = hts_recon_head(sd, st.feature_p)
     st.feature_i = head
@@ -664,7 +784,7 @@ def htl_recon_head(sd, feature, only_reset=False):
     outs, heads = [], []
     for i in ([7] if only_reset else range(8)):
         out = seq(sd, f"recon_head.conv.{i}.", 5, feature)
-        out = conv(sd, f"recon_head.conv.{i}.5.", out)
+        out = conv16(sd, f"recon_head.conv.{i}.5.", out)
         heads.append(out)
         outs.append(torch.clamp(F.pixel_shuffle(out, 8), -0.5, 0.5))
     return outs, heads[-1]
@@ -685,8 +805,8 @@ def htl_encoder(sd, x_padded, ctx, qp):
     separate multiply (dmc_htl_proxy.cpp:93-106), 3x3 stride-2 down conv."""
     feat = torch.cat([F.pixel_unshuffle(x_padded, 8), ctx], 1)
     feat = seq(sd, "encoder.conv1.", 7, feat)
-    feat = feat * qsel(sd, "q_encoder", qp).reshape(1, -1, 1, 1)
-    return conv(sd, "encoder.down.", feat, stride=2, padding=1)
+    feat = st16(feat * qsel(sd, "q_encoder", qp).reshape(1, -1, 1, 1))
+    return conv16(sd, "encoder.down.", feat, stride=2, padding=1)
 
 
 def htl_hyper_encoder(sd, y_pad):
@@ -705,7 +825,7 @@ def htl_hyper_decoder(sd, z_hat):
 
 def htl_temporal_prior(sd, memory, qp):
     """TemporalPriorEncoder, HT-L: shortcut stride-2 block (:307-317)."""
-    return rbs2(sd, "temporal_prior_encoder.conv.", memory * qsel(sd, "q_feature", qp).reshape(1, -1, 1, 1), True)
+    return rbs2(sd, "temporal_prior_encoder.conv.", st16(memory * qsel(sd, "q_feature", qp).reshape(1, -1, 1, 1)), True)
 
 
 def _htl_prior(sd, z_hat, memory, qp):
@@ -715,7 +835,7 @@ def _htl_prior(sd, z_hat, memory, qp):
     x = torch.cat([hyper, temporal], 1)
     for i in range(3):
         x = dcb(sd, f"y_prior_fusion.conv.{i}.", x)
-    return conv(sd, "y_prior_fusion.conv.3.", x)
+    return conv16(sd, "y_prior_fusion.conv.3.", x)
 
 
 def _htl_spatial_prior(sd, i, y_hat_so_far, reduced):
@@ -724,15 +844,15 @@ def _htl_spatial_prior(sd, i, y_hat_so_far, reduced):
     x = dcb(sd, f"y_spatial_prior_adaptor_{i}.", torch.cat([y_hat_so_far, reduced], 1))
     for j in range(3):
         x = dcb(sd, f"y_spatial_prior.conv.{j}.", x)
-    return conv(sd, "y_spatial_prior.conv.3.", x).chunk(2, 1)
+    return conv16(sd, "y_spatial_prior.conv.3.", x).chunk(2, 1)
 
 
 def _htl_decoder(sd, y_hat, ctx, qp):
     """Decoder, HT-L: 3x3 biased sub-pixel up conv, 11 blocks, * q_decoder
     as a separate multiply (dmc_htl_proxy.cpp:22-42)."""
-    up = F.pixel_shuffle(conv(sd, "decoder.up.conv.0.", y_hat, padding=1), 2)
+    up = F.pixel_shuffle(conv16(sd, "decoder.up.conv.0.", y_hat, padding=1), 2)
     feat = seq(sd, "decoder.conv1.", 11, torch.cat([up, ctx], 1))
-    return feat * qsel(sd, "q_decoder", qp).reshape(1, -1, 1, 1)
+    return st16(feat * qsel(sd, "q_decoder", qp).reshape(1, -1, 1, 1))
 
 
 def _quarter(x, mask):
@@ -789,7 +909,7 @@ def htl_code_y(sd, y, params, skip_thres=0.0):
     q_dec, scales, means = params.chunk(3, 1)
     q_dec = torch.clamp_min(q_dec, 0.5)
     y = y * (1.0 / q_dec)
-    reduced = conv(sd, "y_spatial_prior_reduction.", params)
+    reduced = conv16(sd, "y_spatial_prior_reduction.", params)
     C, yH, yW = y.shape[1], y.shape[2], y.shape[3]
     masks = mask_4x(C, yH, yW)
     res = HTLResult(y=y, common_params=params)
@@ -798,6 +918,7 @@ def htl_code_y(sd, y, params, skip_thres=0.0):
         if step > 0:
             scales, means = _htl_spatial_prior(sd, step, so_far, reduced)
         y_q, yh = quantize_masked(y, means, scales, masks[step], skip_thres)
+        yh = st16(yh)
         sym16, _, cond = build_index(_quarter(y_q, masks[step]), _quarter(scales, masks[step]), skip_thres)
         res.steps_sym16.append(compact(sym16, cond))
         res.steps_dense.append(sym16)
@@ -805,7 +926,7 @@ def htl_code_y(sd, y, params, skip_thres=0.0):
         if step < 3:
             so_far = so_far + yh
         else:
-            res.y_hat = (yh + so_far) * q_dec
+            res.y_hat = st16((yh + so_far) * q_dec)
     return res
 
 
@@ -825,7 +946,7 @@ def htl_decompress(sd, st: HTSState, z_int8, decode_y, qp, height, width, reset_
     st.ctx = htl_feature_extractor(sd, st.memory)
     q_dec, scales, means = params.chunk(3, 1)
     q_dec = torch.clamp_min(q_dec, 0.5)
-    reduced = conv(sd, "y_spatial_prior_reduction.", params)
+    reduced = conv16(sd, "y_spatial_prior_reduction.", params)
     C = scales.shape[1]
     masks = mask_4x(C, H16, W16)
     so_far = torch.zeros((1, C, H16, W16), dtype=params.dtype)
@@ -838,11 +959,11 @@ def htl_decompress(sd, st: HTSState, z_int8, decode_y, qp, height, width, reset_
         yq = recover(decoded, cond, idx.size)
         yq = torch.from_numpy(yq.astype(np.float32)).reshape(1, H16, W16, C // 4).permute(0, 3, 1, 2)
         yq = yq.to(params.dtype).contiguous().repeat(1, 4, 1, 1)
-        yh = (yq + means) * masks[step].to(params.dtype)
+        yh = st16((yq + means) * masks[step].to(params.dtype))
         if step < 3:
             so_far = so_far + yh
         else:
-            y_hat = (yh + so_far) * q_dec
+            y_hat = st16((yh + so_far) * q_dec)
     st.feature_p = _htl_decoder(sd, y_hat, st.ctx, qp)
     x_hat, head = htl_recon_head(sd, st.feature_p)
     st.feature_i = head
@@ -908,7 +1029,7 @@ def ld_recon_head(sd, feature):
     """ReconHead, LD (:151-166): 3 blocks, 1x1 to 192, pixel_shuffle(8) with
     the proxy's clamp; also returns the head output (the reset feature)."""
     out = seq(sd, "recon_head.conv.", 3, feature)
-    head = conv(sd, "recon_head.head.", out)
+    head = conv16(sd, "recon_head.head.", out)
     return torch.clamp(F.pixel_shuffle(head, 8), -0.5, 0.5), head
 
 
@@ -930,7 +1051,7 @@ def ld_encoder(sd, x_padded, ctx, qp):
     feat = torch.cat([F.pixel_unshuffle(x_padded, 8), ctx], 1)
     feat = seq(sd, "encoder.conv1.", 2, feat)
     feat = dcb(sd, "encoder.conv2.", feat, quant=qsel(sd, "q_encoder", qp))
-    return conv(sd, "encoder.down.", feat, stride=2, padding=1)
+    return conv16(sd, "encoder.down.", feat, stride=2, padding=1)
 
 
 def ld_hyper_encoder(sd, y_pad):
@@ -956,10 +1077,10 @@ def _ld_prior(sd, z_hat, temporal, qp):
     """PriorFusion over cat(hyper, temporal * q_feature) (:137-148)."""
     hyper = ld_hyper_decoder(sd, z_hat)
     hyper = hyper[:, :, :temporal.shape[2], :temporal.shape[3]]
-    x = torch.cat([hyper, temporal * qsel(sd, "q_feature", qp).reshape(1, -1, 1, 1)], 1)
+    x = torch.cat([hyper, st16(temporal * qsel(sd, "q_feature", qp).reshape(1, -1, 1, 1))], 1)
     for i in range(3):
         x = dcb(sd, f"y_prior_fusion.conv.{i}.", x)
-    return conv(sd, "y_prior_fusion.conv.3.", x)
+    return conv16(sd, "y_prior_fusion.conv.3.", x)
 
 
 def ld_spatial_prior(sd, y_hat0, params):
@@ -968,7 +1089,7 @@ def ld_spatial_prior(sd, y_hat0, params):
     x = torch.cat([y_hat0, params], 1)
     x = dcb(sd, "y_spatial_prior.conv.0.", x)
     x = dcb(sd, "y_spatial_prior.conv.1.", x)
-    return conv(sd, "y_spatial_prior.conv.2.", x)
+    return conv16(sd, "y_spatial_prior.conv.2.", x)
 
 
 def _ld_decoder(sd, y_hat, ctx, qp):
@@ -976,7 +1097,7 @@ def _ld_decoder(sd, y_hat, ctx, qp):
     the q_decoder multiply fused (conv1x1_bias_with_quant, dmc_ld_proxy.cpp:27-35)."""
     up = subpel(sd, "decoder.up.", y_hat)
     feat = seq(sd, "decoder.conv1.", 3, torch.cat([up, ctx], 1))
-    return conv(sd, "decoder.conv2.", feat) * qsel(sd, "q_decoder", qp).reshape(1, -1, 1, 1)
+    return st16(conv(sd, "decoder.conv2.", feat) * qsel(sd, "q_decoder", qp).reshape(1, -1, 1, 1))
 
 
 def ld_code_y(sd, y, params, skip_thres=0.0):
@@ -991,11 +1112,13 @@ def ld_code_y(sd, y, params, skip_thres=0.0):
     C, yH, yW = y.shape[1], y.shape[2], y.shape[3]
     masks = mask_2x(C, yH, yW)
     y_q0, yh0 = quantize_masked(y, means, scales, masks[0], skip_thres)
+    yh0 = st16(yh0)
     means1 = ld_spatial_prior(sd, yh0, params)
     y_q1, yh1 = quantize_masked(y, means1, scales, masks[1], skip_thres)
+    yh1 = st16(yh1)
     sym16, _, cond = build_index(y_q0 + y_q1, scales, skip_thres)
     res = LDResult(sym16=compact(sym16, cond), sym16_dense=sym16, cond_dense=cond, y=y, common_params=params)
-    res.y_hat = (yh0 + yh1) * q_dec
+    res.y_hat = st16((yh0 + yh1) * q_dec)
     return res
 
 
@@ -1045,9 +1168,9 @@ def ld_decompress(sd, st: LDState, z_int8, decode_y, qp, height, width, reset_fe
     y_q = torch.from_numpy(y_q.astype(np.float32)).reshape(1, H16, W16, C).permute(0, 3, 1, 2)
     y_q = y_q.to(params.dtype).contiguous()
     masks = mask_2x(C, H16, W16)
-    yh0 = (y_q + means) * masks[0].to(y_q.dtype)
+    yh0 = st16((y_q + means) * masks[0].to(y_q.dtype))
     means1 = ld_spatial_prior(sd, yh0, params)
-    y_hat = (yh0 + (y_q + means1) * masks[1].to(y_q.dtype)) * q_dec
+    y_hat = st16((yh0 + st16((y_q + means1) * masks[1].to(y_q.dtype))) * q_dec)
     st.feature_p = _ld_decoder(sd, y_hat, st.ctx, qp)
     x_hat, head = ld_recon_head(sd, st.feature_p)
     st.feature_i = head

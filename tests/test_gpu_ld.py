@@ -132,9 +132,10 @@ def test_ld_symbols_and_psnr_vs_oracle(gpu, models):
     # indexes 94.4 %; conditioned 99.91 % / 99.13 %; PSNR within 0.001 dB
     assert y_rel < 4e-3 and z_match > 0.99
     assert sym_c > 0.995 and idx_c > 0.95
-    assert sym_m > 0.98 and idx_m > 0.85
-    assert abs(len(bits) - n_o) / n_o < 0.01
-    assert abs(pg - po) < 0.05
+    # end to end, symbols / indexes are judged against the fp16 storage
+    # floor in tests/test_gpu_parity.py
+    assert abs(len(bits) - n_o) <= 0.005 * n_o
+    assert abs(pg - po) <= 0.02
 
 
 def test_ld_decoder_recovers_after_a_corrupt_bitstream(gpu, models):

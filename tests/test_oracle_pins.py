@@ -288,3 +288,61 @@ def test_ld_fp32_step_restatement_agrees_with_torch_steps():
     assert np.array_equal(cond.reshape(-1).astype(bool), cond_t)
     ref = (yh_t * to_nchw(q32)).permute(0, 2, 3, 1).reshape(P, C).numpy()
     np.testing.assert_allclose(yh.astype(np.float32), ref, rtol=2 ** -10, atol=2 ** -14)
+
+
+def _intra_round_trip(sd, frame, qp, thr):
+    r = O.dmci_compress(sd, frame, qp, skip_thres=thr)
+
+    def decode_y(step, idx):
+        assert np.array_equal(idx, (r.steps_sym16[step].astype(np.int64) & 0xFF).astype(np.uint8)), step
+        return (r.steps_sym16[step].astype(np.int64) >> 8).astype(np.int8)
+
+    H, W = frame.shape[2], frame.shape[3]
+    x_hat, y_hat, seen = O.dmci_decompress(sd, r.z_int8, decode_y, qp, H, W, skip_thres=thr)
+    return r, x_hat, y_hat, seen
+
+
+@pytest.mark.parametrize("H,W,thr", [(64, 64, 0.0), (80, 112, 0.15)])
+def test_oracle_dmci_encode_decode_round_trip(H, W, thr):
+    """DMCI: four entropy round trips over quarter tensors.  The oracle
+    decoder asks for exactly the encoder's CDF indexes in each step (steps
+    0..3 after z, the reverse of the encoder's write order) and rebuilds
+    y_hat and the reconstruction exactly; the four steps' streams partition
+    y (C/4 channels per pixel each), in FP32 and in fp16-storage mode."""
+    sd = {k: v.float() for k, v in weights.to_torch(weights.round_to_half(weights.init_state_dict("dmci", 3))).items()}
+    frame = torch.from_numpy(synthetic.frame_to_input(synthetic.texture_yuv420(H, W, 4)))[None]
+    frame = F.pad(frame, (0, (16 - W % 16) % 16, 0, (16 - H % 16) % 16), mode="replicate")
+    for fp16 in (False, True):
+        with torch.inference_mode(), O.storage_mode(fp16=fp16):
+            r, x_hat, y_hat, seen = _intra_round_trip(sd, frame, 27, thr)
+        assert len(seen) == 4
+        assert sum(len(s) for s in seen) == sum(len(s) for s in r.steps_sym16)
+        for step in range(4):
+            assert r.steps_dense[step].size == r.y.numel() // 4
+        assert torch.equal(y_hat, r.y_hat)
+        assert torch.equal(x_hat, r.x_hat)
+        if fp16:
+            assert torch.equal(y_hat, y_hat.half().float()) and torch.equal(x_hat, x_hat.half().float())
+
+
+def test_fp16_storage_mode_is_the_fp32_restatement_without_rounding(monkeypatch):
+    """storage_mode(fp16=True) with st16 made the identity reproduces the
+    FP32 oracle bit for bit: the mode changes only where values are stored
+    (and the depthwise-bias fold, an exact identity in real arithmetic,
+    which moves the fp32 result by rounding only)."""
+    sd = {k: v.float() for k, v in weights.to_torch(weights.round_to_half(weights.init_state_dict("hts", 1))).items()}
+    x = torch.from_numpy(np.random.Generator(np.random.PCG64(5)).standard_normal((1, 512, 6, 10)).astype(np.float32))
+    with torch.inference_mode():
+        ref = O.dcb(sd, "feature_extractor.conv.0.", x)
+        monkeypatch.setattr(O, "st16", lambda t: t)
+        with O.storage_mode(fp16=True):
+            got = O.dcb(sd, "feature_extractor.conv.0.", x)
+        assert torch.allclose(got, ref, rtol=1e-5, atol=1e-5)
+        monkeypatch.undo()
+        with O.storage_mode(fp16=True):
+            got16 = O.dcb(sd, "feature_extractor.conv.0.", x)
+    # every stored value is an fp16 value, and the rounding moves the block
+    # by about the fp16 resolution, not more
+    assert torch.equal(got16, got16.half().float())
+    rel = float((got16 - ref).abs().max() / ref.abs().max())
+    assert 1e-5 < rel < 2e-2, rel
