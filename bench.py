@@ -17,9 +17,12 @@ FLOPs / bytes, each launch timed with HIP events on its own stream), the
 roofline of the kernel class that takes the most time, and the FP32 CPU
 oracle timed on this host.
 
---profile-step runs the warm-up and one step, then a 256 MiB device copy
-(the calibration kernel for the FETCH_SIZE / WRITE_SIZE counters), and
-prints nothing else: the program to put under rocprofv3.
+--profile-step runs the warm-up, then --profile-steps timed steps between
+two marker launches (torch's spin kernel, used nowhere else), then a 256 MiB
+device copy (the calibration kernel for the FETCH_SIZE / WRITE_SIZE
+counters), and prints nothing else: the program to put under rocprofv3.
+`python -m dcvc_amd.profiling` keeps only the dispatches between the last
+two markers, so the summaries cover the timed steps alone.
 """
 from __future__ import annotations
 
@@ -53,7 +56,7 @@ def layer_trace(enc, dec, x, qp, pad_b, pad_r, H, W):
     import torch
     from dcvc_amd import _lib
     lib = _lib.load()
-    buf = np.zeros((50000, 8), np.int64)
+    buf = np.zeros((50000, 15), np.int64)
     ptr = buf.ctypes.data_as(C.c_void_p)
     enc.proxy.set_use_graphs(False)
     dec.proxy.set_use_graphs(False)
@@ -92,27 +95,69 @@ def kernel_classes(recs):
     return dict(sorted(out.items(), key=lambda kv: -kv[1]["time_ms"]))
 
 
-def roofline_of(classes):
-    """Roofline object for the class with the largest total time: MFMA-bound
-    when its algorithmic intensity is above the FP16 ridge, HBM-bound below."""
-    name, c = next(iter(classes.items()))
+# tile configuration id -> Tile<BM, BN, WM, WN, STAGES> of the fp16 dispatch (gemm.hip)
+TILES_QUAD = {1: (256, 256, 2, 2, 5), 2: (128, 256, 2, 2, 5), 3: (128, 128, 4, 1, 5), 4: (128, 256, 2, 2, 3),
+              5: (256, 256, 2, 2, 4), 6: (256, 256, 4, 2, 4), 7: (256, 256, 4, 2, 3)}
+TILES = {1: (256, 256, 2, 2, 5), 2: (256, 128, 2, 2, 5), 3: (128, 128, 2, 2, 5), 4: (256, 128, 2, 2, 3),
+         5: (128, 256, 2, 2, 3), 6: (256, 256, 4, 2, 4), 7: (256, 256, 4, 2, 3)}
+
+
+def instance_name(r):
+    """Kernel instantiation of a trace record, named as the rocprof summaries
+    name it (dcvc_amd.profiling.short_name)."""
+    kind, epi, amode, dmode, prec, cfg = (int(r[i]) for i in (0, 1, 8, 9, 10, 11))
+    if kind == 3:
+        return f"dw3x3_kernel<{cfg}>"
+    if kind == 4 or prec != 0:
+        return f"{KIND_NAMES[kind]}_{EPI_NAMES[epi]}"
+    t = (TILES_QUAD if epi == 6 else TILES)[cfg]
+    return "gemm_kernel<" + ",".join(str(v) for v in (*t, epi, amode, dmode, prec)) + ">"
+
+
+def kernel_instances(recs):
+    """Aggregate trace records per kernel instantiation."""
+    out = {}
+    names = [instance_name(r) for r in recs]
+    for n in sorted(set(names)):
+        r = recs[np.array([x == n for x in names])]
+        t = float(r[:, 7].sum()) * 1e-9
+        k0 = r[0]
+        out[n] = {"class": KIND_NAMES[int(k0[0])] + ("" if k0[0] == 3 else "_" + EPI_NAMES[int(k0[1])]),
+                  "launches": int(len(r)), "time_ms": round(t * 1e3, 4),
+                  "tflops": round(float(r[:, 5].sum()) / t / 1e12, 2) if t > 0 else None,
+                  "gbs": round(float(r[:, 6].sum()) / t / 1e9, 1) if t > 0 else None,
+                  "flops_per_launch": int(r[:, 5].sum() // len(r)), "bytes_per_launch": int(r[:, 6].sum() // len(r)),
+                  "avg_launch_ms": round(t * 1e3 / len(r), 5)}
+    return dict(sorted(out.items(), key=lambda kv: -kv[1]["time_ms"]))
+
+
+def roofline_of(instances, classes):
+    """Roofline object for the kernel instantiation with the largest total
+    time: MFMA-bound when its algorithmic intensity is above the FP16 ridge,
+    HBM-bound below; the HBM-bound classes are listed beside it."""
+    name, c = next(iter(instances.items()))
     ridge = PEAK_FP16_TFLOPS * 1e12 / (PEAK_HBM_GBS * 1e9)
     intensity = c["flops_per_launch"] / max(c["bytes_per_launch"], 1)
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
-        if t.get("kernel_class") == name:
+        if t.get("kernel") == name:
             traffic = t.get("bytes_per_launch")
+    common = {"kernel": name, "kernel_class": c["class"], "traffic": traffic, "avg_launch_ms": c["avg_launch_ms"],
+              "launches_per_step": c["launches"], "flops_per_launch": c["flops_per_launch"],
+              "bytes_per_launch": c["bytes_per_launch"]}
     if intensity >= ridge and c["tflops"] is not None:
-        return {"bound": "mfma", "kernel_class": name, "achieved": c["tflops"], "peak": PEAK_FP16_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(c["tflops"] / PEAK_FP16_TFLOPS, 4), "traffic": traffic,
-                "avg_launch_ms": c["avg_launch_ms"], "launches_per_step": c["launches"],
-                "flops_per_launch": c["flops_per_launch"], "bytes_per_launch": c["bytes_per_launch"]}
-    return {"bound": "hbm", "kernel_class": name, "achieved": c["gbs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": round(c["gbs"] / PEAK_HBM_GBS, 4) if c["gbs"] else None, "traffic": traffic,
-            "avg_launch_ms": c["avg_launch_ms"], "launches_per_step": c["launches"],
-            "flops_per_launch": c["flops_per_launch"], "bytes_per_launch": c["bytes_per_launch"]}
+        out = {"bound": "mfma", "achieved": c["tflops"], "peak": PEAK_FP16_TFLOPS, "unit": "TFLOP/s",
+               "frac": round(c["tflops"] / PEAK_FP16_TFLOPS, 4), **common}
+    else:
+        out = {"bound": "hbm", "achieved": c["gbs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
+               "frac": round(c["gbs"] / PEAK_HBM_GBS, 4) if c["gbs"] else None, **common}
+    out["hbm_bound_classes"] = {
+        n: {"achieved_gbs": v["gbs"], "frac": round(v["gbs"] / PEAK_HBM_GBS, 4), "time_ms": v["time_ms"]}
+        for n, v in classes.items()
+        if v["gbs"] and v["flops_per_launch"] / max(v["bytes_per_launch"], 1) < ridge}
+    return out
 
 
 def precision_forms(sdp, p_enc_ref, x_hat_i, chunks, H, W, pad_b, pad_r, skip, dev, steps=30):
@@ -243,6 +288,50 @@ def htl_form(x_hat_e, x_hat_d, chunks, H, W, pad_b, pad_r, skip, qp, steps=40, w
     return out
 
 
+def tuning_ab(sdp, x_hat_e, x_hat_d, chunks, H, W, pad_b, pad_r, skip, qp, steps=40, warmup=4, rounds=2):
+    """The same HT-S step with the autotuner's tile database and with the
+    built-in heuristics, on fresh codec pairs (graphs are captured with the
+    table in force at their first use), alternated `rounds` times."""
+    import torch
+    from dcvc_amd import codec, tuning
+    if not tuning.DB_PATH.exists():
+        return {"database": None}
+    with open(tuning.DB_PATH) as f:
+        db = json.load(f)
+    res = {"tuned": [], "heuristic": []}
+    n = len(chunks)
+    try:
+        for _ in range(rounds):
+            for mode in ("heuristic", "tuned"):
+                if mode == "tuned":
+                    tuning.install(db)
+                else:
+                    tuning.clear()
+                e = codec.VideoCodecHTS(sdp, skip)
+                d = codec.VideoCodecHTS(share_from=e)
+                e.add_ref_feature_from_frame(x_hat_e)
+                d.add_ref_feature_from_frame(x_hat_d, apply_feature_adaptor=False)
+                for i in range(warmup):
+                    bits, ec = e.proxy.compress(chunks[i % n], qp, False, pad_b, pad_r)
+                    d.proxy.decompress(bits, qp, H, W, ec, False)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for i in range(steps):
+                    bits, ec = e.proxy.compress(chunks[i % n], qp, False, pad_b, pad_r)
+                    d.proxy.decompress(bits, qp, H, W, ec, False)
+                torch.cuda.synchronize()
+                res[mode].append(round(1000 * (time.perf_counter() - t0) / steps, 3))
+                del e, d
+    finally:
+        tuning.install(db)
+    summ = tuning.summary(db)
+    return {"database": os.path.relpath(str(tuning.DB_PATH), ROOT), "gemm_shapes": summ["gemm"]["shapes"],
+            "gemm_changed": summ["gemm"]["changed"], "d3x3_shapes": summ["d3x3"]["shapes"],
+            "d3x3_changed": summ["d3x3"]["changed"], "ms_per_step_tuned": res["tuned"],
+            "ms_per_step_heuristic": res["heuristic"],
+            "speedup": round(float(np.mean(res["heuristic"])) / float(np.mean(res["tuned"])), 4)}
+
+
 def skip0_form(sdp, x_hat_e, x_hat_d, chunks, H, W, pad_b, pad_r, qp, steps=40, warmup=3):
     """HT-S on the same stream with skip_thres 0, the reference's default:
     every y symbol is coded, so the encoder hands all of y to the CPU coder
@@ -352,41 +441,39 @@ def serving_form(sdi, sdp, duration=10.0, rate=120.0, n_streams=64, slots=8):
     return st
 
 
-def cpu_baseline(budget_s=20.0):
-    """FP32 CPU oracle on this host: HT-S encode (+context update) and decode
-    of a cropped sample of the same synthetic chunk, frames/s scaled by area
-    to 1088x1920 (the headline), plus the 256x256 intra frame at all threads
-    and at 1 thread, and the C++ rANS coder in symbols/s."""
+def cpu_baseline():
+    """FP32 CPU oracle on this host: HT-S encode (with its context update) and
+    decode of one full chunk of the same synthetic 1920x1080 stream (4 real
+    frames padded to 8, qp 32, skip 0.15), timed once each; plus the 256x256
+    intra frame at all threads and at 1 thread, and the C++ rANS coder in
+    symbols/s."""
     import torch
     from dcvc_amd import entropy, synthetic, weights
     from oracle import dcvc_oracle as O
     threads = torch.get_num_threads()
-    Hs, Ws = 256, 448
+    H, W = 1080, 1920
     sdp = {k: v.float() for k, v in weights.to_torch(weights.round_to_half(weights.init_state_dict("hts", 1))).items()}
-    frames = synthetic.motion_sequence_yuv420(Hs, Ws, 5, seed=0)
-    ref = torch.from_numpy(synthetic.frame_to_input(frames[0]))[None]
-    x = torch.from_numpy(synthetic.chunk_to_input(frames[1:5]))[None]
+    frames = synthetic.motion_sequence_yuv420(H, W, 5, seed=0)
+    ref = O.pad_to(torch.from_numpy(synthetic.frame_to_input(frames[0]))[None], 16)
+    x = torch.from_numpy(synthetic.chunk_to_input(frames[1:5], 8))[None]
+    del frames
     st_e, st_d = O.HTSState(), O.HTSState()
-    O.hts_add_ref_feature_from_frame(sdp, st_e, ref, True)
-    O.hts_add_ref_feature_from_frame(sdp, st_d, ref, False)
-    times = []
-    t_all = time.perf_counter()
     with torch.inference_mode():
-        while True:
-            t0 = time.perf_counter()
-            r = O.hts_compress(sdp, st_e, x, 32, False, skip_thres=0.15)
-            sym = r.sym16_dense
-            O.hts_decompress(sdp, st_d, r.z_int8, lambda idx: (sym[r.cond_dense] >> 8).astype(np.int8), 32, Hs, Ws,
-                             False, skip_thres=0.15)
-            times.append(time.perf_counter() - t0)
-            if time.perf_counter() - t_all > budget_s / 2 or len(times) >= 5:
-                break
-    t = float(np.median(times))
-    scale = (1088 * 1920) / (Hs * Ws)
-    out = {"value": round(4.0 / (t * scale), 4), "unit": "frames/s", "cores": threads, "kind": "port",
-           "sample": f"FP32 torch oracle, HT-S encode+decode of a {Hs}x{Ws} crop of the same 4-frame chunk "
-                     f"(padded to 8), {len(times)} run(s), median {t:.2f} s, scaled by area x{scale:.2f} "
-                     f"to 1088x1920"}
+        O.hts_add_ref_feature_from_frame(sdp, st_e, ref, True)
+        O.hts_add_ref_feature_from_frame(sdp, st_d, ref, False)
+        t0 = time.perf_counter()
+        r = O.hts_compress(sdp, st_e, x, 32, False, skip_thres=0.15)
+        t_enc = time.perf_counter() - t0
+        sym = r.sym16_dense
+        t0 = time.perf_counter()
+        O.hts_decompress(sdp, st_d, r.z_int8, lambda idx: (sym[r.cond_dense] >> 8).astype(np.int8), 32, H, W,
+                         False, skip_thres=0.15)
+        t_dec = time.perf_counter() - t0
+    out = {"value": round(4.0 / (t_enc + t_dec), 4), "unit": "frames/s", "cores": threads, "kind": "port",
+           "sample": f"FP32 torch oracle (the port of the reference's algorithm), one full HT-S chunk of the same "
+                     f"1920x1080 stream (4 frames padded to 8, padded to 1088x1920), encode {t_enc:.2f} s + "
+                     f"decode {t_dec:.2f} s on {threads} threads; 4 frames counted",
+           "encode_s": round(t_enc, 3), "decode_s": round(t_dec, 3)}
     # 256x256 intra frame (encode incl. reconstruction), all threads and 1 thread
     sdi = {k: v.float() for k, v in weights.to_torch(weights.round_to_half(weights.init_state_dict("dmci", 0))).items()}
     xi = torch.from_numpy(synthetic.frame_to_input(synthetic.texture_yuv420(256, 256, seed=0)))[None]
@@ -445,7 +532,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--profile-step", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--no-precision-forms", action="store_true")
+    ap.add_argument("--no-tuning-ab", action="store_true", help="skip the tuned-vs-heuristic tile comparison")
     ap.add_argument("--no-batching", action="store_true")
     ap.add_argument("--no-htl", action="store_true", help="skip the HT-L chunk-model section")
     ap.add_argument("--no-ld", action="store_true", help="skip the low-delay model section")
@@ -501,7 +590,12 @@ def main():
         step()
     torch.cuda.synchronize()
     if args.profile_step:
-        step()
+        torch.cuda.synchronize()
+        torch.cuda._sleep(1000)  # window marker
+        for _ in range(args.profile_steps):
+            step()
+        torch.cuda._sleep(1000)
+        torch.cuda.synchronize()
         a = torch.empty(64 << 20, dtype=torch.float32, device=dev)
         b = torch.empty_like(a)
         b.copy_(a)  # 256 MiB read + 256 MiB write: counter calibration
@@ -556,7 +650,13 @@ def main():
         }
         classes = kernel_classes(recs)
         out["kernel_classes"] = classes
-        out["roofline"] = roofline_of(classes)
+        instances = kernel_instances(recs)
+        out["kernel_instances"] = dict(list(instances.items())[:8])
+        out["roofline"] = roofline_of(instances, classes)
+    if not args.no_tuning_ab:
+        log("tuned vs heuristic tiles")
+        out["tuning"] = tuning_ab(sdp, enc_i["x_hat"], dec_i["x_hat"], chunks, H, W, pad_b, pad_r, args.skip_thres,
+                                  args.qp)
     if not args.no_precision_forms:
         log("precision forms")
         out["precision_forms"] = precision_forms(sdp, p_enc, enc_i["x_hat"], chunks, H, W, pad_b, pad_r,

@@ -321,7 +321,7 @@ int dcvc_trace_end(long long* records, int max_records) {
         if (records)
             for (int i = 0; i < n && i < max_records; ++i) {
                 const TraceRecord& r = t[i];
-                long long* o = records + 8LL * i;
+                long long* o = records + 15LL * i;
                 o[0] = r.kind;
                 o[1] = r.epi;
                 o[2] = r.M;
@@ -330,6 +330,13 @@ int dcvc_trace_end(long long* records, int max_records) {
                 o[5] = r.flops;
                 o[6] = r.bytes;
                 o[7] = r.ns;
+                o[8] = r.amode;
+                o[9] = r.dmode;
+                o[10] = r.prec;
+                o[11] = r.cfg;
+                o[12] = r.n_img;
+                o[13] = r.H;
+                o[14] = r.W;
             }
     });
     return st == DCVC_OK ? n : -st;

@@ -15,5 +15,10 @@ struct DwArgs {
 
 int dw3x3_validate(const DwArgs& a);
 hipError_t dw3x3_launch(const DwArgs& a, hipStream_t stream);
+// Replaces the strip-length table with n entries {n_img, H, W, C, rows (4 or
+// 8)}; n = 0 clears it.
+int dw3x3_tune_set(const long long* entries, int n);
+// Strip length a launch of this shape uses now (table entry or heuristic).
+int dw3x3_strip_for(const DwArgs& a);
 
 }  // namespace dcvc

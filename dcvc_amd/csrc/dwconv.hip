@@ -11,6 +11,10 @@
 // reads contiguous NHWC rows.  The tap order (ky, kx) is fixed, so the
 // result does not depend on the launch shape.
 #include "common.h"
+#include <mutex>
+#include <shared_mutex>
+#include <unordered_map>
+
 #include "dwconv.h"
 
 namespace dcvc {
@@ -91,21 +95,62 @@ int dw3x3_validate(const DwArgs& a) {
     return DCVC_OK;
 }
 
+namespace {
+// Tuned strip lengths, (n_img, H, W, C) -> 4 or 8 rows, from the autotuner's
+// database; shapes with no entry use the heuristic below.
+std::shared_mutex g_dw_mu;
+std::unordered_map<uint64_t, int> g_dw_tune;
+
+bool dw_key(long long n_img, long long H, long long W, long long C, uint64_t* key) {
+    if (n_img <= 0 || n_img >= (1 << 12) || H <= 0 || H >= (1 << 16) || W <= 0 || W >= (1 << 16) || C <= 0 ||
+        C >= (1 << 16))
+        return false;
+    *key = ((uint64_t)n_img << 48) | ((uint64_t)H << 32) | ((uint64_t)W << 16) | (uint64_t)C;
+    return true;
+}
+}  // namespace
+
+int dw3x3_tune_set(const long long* entries, int n) {
+    if (n < 0 || (n > 0 && !entries)) return DCVC_ERR_ARG;
+    std::unordered_map<uint64_t, int> t;
+    for (int i = 0; i < n; ++i) {
+        const long long* e = entries + 5LL * i;
+        uint64_t key;
+        if (!dw_key(e[0], e[1], e[2], e[3], &key) || (e[4] != 4 && e[4] != 8)) return DCVC_ERR_ARG;
+        t[key] = (int)e[4];
+    }
+    std::unique_lock<std::shared_mutex> lk(g_dw_mu);
+    g_dw_tune.swap(t);
+    return DCVC_OK;
+}
+
 // Strips of 8 rows, or of 4 when 8-row strips give fewer than 480 blocks
 // (under two per CU): measured on MI355X, 4-row strips take 10-35 % less
-// time on the 68x120 and 34x60 grids and are slower on 136x240.  Each
-// output's arithmetic is the same for either strip length.
+// time on the 68x120 and 34x60 grids and are slower on 136x240.  A tuning
+// table entry overrides the rule.  Each output's arithmetic is the same for
+// either strip length.
+int dw3x3_strip_for(const DwArgs& a) {
+    uint64_t key;
+    if (dw_key(a.n_img, a.H, a.W, a.C, &key)) {
+        std::shared_lock<std::shared_mutex> lk(g_dw_mu);
+        const auto it = g_dw_tune.find(key);
+        if (it != g_dw_tune.end()) return it->second;
+    }
+    const long strips = (a.H + 7) / 8;
+    const long b8 = ((long)(a.C / 8) * a.W * strips * a.n_img + 255) / 256;
+    return b8 < 480 ? 4 : 8;
+}
+
 hipError_t dw3x3_launch(const DwArgs& a, hipStream_t stream) {
     auto blocks_for = [&](int th) {
         const long strips = (a.H + th - 1) / th;
         const long total = (long)(a.C / 8) * a.W * strips * a.n_img;
         return (long)((total + 255) / 256);
     };
-    const long b8 = blocks_for(8);
-    if (b8 < 480)
+    if (dw3x3_strip_for(a) == 4)
         hipLaunchKernelGGL(dw3x3_kernel<4>, dim3((unsigned)blocks_for(4)), dim3(256), 0, stream, a);
     else
-        hipLaunchKernelGGL(dw3x3_kernel<8>, dim3((unsigned)b8), dim3(256), 0, stream, a);
+        hipLaunchKernelGGL(dw3x3_kernel<8>, dim3((unsigned)blocks_for(8)), dim3(256), 0, stream, a);
     return hipGetLastError();
 }
 

@@ -71,8 +71,13 @@ struct GemmArgs {
 
 int gemm_validate(const GemmArgs& g, int epi, int amode, int dmode);
 hipError_t gemm_launch(const GemmArgs& g, int epi, int amode, int dmode, hipStream_t stream);
-// 0: automatic tile choice; 1-5: force one tile configuration (tests, tuning)
+// 0: automatic tile choice; 1-7: force one tile configuration (tests, tuning)
 int gemm_set_tile_policy(int policy);
+// Replaces the tuning table with n entries of 8 values {prec, epi, amode,
+// dmode, M, N, K, configuration}; n = 0 clears it.
+int gemm_tune_set(const long long* entries, int n);
+// The configuration a launch of this GEMM uses now (policy, table or heuristic).
+int gemm_tile_for(const GemmArgs& g, int epi, int amode, int dmode);
 
 // Weight conversion for the low-precision products, on the device: rows x K
 // fp16 -> bf16 (round to nearest even), or -> fp8 e4m3 with scale[r] =

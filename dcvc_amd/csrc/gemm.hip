@@ -41,6 +41,10 @@
 //   transposed_conv (2x2 s2).  Epilogue op order follows
 //   cutlass/cutlass_epilogue.h:79-131,175-224: (acc + b) [-> wsilu] [+ S1]
 //   [+ S2 | * q].
+#include <mutex>
+#include <shared_mutex>
+#include <unordered_map>
+
 #include "common.h"
 #include "gemm.h"
 #include "gemm_dev.h"
@@ -50,6 +54,21 @@ namespace dcvc {
 namespace {
 
 int g_tile_policy = 0;  // 0 automatic, otherwise a forced configuration id
+
+// Tuned tile choices, (prec, epi, amode, dmode, M, N, K) -> configuration id,
+// installed from the autotuner's database.  Read at launch (under graph
+// capture: once, when the graph is recorded); shapes with no entry use the
+// built-in heuristic.
+std::shared_mutex g_tune_mu;
+std::unordered_map<uint64_t, int> g_tune;
+
+bool tune_key(int prec, int epi, int amode, int dmode, long long M, long long N, long long K, uint64_t* key) {
+    if (prec < 0 || prec > 2 || epi < 0 || epi > 15 || amode < 0 || amode > 3 || dmode < 0 || dmode > 1) return false;
+    if (M <= 0 || M >= (1LL << 24) || N <= 0 || N >= (1LL << 15) || K <= 0 || K >= (1LL << 16)) return false;
+    *key = ((uint64_t)prec << 62) | ((uint64_t)epi << 58) | ((uint64_t)amode << 56) | ((uint64_t)dmode << 55) |
+           ((uint64_t)M << 31) | ((uint64_t)N << 16) | (uint64_t)K;
+    return true;
+}
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef int i32x8 __attribute__((ext_vector_type(8)));
@@ -575,8 +594,7 @@ static hipError_t launch_tile(GemmArgs g, hipStream_t stream) {
 // but inside the codecs, where two streams run concurrently, one block per CU
 // cannot share a CU with the other stream's blocks and the layers took
 // 5-17 % longer, so they are only used when forced.
-static int pick_tile(const GemmArgs& g, bool quad) {
-    if (g_tile_policy >= 1 && g_tile_policy <= kPolicies) return g_tile_policy;
+static int heuristic_tile(const GemmArgs& g, bool quad) {
     const long long m256 = ceil_div(g.M, 256), m128 = ceil_div(g.M, 128);
     if (quad) return (m128 * ceil_div(g.N, 64) >= 480) ? 4 : 3;
     if (g.K >= 1024 && m256 * ceil_div(g.N, 256) >= 240) return 1;
@@ -584,9 +602,22 @@ static int pick_tile(const GemmArgs& g, bool quad) {
     return 3;
 }
 
+// Forced policy, else the tuning table's entry for this shape, else the
+// heuristic.  Every configuration gives the same bits.
+static int pick_tile(const GemmArgs& g, int epi, int amode, int dmode) {
+    if (g_tile_policy >= 1 && g_tile_policy <= kPolicies) return g_tile_policy;
+    uint64_t key;
+    if (tune_key(g.prec, epi, amode, dmode, g.M, g.N, g.K, &key)) {
+        std::shared_lock<std::shared_mutex> lk(g_tune_mu);
+        const auto it = g_tune.find(key);
+        if (it != g_tune.end()) return it->second;
+    }
+    return heuristic_tile(g, epi == EPI_QUAD);
+}
+
 template <int EPI, int AMODE, int DMODE, int PREC = PREC_F16>
 static hipError_t dispatch_tiles(const GemmArgs& g, hipStream_t stream) {
-    const int t = pick_tile(g, EPI == EPI_QUAD);
+    const int t = pick_tile(g, EPI, AMODE, DMODE);
     if constexpr (PREC == PREC_FP8) {
         // fp8 K tiles are 48 KB (256 x 256) / 24 KB (128 x 128): 3 stages.  The
         // shortcut epilogues (HBM-bound) stay on 128 x 128, whose registers
@@ -780,5 +811,22 @@ int gemm_set_tile_policy(int policy) {
     g_tile_policy = policy;
     return DCVC_OK;
 }
+
+int gemm_tune_set(const long long* entries, int n) {
+    if (n < 0 || (n > 0 && !entries)) return DCVC_ERR_ARG;
+    std::unordered_map<uint64_t, int> t;
+    for (int i = 0; i < n; ++i) {
+        const long long* e = entries + 8LL * i;
+        uint64_t key;
+        if (!tune_key((int)e[0], (int)e[1], (int)e[2], (int)e[3], e[4], e[5], e[6], &key)) return DCVC_ERR_ARG;
+        if (e[7] < 1 || e[7] > kPolicies) return DCVC_ERR_ARG;
+        t[key] = (int)e[7];
+    }
+    std::unique_lock<std::shared_mutex> lk(g_tune_mu);
+    g_tune.swap(t);
+    return DCVC_OK;
+}
+
+int gemm_tile_for(const GemmArgs& g, int epi, int amode, int dmode) { return pick_tile(g, epi, amode, dmode); }
 
 }  // namespace dcvc

@@ -88,6 +88,32 @@ int dcvc_scale_index_thresholds(float* th128) {
 
 int dcvc_gemm_set_tile_policy(int policy) { return gemm_set_tile_policy(policy); }
 
+int dcvc_gemm_tune_set(const long long* entries, int n) { return gemm_tune_set(entries, n); }
+
+int dcvc_gemm_tile_for(int prec, int epi, int amode, int dmode, long M, int N, int K) {
+    if (M <= 0 || N <= 0 || K <= 0 || epi < EPI_BIAS || epi > EPI_QUAD || amode < A_PLAIN || amode > A_S1K3 ||
+        dmode < D_PLAIN || dmode > D_SUBPEL || prec < PREC_F16 || prec > PREC_FP8 || M > 0x7fffffffL)
+        return -DCVC_ERR_ARG;
+    GemmArgs g;
+    g.M = (int)M;
+    g.N = N;
+    g.K = K;
+    g.prec = prec;
+    return gemm_tile_for(g, epi, amode, dmode);
+}
+
+int dcvc_d3x3_tune_set(const long long* entries, int n) { return dw3x3_tune_set(entries, n); }
+
+int dcvc_d3x3_strip_for(int n_img, int H_, int W_, int C) {
+    if (n_img <= 0 || H_ <= 0 || W_ <= 0 || C <= 0) return -DCVC_ERR_ARG;
+    DwArgs a;
+    a.n_img = n_img;
+    a.H = H_;
+    a.W = W_;
+    a.C = C;
+    return dw3x3_strip_for(a);
+}
+
 int dcvc_conv1x1_bias(const void* A, int lda, int M, int K, const void* W, int N, const float* bias, void* D,
                       int ldd, void* stream) {
     GemmArgs g = plain_args(A, lda, M, K, W, N, bias, D, ldd);

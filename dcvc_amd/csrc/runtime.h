@@ -235,8 +235,15 @@ struct TraceRecord {
     long long flops;        // algorithmic multiply-adds x 2
     long long bytes;        // algorithmic HBM bytes: inputs + weights + outputs once
     long long ns;           // launch duration (timed traces), else -1
+    // tuning key and choice: A / D modes, operand precision, tile
+    // configuration (depthwise: strip rows; fused FFN: -1), and the output
+    // grid (images, rows, columns)
+    int amode = 0, dmode = 0, prec = 0, cfg = -1;
+    int n_img = 0, H = 0, W = 0;
 };
 void trace_record(const TraceRecord& r, hipStream_t s);
+// Sets the tuning key of the last record.
+void trace_key(int amode, int dmode, int prec, int cfg, int n_img, int H, int W);
 void trace_launched(hipStream_t s);
 bool trace_enabled();
 void trace_set(bool on, bool timed);

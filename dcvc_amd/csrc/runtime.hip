@@ -127,6 +127,8 @@ void Conv1x1::run(const Exec& e, const Act& a, const Act& dst, int epi, const Ac
     if (v != DCVC_OK) throw StatusError(v, "conv1x1: invalid GEMM arguments");
     trace_gemm(0, epi, g.M, N, K, epi == EPI_QUAD,
                (epi == EPI_SHORTCUT || epi == EPI_SHORTCUT_QUANT) ? 1 : (epi == EPI_SHORTCUT2 ? 2 : 0), e.stream);
+    if (trace_enabled())
+        trace_key(A_PLAIN, D_PLAIN, g.prec, gemm_tile_for(g, epi, A_PLAIN, D_PLAIN), a.n_img, a.H, a.W);
     hip_check(gemm_launch(g, epi, A_PLAIN, D_PLAIN, e.stream), "gemm launch");
     trace_launched(e.stream);
 }
@@ -157,6 +159,7 @@ void DW3x3::run(const Exec& e, const Act& in, const Act& out) const {
     if (trace_enabled()) {
         const long long M = in.pixels();
         trace_record(TraceRecord{3, 0, (int)M, C, 9, 18LL * M * C, 2LL * (2 * M * C + 9LL * C), -1}, e.stream);
+        trace_key(0, 0, 0, dw3x3_strip_for(a), a.n_img, a.H, a.W);
     }
     hip_check(dw3x3_launch(a, e.stream), "d3x3 launch");
     trace_launched(e.stream);
@@ -331,6 +334,8 @@ void ConvS2::run(const Exec& e, const Act& in, const Act& out) const {
                                  2LL * (in.pixels() * (long long)Cin + (long long)N * g.K + M * N), -1},
                      e.stream);
     }
+    if (trace_enabled())
+        trace_key(amode, D_PLAIN, g.prec, gemm_tile_for(g, EPI_BIAS, amode, D_PLAIN), out.n_img, out.H, out.W);
     hip_check(gemm_launch(g, EPI_BIAS, amode, D_PLAIN, e.stream), "strided conv launch");
     trace_launched(e.stream);
 }
@@ -393,6 +398,8 @@ void Subpel::run(const Exec& e, const Act& in, const Act& out) const {
                                  2LL * (M * Cin + 4LL * Cout * g.K + M * 4LL * Cout), -1},
                      e.stream);
     }
+    if (trace_enabled())
+        trace_key(amode, D_SUBPEL, g.prec, gemm_tile_for(g, EPI_BIAS, amode, D_SUBPEL), out.n_img, out.H, out.W);
     hip_check(gemm_launch(g, EPI_BIAS, amode, D_SUBPEL, e.stream), "subpel launch");
     trace_launched(e.stream);
 }
@@ -440,6 +447,18 @@ void trace_record(const TraceRecord& r, hipStream_t s) {
         hip_check(hipEventRecord(t.e0, s), "trace event");
     }
     g_trace.push_back(t);
+}
+
+void trace_key(int amode, int dmode, int prec, int cfg, int n_img, int H, int W) {
+    if (!g_trace_on || g_trace.empty()) return;
+    TraceRecord& r = g_trace.back().r;
+    r.amode = amode;
+    r.dmode = dmode;
+    r.prec = prec;
+    r.cfg = cfg;
+    r.n_img = n_img;
+    r.H = H;
+    r.W = W;
 }
 
 void trace_launched(hipStream_t s) {

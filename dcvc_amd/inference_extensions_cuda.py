@@ -19,7 +19,7 @@ import ctypes as C
 import numpy as np
 import torch
 
-from . import _lib
+from . import _lib, tuning
 
 
 _PREC = {"fp16": 0, "bf16": 1, "fp8": 2}
@@ -95,6 +95,7 @@ class DMCIProxy:
     def set_param(self, state_dict, skip_threshold):
         dev = state_dict["q_scale_dec"].device if isinstance(state_dict["q_scale_dec"], torch.Tensor) else None
         self._device = dev if dev is not None and dev.type == "cuda" else torch.device("cuda", torch.cuda.current_device())
+        tuning.ensure_loaded()  # tile choices, before any graph is captured
         a = _StateDictArgs(state_dict)
         _lib.call("dcvc_dmci_set_param", self._h, *a.args(), float(skip_threshold))
 
@@ -174,6 +175,7 @@ class _ChunkProxy:
         q = state_dict["q_decoder"]
         dev = q.device if isinstance(q, torch.Tensor) else None
         self._device = dev if dev is not None and dev.type == "cuda" else torch.device("cuda", torch.cuda.current_device())
+        tuning.ensure_loaded()  # tile choices, before any graph is captured
         a = _StateDictArgs(state_dict)
         self._call("set_param", *a.args(), float(skip_threshold))
 

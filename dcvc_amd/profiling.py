@@ -1,13 +1,26 @@
 """Summaries of rocprofv3 output for the library's kernels.
 
-kernel_stats_markdown(): the `--kernel-trace --stats` kernel table as text.
-traffic_per_launch(): HBM bytes per launch of one GEMM class from two
+`bench.py --profile-step` brackets its timed steps with two launches of a
+marker kernel (torch's spin kernel); every summary here keeps only the
+dispatches between the last two markers, so warm-up steps, intra frames and
+weight conversions stay out.
+
+window_stats_markdown(): per-kernel table of the window from a
+`--kernel-trace` CSV.
+traffic_per_launch(): HBM bytes per launch of one kernel from two
 `--pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs, each scaled by what the same
-counter reads for a device copy of known size in the same program.
+counter reads for a device copy of known size in the same program (its last
+dispatch).
+counters_per_launch(): any other counters (e.g. SQ_VALU_MFMA_BUSY_CYCLES,
+SQ_BUSY_CYCLES, GRBM_GUI_ACTIVE) averaged over the window's dispatches of
+one kernel.
 
 Usage (after the rocprofv3 runs of `bench.py --profile-step`):
-  python -m dcvc_amd.profiling stats <kernel_stats.csv> <title> <command...>
-  python -m dcvc_amd.profiling traffic <fetch.csv> <write.csv> <class> <copy_bytes>
+  python -m dcvc_amd.profiling stats <kernel_trace.csv> <title> <command...>
+  python -m dcvc_amd.profiling traffic <fetch.csv> <write.csv> <kernel> <copy_bytes>
+  python -m dcvc_amd.profiling counters <pmc.csv> <kernel> <counter,...>
+A <kernel> is a short name as printed by `stats` (e.g.
+gemm_kernel<128,256,2,2,3,6,0,0,0>) or a kernel class (e.g. conv1x1_quad).
 """
 from __future__ import annotations
 
@@ -49,15 +62,52 @@ def short_name(kernel_name: str) -> str:
     return kernel_name.split("(")[0][:90]
 
 
-def kernel_stats_markdown(stats_csv: str, title: str, command: str, top: int = 25) -> str:
-    rows = list(csv.DictReader(open(stats_csv)))
-    total = sum(float(r["TotalDurationNs"]) for r in rows)
+MARKER = "spin_kernel"
+
+
+def _window(ids_names):
+    """Dispatch ids strictly between the last two marker dispatches."""
+    marks = sorted(d for d, n in ids_names if MARKER in n)
+    if len(marks) < 2:
+        raise ValueError("fewer than two marker dispatches: not a bench.py --profile-step run")
+    return marks[-2], marks[-1]
+
+
+def _matches(name: str, kernel: str) -> bool:
+    return short_name(name).split(" ")[0] == kernel or gemm_class(name) == kernel
+
+
+def window_rows(trace_csv: str):
+    rows = list(csv.DictReader(open(trace_csv)))
+    lo, hi = _window([(int(r["Dispatch_Id"]), r["Kernel_Name"]) for r in rows])
+    return [r for r in rows if lo < int(r["Dispatch_Id"]) < hi]
+
+
+def window_stats(trace_csv: str) -> dict:
+    """short name -> {calls, total_ns, avg_ns, class} over the window."""
+    out = {}
+    for r in window_rows(trace_csv):
+        n = short_name(r["Kernel_Name"]).split(" ")[0]
+        e = out.setdefault(n, {"calls": 0, "total_ns": 0, "class": gemm_class(r["Kernel_Name"])})
+        e["calls"] += 1
+        e["total_ns"] += int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+    for e in out.values():
+        e["avg_ns"] = e["total_ns"] / e["calls"]
+    return dict(sorted(out.items(), key=lambda kv: -kv[1]["total_ns"]))
+
+
+def window_stats_markdown(trace_csv: str, title: str, command: str, top: int = 25) -> str:
+    st = window_stats(trace_csv)
+    rows = window_rows(trace_csv)
+    total = sum(e["total_ns"] for e in st.values())
+    span = max(int(r["End_Timestamp"]) for r in rows) - min(int(r["Start_Timestamp"]) for r in rows)
     lines = [f"# {title}", "", "Command:", "", "```", command, "```", "",
-             f"Total kernel time {total / 1e6:.3f} ms over {sum(int(r['Calls']) for r in rows)} dispatches.", "",
-             "| kernel | calls | total ms | avg us | % |", "|---|---:|---:|---:|---:|"]
-    for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"]))[:top]:
-        lines.append(f"| {short_name(r['Name'])} | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.3f} | "
-                     f"{float(r['AverageNs']) / 1e3:.2f} | {float(r['Percentage']):.2f} |")
+             f"Window between the two markers: {len(rows)} dispatches, {span / 1e6:.3f} ms wall, "
+             f"{total / 1e6:.3f} ms summed kernel time.", "",
+             "| kernel | class | calls | total ms | avg us | % |", "|---|---|---:|---:|---:|---:|"]
+    for n, e in list(st.items())[:top]:
+        lines.append(f"| {n} | {e['class'] or ''} | {e['calls']} | {e['total_ns'] / 1e6:.3f} | "
+                     f"{e['avg_ns'] / 1e3:.2f} | {100.0 * e['total_ns'] / total:.2f} |")
     return "\n".join(lines) + "\n"
 
 
@@ -65,26 +115,26 @@ def _per_dispatch(pmc_csv: str, counter: str):
     vals = defaultdict(float)
     names = {}
     for r in csv.DictReader(open(pmc_csv)):
-        if r["Counter_Name"] != counter:
-            continue
         d = int(r["Dispatch_Id"])
-        vals[d] += float(r["Counter_Value"])
         names[d] = r["Kernel_Name"]
+        if r["Counter_Name"] == counter:
+            vals[d] += float(r["Counter_Value"])
     return vals, names
 
 
-def traffic_per_launch(fetch_csv: str, write_csv: str, kernel_class: str, copy_bytes: int) -> dict:
-    """FETCH_SIZE / WRITE_SIZE (KiB per dispatch) averaged over the dispatches of
-    `kernel_class`, scaled so the calibration copy (the program's last
-    dispatch, a device copy of copy_bytes) reads copy_bytes and writes
+def traffic_per_launch(fetch_csv: str, write_csv: str, kernel: str, copy_bytes: int) -> dict:
+    """FETCH_SIZE / WRITE_SIZE (KiB per dispatch) averaged over the window's
+    dispatches of `kernel`, scaled so the calibration copy (the program's
+    last dispatch, a device copy of copy_bytes) reads copy_bytes and writes
     copy_bytes."""
-    out = {"kernel_class": kernel_class}
+    out = {"kernel": kernel}
     for counter, path, key in (("FETCH_SIZE", fetch_csv, "fetch"), ("WRITE_SIZE", write_csv, "write")):
         vals, names = _per_dispatch(path, counter)
+        lo, hi = _window(names.items())
         last = max(vals)
         ref = vals[last] * 1024.0
         factor = copy_bytes / ref
-        mine = [v for d, v in vals.items() if gemm_class(names[d]) == kernel_class]
+        mine = [v for d, v in vals.items() if lo < d < hi and _matches(names[d], kernel)]
         out[f"{key}_kib_raw_avg"] = sum(mine) / len(mine) if mine else None
         out[f"{key}_scale"] = factor
         out[f"{key}_bytes_per_launch"] = (sum(mine) / len(mine)) * 1024.0 * factor if mine else None
@@ -95,11 +145,25 @@ def traffic_per_launch(fetch_csv: str, write_csv: str, kernel_class: str, copy_b
     return out
 
 
+def counters_per_launch(pmc_csv: str, kernel: str, counters: list[str]) -> dict:
+    """Each counter averaged over the window's dispatches of `kernel`."""
+    out = {"kernel": kernel}
+    for c in counters:
+        vals, names = _per_dispatch(pmc_csv, c)
+        lo, hi = _window(names.items())
+        mine = [v for d, v in vals.items() if lo < d < hi and _matches(names[d], kernel)]
+        out[c] = sum(mine) / len(mine) if mine else None
+        out["dispatches"] = len(mine)
+    return out
+
+
 def main(argv):
     if argv[0] == "stats":
-        print(kernel_stats_markdown(argv[1], argv[2], " ".join(argv[3:])), end="")
+        print(window_stats_markdown(argv[1], argv[2], " ".join(argv[3:])), end="")
     elif argv[0] == "traffic":
         print(json.dumps(traffic_per_launch(argv[1], argv[2], argv[3], int(argv[4])), indent=1))
+    elif argv[0] == "counters":
+        print(json.dumps(counters_per_launch(argv[1], argv[2], argv[3].split(",")), indent=1))
     else:
         raise SystemExit(__doc__)
 

@@ -192,11 +192,14 @@ long dcvc_ld_last_symbol_count(void* h);
 /* Layer-table trace (no reference counterpart): between begin and end, every
  * GEMM / conv / depthwise launch issued from the host is recorded; end
  * returns the record count (or -status) and fills up to max_records rows of
- * 8 int64: kind (0 1x1, 1 stride-2 conv, 2 sub-pixel conv, 3 depthwise),
- * epilogue, M, N, K, algorithmic FLOPs, algorithmic bytes, launch duration
- * in ns (timed != 0: HIP events recorded around each launch on its own
- * stream; else -1).  Graph replays launch nothing from the host, so trace
- * with graphs off. */
+ * 15 int64: kind (0 1x1, 1 stride-2 conv, 2 sub-pixel conv, 3 depthwise,
+ * 4 fused FFN), epilogue, M, N, K, algorithmic FLOPs, algorithmic bytes,
+ * launch duration in ns (timed != 0: HIP events recorded around each launch
+ * on its own stream; else -1), then the tuning key and choice: A mode,
+ * D mode, operand precision, tile configuration (depthwise: strip rows;
+ * fused FFN: -1), and the grid the launch works on (images, rows, columns;
+ * the output grid for GEMMs, 0 for the fused FFN).  Graph replays launch nothing from the host, so trace with
+ * graphs off. */
 int dcvc_trace_begin(int timed);
 int dcvc_trace_end(long long* records, int max_records);
 long dcvc_dmci_step_symbol_count(void* h, int step);

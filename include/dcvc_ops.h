@@ -30,11 +30,27 @@ int dcvc_version(void);
  * (def_const.h:6-11, elementwise/stream.cu:77-87).  Host-only. */
 int dcvc_scale_index_thresholds(float* th128);
 
-/* GEMM tile configuration: 0 automatic (default), 1-5 force one of the
- * three tile shapes for every GEMM launched afterwards (process-wide).
- * No counterpart in the reference: used by tests (all shapes must give
- * bit-identical results) and by tile tuning.  Host-only. */
+/* GEMM tile configuration: 0 automatic (default), 1-7 force one of the
+ * tile configurations for every GEMM launched afterwards (process-wide).
+ * No counterpart in the reference: used by tests (all configurations must
+ * give bit-identical results) and by the autotuner.  Host-only. */
 int dcvc_gemm_set_tile_policy(int policy);
+
+/* Tile-choice database of the autotuner, the counterpart of the reference's
+ * per-shape tuned maps HINT_MAP_BIAS (cutlass/conv1x1_bias.cu:8-522) and
+ * HINT_MAP_D3x3 (cutlass/d3x3.cu:7-441), which launch_cutlass_helper
+ * consults (cutlass/cutlass_kernel.h:233-249).  dcvc_gemm_tune_set replaces
+ * the GEMM table with n entries of 8 int64 {prec, epi, amode, dmode, M, N, K,
+ * config (1-7)}: a GEMM of that key launches with that configuration, other
+ * shapes keep the built-in heuristic (n = 0 clears).  dcvc_d3x3_tune_set does
+ * the same for the depthwise strip length, entries {n_img, H, W, C, rows (4
+ * or 8)}.  The *_for functions return the choice a launch makes now (or
+ * -1 for bad arguments).  The choice never changes results.  Install before
+ * graphs are captured: a captured graph keeps its launches.  Host-only. */
+int dcvc_gemm_tune_set(const long long* entries, int n);
+int dcvc_gemm_tile_for(int prec, int epi, int amode, int dmode, long M, int N, int K);
+int dcvc_d3x3_tune_set(const long long* entries, int n);
+int dcvc_d3x3_strip_for(int n_img, int H, int W, int C);
 
 /* ---------------- 1x1-conv GEMMs (MFMA, fp32 accumulate) ----------------
  * A [M x K] fp16 (row stride lda), W [N x K] fp16 (PyTorch [N,K,1,1]),
