@@ -160,23 +160,73 @@ def roofline_of(instances, classes):
     return out
 
 
-def precision_forms(sdp, p_enc_ref, x_hat_i, chunks, H, W, pad_b, pad_r, skip, dev, steps=30):
-    """BF16 / FP8 1x1-conv products against the FP16 path: frames/s at qp 32,
-    then per qp in {0, 21, 42, 63} the 8-chunk stream coded in each precision:
-    symbol-match rate of the encoder's dense symbols against FP16, PSNR of the
-    4 real frames of each chunk, bpp; and per-layer error of representative
-    layers against float64 (real weights, N(0,1) activations)."""
+def gemm_layers(sd):
+    """Every 1x1-conv GEMM layer of a state dict with the epilogue it runs
+    with (dc.0: WSiLU, ffn.0: the quad GEMM, the rest: bias; shortcut adds
+    are exact fp32 adds after the product)."""
+    out = []
+    for k, v in sd.items():
+        if not k.endswith(".weight") or v.dim() != 4 or v.shape[2] != 1 or v.shape[3] != 1:
+            continue
+        name = k[:-len(".weight")]
+        epi = "wsilu" if name.endswith(".dc.0") else ("quad" if name.endswith(".ffn.0") else "bias")
+        out.append((name, epi))
+    return out
+
+
+def per_layer_errors(sd, dev, rows=256, seed=0):
+    """Every GEMM layer of the model in FP16 / BF16 / FP8 on the GPU against
+    a float64 host product: the layer's own fp16 weights and bias, `rows`
+    N(0,1) activation rows.  Per precision: mean and max over the layers of
+    the max and mean error relative to the reference's max / mean magnitude."""
     import torch
-    from dcvc_amd import codec, kernels
+    from dcvc_amd import kernels
     from oracle import dcvc_oracle as O
-    out = {"layers": {}, "fps": {}, "fps_loop_y_symbols": {}, "fps_loop_bytes_per_chunk": {},
-           "traced_kernel_ms": {}, "per_qp": {}}
+    r = np.random.Generator(np.random.PCG64(seed))
+    errs = {p: [] for p in ("fp16", "bf16", "fp8")}
+    for name, epi in gemm_layers(sd):
+        w = sd[name + ".weight"].float()[:, :, 0, 0].numpy().astype(np.float16)
+        b = (sd[name + ".bias"].float().numpy() if (name + ".bias") in sd else np.zeros(w.shape[0])).astype(np.float32)
+        a = r.standard_normal((rows, w.shape[1])).astype(np.float16)
+        ref = O.ref_gemm_epilogue(a, w, b, epi)
+        wt = torch.from_numpy(w).to(dev)
+        at, bt = torch.from_numpy(a).to(dev), torch.from_numpy(b).to(dev)
+        for prec in errs:
+            wq, sc = (wt, None) if prec == "fp16" else kernels.weights_to_lowp(wt, prec)
+            got = kernels.gemm_lowp(at, wq, sc, prec, bt, epi=epi).float().cpu().numpy().astype(np.float64)
+            err = np.abs(got - ref)
+            errs[prec].append((name, float(err.max() / np.abs(ref).max()), float(err.mean() / np.abs(ref).mean())))
+    out = {}
+    for prec, e in errs.items():
+        worst = max(e, key=lambda t: t[1])
+        out[prec] = {"layers": len(e), "mean_of_mean_rel": float(np.mean([t[2] for t in e])),
+                     "max_of_mean_rel": float(max(t[2] for t in e)), "mean_of_max_rel": float(np.mean([t[1] for t in e])),
+                     "max_of_max_rel": worst[1], "worst_layer": worst[0]}
+    return out
+
+
+def precision_forms(sdp, p_enc_ref, x_hat_i, chunks, H, W, pad_b, pad_r, skip, dev, steps=30):
+    """BF16 / FP8 1x1-conv products against the FP16 path.
+
+    * frames/s at qp 32 on the main stream (same weights as the headline);
+    * per qp in {0, 21, 42, 63}, with per-QP scale ladders in the weights
+      (weights.qp_ladder; the reference's init has all-ones rows, which would
+      make the qp rows identical), the 8-chunk stream coded in each
+      precision: PSNR of the 4 real frames of each chunk, bpp, symbol match
+      against FP16;
+    * the first chunk's symbols (qp 32) against the FP32 CPU oracle, beside
+      the fp16-storage floor (the oracle with fp16 storage against FP32);
+    * per-layer error against float64 for every GEMM layer of the model."""
+    import torch
+    from dcvc_amd import codec, entropy, weights
+    from oracle import dcvc_oracle as O
+    out = {"fps": {}, "fps_loop_y_symbols": {}, "fps_loop_bytes_per_chunk": {}, "traced_kernel_ms": {}, "per_qp": {}}
     n_chunks = len(chunks)
     P16 = ((H + 15) // 16) * ((W + 15) // 16)
 
-    def make(prec):
-        e = codec.VideoCodecHTS(share_from=p_enc_ref)
-        d = codec.VideoCodecHTS(share_from=p_enc_ref)
+    def make(prec, ref=p_enc_ref):
+        e = codec.VideoCodecHTS(share_from=ref)
+        d = codec.VideoCodecHTS(share_from=ref)
         e.proxy.set_precision(prec)
         d.proxy.set_precision(prec)
         e.add_ref_feature_from_frame(x_hat_i)
@@ -202,14 +252,18 @@ def precision_forms(sdp, p_enc_ref, x_hat_i, chunks, H, W, pad_b, pad_r, skip, d
         # differ by precision (the skip threshold acts on the computed scales)
         out["fps_loop_y_symbols"][prec] = int(np.mean(nsym))
         out["fps_loop_bytes_per_chunk"][prec] = int(np.mean(nbytes))
-        # GPU work of one encode + decode in this precision (launch trace)
         rec_e, rec_d = layer_trace(e, d, chunks[0], 32, pad_b, pad_r, H, W)
         out["traced_kernel_ms"][prec] = round(float(np.concatenate([rec_e, rec_d])[:, 7].sum()) * 1e-6, 3)
+        del e, d
+    # qp sweep with per-QP ladders
+    sdl = entropy.add_cdf_to_state_dict(weights.to_torch(weights.round_to_half(
+        weights.qp_ladder(weights.init_state_dict("hts", 1)))))
+    ref_l = codec.VideoCodecHTS(sdl, skip)
     for qp in (0, 21, 42, 63):
         res = {}
         ref_syms = None
         for prec in ("fp16", "bf16", "fp8"):
-            e, d = make(prec)
+            e, d = make(prec, ref_l)
             nbytes, psnrs, syms = 0, [], []
             for c in range(n_chunks):
                 bits, ec = e.proxy.compress(chunks[c], qp, False, pad_b, pad_r)
@@ -227,28 +281,33 @@ def precision_forms(sdp, p_enc_ref, x_hat_i, chunks, H, W, pad_b, pad_r, skip, d
             match = float(np.mean([np.mean(a == b) for a, b in zip(syms, ref_syms)]))
             res[prec] = {"bpp": round(8.0 * nbytes / (4 * n_chunks * H * W), 5),
                          "psnr_db": round(float(np.mean(psnrs)), 4), "symbol_match_vs_fp16": round(match, 5)}
+            del e, d
         for prec in ("bf16", "fp8"):
             res[prec]["psnr_delta_db"] = round(res[prec]["psnr_db"] - res["fp16"]["psnr_db"], 4)
             res[prec]["bpp_delta_pct"] = round(100 * (res[prec]["bpp"] / res["fp16"]["bpp"] - 1), 3)
         out["per_qp"][str(qp)] = res
-    # per-layer error against float64 on real weights
-    r = np.random.Generator(np.random.PCG64(0))
-    M = 8160
-    for name, epi in (("encoder.conv1.1.dc.0", "wsilu"), ("encoder.conv1.1.ffn.0", "quad"),
-                      ("encoder.conv1.1.ffn.2", "bias"), ("encoder.conv1.0.adaptor", "bias")):
-        w = sdp[name + ".weight"].float()[:, :, 0, 0].numpy().astype(np.float16)
-        b = sdp[name + ".bias"].float().numpy().astype(np.float32)
-        a = r.standard_normal((M, w.shape[1])).astype(np.float16)
-        ref = O.ref_gemm_epilogue(a, w, b, epi)
-        row = {}
-        for prec in ("fp16", "bf16", "fp8"):
-            wt = torch.from_numpy(w).to(dev)
-            wq, sc = (wt, None) if prec == "fp16" else kernels.weights_to_lowp(wt, prec)
-            got = kernels.gemm_lowp(torch.from_numpy(a).to(dev), wq, sc, prec, torch.from_numpy(b).to(dev),
-                                    epi=epi).float().cpu().numpy().astype(np.float64)
-            err = np.abs(got - ref)
-            row[prec] = {"max_rel": float(err.max() / np.abs(ref).max()), "mean_rel": float(err.mean() / np.abs(ref).mean())}
-        out["layers"][name] = row
+    out["qp_weights"] = "per-QP ladders (weights.qp_ladder: encoder side 0.5 -> 2 over qp, decoder side inverse)"
+    # first chunk of the stream at qp 32 against the FP32 oracle (skip 0.15)
+    sdo = {k: v.float() for k, v in sdp.items()}  # the headline codec's weights
+    ref = x_hat_i.float().cpu().contiguous()
+    xin = chunks[0].float().cpu().contiguous(memory_format=torch.contiguous_format)
+    syms = {}
+    for prec in ("fp16", "bf16", "fp8"):
+        e, _ = make(prec)
+        e.proxy.compress(chunks[0], 32, False, pad_b, pad_r)
+        syms[prec] = e.proxy.debug_copy(5, P16 * 256 * 2, np.int16).astype(np.int64)
+        del e
+    oracle = {}
+    with torch.inference_mode():
+        for mode, kw in (("fp32", {}), ("fp16_storage", {"fp16": True})):
+            with O.storage_mode(**kw):
+                st = O.HTSState()
+                O.hts_add_ref_feature_from_frame(sdo, st, ref, True)
+                oracle[mode] = O.hts_compress(sdo, st, xin, 32, False, skip_thres=skip).sym16_dense.astype(np.int64)
+    agree = lambda a, b: [round(float(np.mean((a >> 8) == (b >> 8))), 5), round(float(np.mean((a & 255) == (b & 255))), 5)]
+    out["vs_fp32_oracle"] = {p: {"symbols_indexes": agree(v, oracle["fp32"])} for p, v in syms.items()}
+    out["vs_fp32_oracle"]["fp16_storage_floor"] = {"symbols_indexes": agree(oracle["fp16_storage"], oracle["fp32"])}
+    out["per_layer_vs_fp64"] = per_layer_errors(sdo, dev)
     return out
 
 
@@ -399,46 +458,94 @@ def ld_form(x_hat_e, x_hat_d, chunks, H, W, pad_b, pad_r, skip, qp, steps=60, wa
 def batching_form(sdi, sdp, rounds=2):
     """16 concurrent streams of uneven resolution (4x 854x480, 6x 1280x720,
     4x 1920x1080, 2x 3840x2160), one 4-frame chunk each per round at a random
-    qp: one stream at a time vs all at once."""
+    qp.  The plain path (FP16, one stream at a time) against all streams at
+    once in FP16 and in BF16 (the batching form's precision).  Also the
+    bound batching could reach: every stream at the per-pixel rate of the
+    best-utilised resolution in the plain path."""
     import torch
     from dcvc_amd import batching
-    setup, _ = batching.build_streams(sdi, sdp, batching.MIX_16, n_chunks=rounds + 1, seed=0)
-    n = len(setup.contexts)
+    n = len(batching.MIX_16)
     frames = 4 * n * rounds
-    batching.code_concurrently(setup.contexts, [c[0] for c in setup.chunks], setup.qps)  # warm-up, graph capture
+    out = {"streams": "4x 854x480, 6x 1280x720, 4x 1920x1080, 2x 3840x2160", "rounds": rounds,
+           "frames_counted": frames}
+    setup, _ = batching.build_streams(sdi, sdp, batching.MIX_16, n_chunks=rounds + 1, seed=0)
+    batching.code_concurrently(setup.contexts, [c[0] for c in setup.chunks], setup.qps)  # graph capture
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    per_res = {}
     for r in range(rounds):
-        batching.code_sequentially(setup.contexts, [c[1 + r] for c in setup.chunks], setup.qps)
+        for res in batching.code_sequentially(setup.contexts, [c[1 + r] for c in setup.chunks], setup.qps):
+            ctx = setup.contexts[res.stream]
+            per_res.setdefault((ctx.H, ctx.W), []).append(res.encode_s + res.decode_s)
     t_seq = time.perf_counter() - t0
-    setup2, _ = batching.build_streams(sdi, sdp, batching.MIX_16, n_chunks=rounds + 1, seed=0)
+    t0 = time.perf_counter()
+    for r in range(rounds):
+        batching.code_concurrently(setup.contexts, [c[1 + r] for c in setup.chunks], setup.qps)
+    t_con16 = time.perf_counter() - t0
+    del setup
+    setup2, _ = batching.build_streams(sdi, sdp, batching.MIX_16, n_chunks=rounds + 1, seed=0, precision="bf16")
     batching.code_concurrently(setup2.contexts, [c[0] for c in setup2.chunks], setup2.qps)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for r in range(rounds):
         batching.code_concurrently(setup2.contexts, [c[1 + r] for c in setup2.chunks], setup2.qps)
     t_con = time.perf_counter() - t0
-    return {"streams": "4x 854x480, 6x 1280x720, 4x 1920x1080, 2x 3840x2160", "rounds": rounds,
-            "frames_counted": frames, "sequential_frames_per_s": round(frames / t_seq, 2),
-            "concurrent_frames_per_s": round(frames / t_con, 2), "speedup": round(t_seq / t_con, 3),
-            "precision": "fp16"}
+    del setup2
+    # per-pixel cost of each resolution alone (plain path), and the bound
+    px = {k: ((k[0] + 15) // 16 * 16) * ((k[1] + 15) // 16 * 16) for k in per_res}
+    us_per_mpx = {f"{k[1]}x{k[0]}": round(1e6 * float(np.mean(v)) / (px[k] / 1e6), 1) for k, v in per_res.items()}
+    best = min(float(np.mean(v)) / px[k] for k, v in per_res.items())
+    t_bound = rounds * sum(best * ((H + 15) // 16 * 16) * ((W + 15) // 16 * 16) for H, W in batching.MIX_16)
+    out.update({"plain_fp16_sequential_frames_per_s": round(frames / t_seq, 2),
+                "concurrent_fp16_frames_per_s": round(frames / t_con16, 2),
+                "concurrent_bf16_frames_per_s": round(frames / t_con, 2),
+                "speedup_bf16_vs_plain": round(t_seq / t_con, 3), "speedup_fp16_vs_plain": round(t_seq / t_con16, 3),
+                "target_speedup": 3.0,
+                "plain_us_per_megapixel_step": us_per_mpx,
+                "bound_frames_per_s_at_best_per_pixel_rate": round(frames / t_bound, 2),
+                "bound_speedup_vs_plain": round(t_seq / t_bound, 3), "precision": "bf16 (fp16 plain baseline)"})
+    return out
 
 
-def serving_form(sdi, sdp, duration=10.0, rate=120.0, n_streams=64, slots=8):
+def serving_form(sdi, sdp, duration=60.0, rate=120.0, n_streams=64, slots=8, sweep=(200.0, 280.0, 360.0),
+                 sweep_seconds=10.0, p99_bound_ms=100.0):
     """Poisson arrivals (encode or decode of one 4-frame chunk of one of 64
     persistent streams, resolutions 480p 30 %, 720p 40 %, 1080p 25 %,
-    2160p 5 %) served by the continuous-batching loop, HBM budget 200 GB."""
+    2160p 5 %) served by the continuous-batching loop, HBM budget 200 GB,
+    every stream's contexts allocated and graphs captured before the trace
+    (warm start).  Then the same load on the plain path (one request at a
+    time), and the capacity: the arrival rates of `sweep` for
+    `sweep_seconds` each, the highest one whose p99 stays under
+    `p99_bound_ms` and whose requests are all served."""
     from dcvc_amd import serving
     res = serving.resolution_mix(n_streams, seed=0)
-    ex = serving.GpuExecutor(sdi, sdp, res, chunks_per_stream=4, seed=0)
-    reqs = serving.poisson_trace(rate, duration, n_streams, seed=0)
-    srv = serving.Server(ex, slots=slots, hbm_budget=200 << 30)
-    srv.run(reqs, realtime=True)
-    st = serving.Server.stats(reqs)
-    st.update({"rate_per_s": rate, "duration_s": duration, "streams": n_streams, "slots": slots,
-               "resolution_mix": serving.MIX_64, "max_running": srv.max_running,
-               "resident_gb": round(srv.max_resident_bytes / 2 ** 30, 2)})
-    return st
+    ex = serving.GpuExecutor(sdi, sdp, res, chunks_per_stream=4, seed=0, keep_results=False)
+    t0 = time.perf_counter()
+    ex.prewarm()
+    warm_s = time.perf_counter() - t0
+
+    def serve(rate_, dur, slots_, seed):
+        reqs = serving.poisson_trace(rate_, dur, n_streams, seed=seed)
+        srv = serving.Server(ex, slots=slots_, hbm_budget=200 << 30)
+        srv.run(reqs, realtime=True)
+        st = serving.Server.stats(reqs)
+        st.update({"rate_per_s": rate_, "duration_s": dur, "slots": slots_, "max_running": srv.max_running,
+                   "resident_gb": round(srv.max_resident_bytes / 2 ** 30, 2)})
+        return st
+
+    out = serve(rate, duration, slots, 0)
+    out.update({"streams": n_streams, "resolution_mix": serving.MIX_64, "prewarm_s": round(warm_s, 2)})
+    out["plain_one_at_a_time"] = serve(rate, min(duration, 20.0), 1, 1)
+    cap = []
+    for r in sweep:
+        st = serve(r, sweep_seconds, slots, 2)
+        cap.append({k: st[k] for k in ("rate_per_s", "served", "requests", "p50_ms", "p99_ms", "frames_per_s")})
+    ok = [c["rate_per_s"] for c in [{"rate_per_s": rate, "p99_ms": out["p99_ms"], "served": out["served"],
+                                     "requests": out["requests"]}] + cap
+          if c["p99_ms"] is not None and c["p99_ms"] <= p99_bound_ms and c["served"] == c["requests"]]
+    out["capacity"] = {"p99_bound_ms": p99_bound_ms, "sweep": cap,
+                       "max_rate_within_bound_per_s": max(ok) if ok else None}
+    return out
 
 
 def cpu_baseline():
@@ -539,7 +646,7 @@ def main():
     ap.add_argument("--no-htl", action="store_true", help="skip the HT-L chunk-model section")
     ap.add_argument("--no-ld", action="store_true", help="skip the low-delay model section")
     ap.add_argument("--no-skip0", action="store_true", help="skip the HT-S skip_thres 0 section")
-    ap.add_argument("--serving-seconds", type=float, default=10.0,
+    ap.add_argument("--serving-seconds", type=float, default=60.0,
                     help="length of the Poisson serving trace (0 skips it)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))

@@ -87,6 +87,7 @@ void DMCIProxy::set_param(const HostDict& d, float skip_thres) {
     auto w = std::make_unique<DMCIWeights>();
     w->load(d, skip_thres);
     m_w = std::move(w);
+    if (m_prec != PREC_F16) m_w->mem.ensure_low_precision();
     for (int i = 0; i < 2; ++i) {
         m_enc.set_cdf(m_w->cdf[i].data(), (int)m_w->cdf_len[i].size(), m_w->cdf_row[i], m_w->cdf_len[i].data(), i);
         m_dec.set_cdf(m_w->cdf[i].data(), (int)m_w->cdf_len[i].size(), m_w->cdf_row[i], m_w->cdf_len[i].data(), i);

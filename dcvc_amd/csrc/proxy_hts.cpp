@@ -115,6 +115,7 @@ void HTSProxy::set_param(const HostDict& d, float skip_thres) {
 
 void HTSProxy::share_weights(const std::shared_ptr<HTSWeights>& w) {
     m_w = w;
+    if (m_prec != PREC_F16) m_w->mem.ensure_low_precision();
     for (RansEncoder* e : {&m_enc}) {
         for (int i = 0; i < 2; ++i)
             e->set_cdf(w->cdf[i].data(), (int)w->cdf_len[i].size(), w->cdf_row[i], w->cdf_len[i].data(), i);

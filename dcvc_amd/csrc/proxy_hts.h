@@ -52,6 +52,7 @@ public:
     // GemmPrec of every 1x1-conv product (precision form); recaptures the graphs
     void set_precision(int prec) {
         if (prec < PREC_F16 || prec > PREC_FP8) throw std::invalid_argument("precision must be 0, 1 or 2");
+        if (prec != PREC_F16 && m_w) m_w->mem.ensure_low_precision();
         m_prec = prec;
         reset_graphs();
     }

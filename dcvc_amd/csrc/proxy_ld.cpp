@@ -109,6 +109,7 @@ void LDProxy::set_param(const HostDict& d, float skip_thres) {
 
 void LDProxy::share_weights(const std::shared_ptr<LDWeights>& w) {
     m_w = w;
+    if (m_prec != PREC_F16) m_w->mem.ensure_low_precision();
     for (int i = 0; i < 2; ++i) {
         m_enc.set_cdf(w->cdf[i].data(), (int)w->cdf_len[i].size(), w->cdf_row[i], w->cdf_len[i].data(), i);
         m_dec.set_cdf(w->cdf[i].data(), (int)w->cdf_len[i].size(), w->cdf_row[i], w->cdf_len[i].data(), i);

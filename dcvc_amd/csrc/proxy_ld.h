@@ -44,6 +44,7 @@ public:
     void set_use_graphs(bool on) { m_use_graphs = on; reset_graphs(); }
     void set_precision(int prec) {
         if (prec < PREC_F16 || prec > PREC_FP8) throw std::invalid_argument("precision must be 0, 1 or 2");
+        if (prec != PREC_F16 && m_w) m_w->mem.ensure_low_precision();
         m_prec = prec;
         reset_graphs();
     }

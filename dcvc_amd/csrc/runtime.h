@@ -6,6 +6,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "capi_util.h"
@@ -34,6 +35,8 @@ const HostTensor& need(const HostDict& d, const std::string& key);
 bool has(const HostDict& d, const std::string& key);
 
 // Owner of device allocations (freed together).
+struct Conv1x1;
+
 class DevMem {
 public:
     DevMem() = default;
@@ -47,10 +50,18 @@ public:
     }
     void free_all();
     size_t bytes() const { return m_bytes; }
+    // 1x1 convolutions whose weights live here.  Their bf16 / fp8 copies
+    // (the precision form) are made on the first request, on the null
+    // stream, so proxies that only run fp16 hold no extra copies.
+    void register_conv(Conv1x1* c) { m_convs.push_back(c); }
+    void ensure_low_precision();
 
 private:
     std::vector<void*> m_blocks;
     size_t m_bytes = 0;
+    std::vector<Conv1x1*> m_convs;
+    bool m_lowp = false;
+    std::mutex m_lowp_mu;
 };
 
 f16* upload_f16(DevMem& mem, const std::vector<float>& v);
@@ -105,7 +116,7 @@ struct Exec {
 struct Conv1x1 {
     f16* w = nullptr;
     float* b = nullptr;
-    void* w_bf16 = nullptr;   // low-precision copies made at load (PREC_BF16 / PREC_FP8)
+    void* w_bf16 = nullptr;   // low-precision copies (PREC_BF16 / PREC_FP8), DevMem::ensure_low_precision
     void* w_fp8 = nullptr;
     float* w_scale = nullptr;
     int N = 0, K = 0;

@@ -27,6 +27,14 @@ void* DevMem::alloc(size_t bytes) {
     return p;
 }
 
+void DevMem::ensure_low_precision() {
+    std::lock_guard<std::mutex> lk(m_lowp_mu);
+    if (m_lowp) return;
+    for (Conv1x1* c : m_convs) c->make_low_precision(*this);
+    hip_check(hipStreamSynchronize(nullptr), "weights to low precision");
+    m_lowp = true;
+}
+
 void DevMem::free_all() {
     for (void* p : m_blocks) hipFree(p);
     m_blocks.clear();
@@ -58,7 +66,7 @@ void Conv1x1::load_raw(DevMem& mem, const std::vector<float>& wt, const std::vec
     K = K_;
     w = upload_f16(mem, wt);
     b = upload_f32(mem, bias);
-    make_low_precision(mem);
+    mem.register_conv(this);
 }
 
 void Conv1x1::make_low_precision(DevMem& mem) {
@@ -84,7 +92,7 @@ void Conv1x1::load(DevMem& mem, const HostDict& d, const std::string& prefix, bo
     }
     w = upload_f16(mem, wt.data);
     b = upload_f32(mem, bias);
-    make_low_precision(mem);
+    mem.register_conv(this);
 }
 
 void Conv1x1::run(const Exec& e, const Act& a, const Act& dst, int epi, const Act* s1, const Act* s2,

@@ -59,7 +59,10 @@ def short_name(kernel_name: str) -> str:
     f = _FFN.search(kernel_name)
     if f:
         return f"ffn_kernel<{','.join(f.groups())}> [{gemm_class(kernel_name)}]"
-    return kernel_name.split("(")[0][:90]
+    n = kernel_name.split("(")[0]
+    if n.startswith("void "):
+        n = n[5:]
+    return n.replace("dcvc::", "")[:90]
 
 
 MARKER = "spin_kernel"

@@ -206,13 +206,16 @@ class GpuExecutor:
     encode requests code the stream's next chunk with its own encoder."""
 
     def __init__(self, sdi, sdp, resolutions, chunks_per_stream=4, skip_thres=0.15, seed=0, device="cuda",
-                 precision="fp16"):
+                 precision="fp16", keep_results=True):
         import torch
         from . import codec, synthetic
         self.torch = torch
         self.codec = codec
         self.res = list(resolutions)
         self.device = device
+        # keep_results=False returns sizes instead of the bitstream and copies
+        # of the decoded frames, so a long trace does not hold every result
+        self.keep_results = keep_results
         self.precision = precision
         self.shared = codec.VideoCodecHTS(sdp, skip_thres)
         intra_e, intra_d = codec.IntraCodec(sdi, skip_thres), codec.IntraCodec(sdi, skip_thres)
@@ -260,6 +263,33 @@ class GpuExecutor:
                 self.ctx[stream] = c
             return c
 
+    def prewarm(self):
+        """Allocate every stream's contexts and capture their graphs before
+        serving (two encodes and two decodes each, so both the first-chunk
+        and the steady-state graph variants exist), then put every stream
+        back at its I frame.  Without it a stream's first request pays for
+        the allocation and the captures."""
+        torch = self.torch
+        s = getattr(_tls, "stream", None)
+        if s is None:
+            s = _tls.stream = torch.cuda.Stream()
+        for stream in range(len(self.res)):
+            H, W = self.res[stream]
+            cont = self.content[(H, W)]
+            c = self._context(stream)
+            pad_b, pad_r = cont["pad"]
+            with torch.cuda.stream(s):
+                c["enc"].add_ref_feature_from_frame(cont["x_hat_e"])
+                c["dec"].add_ref_feature_from_frame(cont["x_hat_d"], apply_feature_adaptor=False)
+                for i in range(2):
+                    c["enc"].proxy.compress(cont["chunks"][i], 32, False, pad_b, pad_r)
+                    bits, ec = cont["bits"][i]
+                    c["dec"].proxy.decompress(bits, 32, H, W, ec, False)
+                c["enc"].add_ref_feature_from_frame(cont["x_hat_e"])
+                c["dec"].add_ref_feature_from_frame(cont["x_hat_d"], apply_feature_adaptor=False)
+                torch.cuda.current_stream().synchronize()
+            c.update(ne=0, nd=0, init=True)
+
     def run(self, r: Request):
         torch = self.torch
         H, W = self.res[r.stream]
@@ -278,7 +308,7 @@ class GpuExecutor:
                 x = cont["chunks"][c["ne"] % len(cont["chunks"])]
                 c["ne"] += 1
                 bits, ec = c["enc"].proxy.compress(x, r.qp, False, pad_b, pad_r)
-                out = bits.tobytes()
+                out = bits.tobytes() if self.keep_results else len(bits)
             else:
                 n = len(cont["bits"])
                 if c["nd"] and c["nd"] % n == 0:
@@ -287,7 +317,7 @@ class GpuExecutor:
                 bits, ec = cont["bits"][c["nd"] % n]
                 c["nd"] += 1
                 frames = c["dec"].proxy.decompress(bits, 32, H, W, ec, False)
-                out = [f.clone() for f in frames]
+                out = [f.clone() for f in frames] if self.keep_results else len(frames)
             torch.cuda.current_stream().synchronize()
         return out
 

@@ -322,6 +322,30 @@ def init_state_dict(model: str, seed: int = 0, q_jitter: float = 0.0, stable: bo
     return sd
 
 
+QP_ENCODER_SIDE = ("q_encoder", "q_feature", "q_scale_enc", "q_scale_y_enc")
+QP_DECODER_SIDE = ("q_decoder", "q_scale_dec", "q_scale_y_dec")
+
+
+def qp_ladder(sd, lo: float = 0.5, hi: float = 2.0):
+    """Per-QP scale rows that differ by qp, in place of the reference's
+    all-ones initialisation (video_model_ht.py:348-350, image_model.py):
+    encoder-side tables (features and latents multiplied before
+    quantisation) rise geometrically from `lo` at qp 0 to `hi` at qp 63, and
+    decoder-side tables are their reciprocals, the shape a trained model's
+    quantisation ladder has.  Used where a qp sweep must exercise the
+    per-QP selection (the precision form's qp rows); the default weights
+    keep the reference's ones."""
+    out = OrderedDict(sd)
+    g = np.exp(np.linspace(np.log(lo), np.log(hi), QP_NUM))
+    for k, v in sd.items():
+        base = k.split(".")[-1]
+        if base in QP_ENCODER_SIDE:
+            out[k] = np.repeat(g[:, None], v.shape[1], axis=1).astype(np.float32)
+        elif base in QP_DECODER_SIDE:
+            out[k] = np.repeat(1.0 / g[:, None], v.shape[1], axis=1).astype(np.float32)
+    return out
+
+
 def perturb_biases(sd, seed: int = 1, scale: float = 0.02):
     """Give every conv bias a small seeded random value.
 

@@ -172,7 +172,7 @@ def test_hts_precision_form_encoder_decoder_bitexact(gpu, prec):
     _, x_hat_i, _ = i_enc.compress(to(synthetic.frame_to_input(frames[0])), 32, 0, 0)
     chunk = to(synthetic.chunk_to_input(frames[1:9]))
     src = chunk.float()
-    psnr = {}
+    psnr, nbytes = {}, {}
     for p in ("fp16", prec):
         enc, dec = DMCHTSProxy(), DMCHTSProxy()
         enc.set_param(sdp, 0.15)
@@ -195,5 +195,12 @@ def test_hts_precision_form_encoder_decoder_bitexact(gpu, prec):
         assert torch.isfinite(rec).all()
         mse = float(((rec - src) ** 2).mean())
         psnr[p] = 10 * np.log10(1.0 / max(mse, 1e-12))
-        print(f"{p}: {len(bits)} bytes, PSNR {psnr[p]:.3f} dB")
-    assert abs(psnr[prec] - psnr["fp16"]) < 3.0
+        nbytes[p] = len(bits)
+        print(f"{p}: {len(bits)} bytes, PSNR {psnr[p]:.4f} dB")
+    # the product precision moves PSNR and size by its own rounding: BF16
+    # (8-bit mantissa) by little, FP8 E4M3 (3-bit mantissa) by more
+    d_psnr = abs(psnr[prec] - psnr["fp16"])
+    d_bytes = abs(nbytes[prec] / nbytes["fp16"] - 1)
+    print(f"{prec}: PSNR delta {d_psnr:.4f} dB, size delta {100 * d_bytes:.2f} %")
+    bound = {"bf16": (0.01, 0.01), "fp8": (0.02, 0.04)}[prec]  # measured 0.000 dB / 0.25 %, 0.001 dB / 1.5 %
+    assert d_psnr <= bound[0] and d_bytes <= bound[1]
