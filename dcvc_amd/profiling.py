@@ -19,8 +19,12 @@ Usage (after the rocprofv3 runs of `bench.py --profile-step`):
   python -m dcvc_amd.profiling stats <kernel_trace.csv> <title> <command...>
   python -m dcvc_amd.profiling traffic <fetch.csv> <write.csv> <kernel> <copy_bytes>
   python -m dcvc_amd.profiling counters <pmc.csv> <kernel> <counter,...>
+  python -m dcvc_amd.profiling launch <epi> <M> <K> <N> <times>
 A <kernel> is a short name as printed by `stats` (e.g.
 gemm_kernel<128,256,2,2,3,6,0,0,0>) or a kernel class (e.g. conv1x1_quad).
+`launch` runs one 1x1-conv GEMM shape (N weight rows; epi bias, wsilu,
+shortcut or quad) `times` times on N(0,1) data, the program to put under a
+counter pass for one kernel alone.
 """
 from __future__ import annotations
 
@@ -160,11 +164,30 @@ def counters_per_launch(pmc_csv: str, kernel: str, counters: list[str]) -> dict:
     return out
 
 
+def launch(epi: str, M: int, K: int, N: int, times: int):
+    import numpy as np
+    import torch
+    from . import kernels
+    dev = torch.device("cuda:0")
+    r = np.random.Generator(np.random.PCG64(0))
+    a = torch.from_numpy(r.standard_normal((M, K)).astype(np.float16)).to(dev)
+    w = torch.from_numpy((r.standard_normal((N, K)) / np.sqrt(K)).astype(np.float16)).to(dev)
+    b = torch.zeros(N, dtype=torch.float32, device=dev)
+    n_out = N // 4 if epi == "quad" else N
+    s1 = torch.from_numpy(r.standard_normal((M, n_out)).astype(np.float16)).to(dev) if epi == "shortcut" else None
+    out = torch.empty((M, n_out), dtype=torch.float16, device=dev)
+    for _ in range(times):
+        kernels.conv1x1(a, w, b, epi=epi, s1=s1, out=out)
+    torch.cuda.synchronize()
+
+
 def main(argv):
     if argv[0] == "stats":
         print(window_stats_markdown(argv[1], argv[2], " ".join(argv[3:])), end="")
     elif argv[0] == "traffic":
         print(json.dumps(traffic_per_launch(argv[1], argv[2], argv[3], int(argv[4])), indent=1))
+    elif argv[0] == "launch":
+        launch(argv[1], int(argv[2]), int(argv[3]), int(argv[4]), int(argv[5]))
     elif argv[0] == "counters":
         print(json.dumps(counters_per_launch(argv[1], argv[2], argv[3].split(",")), indent=1))
     else:
