@@ -24,6 +24,7 @@ import ctypes as C
 import json
 import os
 import sys
+import threading
 import time
 from pathlib import Path
 
@@ -40,6 +41,7 @@ DW_CANDIDATES = (4, 8)
 COLS = 15  # trace record width (include/dcvc_codec.h)
 
 _loaded = None
+_load_lock = threading.Lock()
 
 
 def install(db: dict) -> tuple[int, int]:
@@ -63,19 +65,17 @@ def ensure_loaded(path: str | os.PathLike | None = None):
     set_param).  DCVC_TUNING_DB names another file, or 'off' for the built-in
     heuristics.  A missing file means the heuristics."""
     global _loaded
-    if _loaded is not None:
+    with _load_lock:
+        if _loaded is not None:
+            return _loaded
+        env = os.environ.get("DCVC_TUNING_DB", "")
+        p = Path(path or env or DB_PATH)
+        if env.lower() == "off" or not p.exists():
+            _loaded = (0, 0)
+        else:
+            with open(p) as f:
+                _loaded = install(json.load(f))
         return _loaded
-    env = os.environ.get("DCVC_TUNING_DB", "")
-    if env.lower() == "off":
-        _loaded = (0, 0)
-        return _loaded
-    p = Path(path or env or DB_PATH)
-    if not p.exists():
-        _loaded = (0, 0)
-        return _loaded
-    with open(p) as f:
-        _loaded = install(json.load(f))
-    return _loaded
 
 
 # ----------------------------------------------------------------- tuner
