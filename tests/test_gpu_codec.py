@@ -69,7 +69,7 @@ def test_dmci_roundtrip_bitexact_and_oracle(gpu, models):
         gc = enc.debug_copy(4 + step, P, np.uint8)
         cond_match.append(np.mean(gc == r.steps_cond[step].astype(np.uint8)))
     print("DMCI vs FP32 oracle per step: symbol", match, "index", idx_match, "skip-cond", cond_match)
-    # symbol / index agreement is judged against the fp16 storage floor in
+    # symbol / index agreement is checked against the fp16 storage floor in
     # tests/test_gpu_parity.py; here every step must code the same y
     # positions nearly everywhere (scales on the same side of skip_thres)
     assert min(cond_match) > 0.99

@@ -156,7 +156,7 @@ def test_htl_symbols_and_psnr_vs_oracle(gpu, models):
     # error e moves the index with probability ~ e / 0.039, the log-bin width,
     # and every step's spatial prior sees the earlier steps' differences)
     assert min(sym_c) > 0.985 and min(idx_c) > 0.90
-    # end to end, symbols / indexes and bytes are judged against the fp16
+    # end to end, symbols / indexes and bytes are checked against the fp16
     # storage floor in tests/test_gpu_parity.py
     assert max(abs(a - b) for a, b in zip(pg, po)) <= 0.02
 

@@ -132,7 +132,7 @@ def test_ld_symbols_and_psnr_vs_oracle(gpu, models):
     # indexes 94.4 %; conditioned 99.91 % / 99.13 %; PSNR within 0.001 dB
     assert y_rel < 4e-3 and z_match > 0.99
     assert sym_c > 0.995 and idx_c > 0.95
-    # end to end, symbols / indexes are judged against the fp16 storage
+    # end to end, symbols / indexes are checked against the fp16 storage
     # floor in tests/test_gpu_parity.py
     assert abs(len(bits) - n_o) <= 0.005 * n_o
     assert abs(pg - po) <= 0.02
