@@ -39,6 +39,10 @@ SIGNATURES = {
     "dcvc_conv_bias": [P, I, I, I, I, I, P, I, P, I, I, P, I, P],
     "dcvc_weights_to_lowp": [P, I, I, I, P, P, P],
     "dcvc_gemm_lowp": [P, I, I, I, P, I, P, I, I, P, P, I, P, I, P, I, P, P, I, P],
+    "dcvc_gemm_fp8_act": [P, I, I, I, I, P, I, P, I, I, P, P, I, P, I, P, I, P, P, I, P, I, P],
+    "dcvc_d3x3_fp8_out": [P, I, I, I, I, I, P, P, I, P, I, P],
+    "dcvc_set_fp8_activations": [I],
+    "dcvc_get_fp8_activations": [],
     "dcvc_transposed_conv": [P, I, I, I, I, I, P, I, P, P, I, P],
     "dcvc_subpel_conv3x3": [P, I, I, I, I, I, P, I, P, P, I, P],
     "dcvc_d3x3": [P, I, I, I, I, I, P, P, I, P],

@@ -8,8 +8,10 @@ struct DwArgs {
     const f16* x = nullptr;  // [n_img][H][W] pixels, ldx halves apart, C channels
     int ldx = 0;
     const f16* wt = nullptr; // [9][C] taps (ky * 3 + kx), channel contiguous
-    f16* y = nullptr;
+    f16* y = nullptr;        // nullptr: no fp16 output (y8 only)
     int ldy = 0;
+    uint8_t* y8 = nullptr;   // optional e4m3 copy of the output (fp8x8_from_f16x8), ldy8 bytes apart
+    int ldy8 = 0;
     int n_img = 1, H = 0, W = 0, C = 0;
 };
 

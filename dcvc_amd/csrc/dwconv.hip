@@ -16,6 +16,7 @@
 #include <unordered_map>
 
 #include "dwconv.h"
+#include "fp8_dev.h"
 
 namespace dcvc {
 
@@ -60,7 +61,7 @@ __global__ __launch_bounds__(256) void dw3x3_kernel(DwArgs a) {
     f16x8 r0[3], r1[3], r2[3];
     load_row(h0 - 1, r0);
     load_row(h0, r1);
-    f16* obase = a.y + (size_t)img * a.H * a.W * a.ldy + c0;
+    const size_t pix0 = (size_t)img * a.H * a.W;
     for (int h = h0; h < h1; ++h) {
         load_row(h + 1, r2);
         f16x8 o;
@@ -78,7 +79,9 @@ __global__ __launch_bounds__(256) void dw3x3_kernel(DwArgs a) {
             s += wt[8][e] * (float)r2[2][e];
             o[e] = (f16)s;
         }
-        *reinterpret_cast<f16x8*>(obase + ((size_t)h * a.W + w) * a.ldy) = o;
+        const size_t pix = pix0 + (size_t)h * a.W + w;
+        if (a.y) *reinterpret_cast<f16x8*>(a.y + pix * a.ldy + c0) = o;
+        if (a.y8) *reinterpret_cast<int2*>(a.y8 + pix * a.ldy8 + c0) = fp8x8_from_f16x8(o);
 #pragma unroll
         for (int t = 0; t < 3; ++t) {
             r0[t] = r1[t];
@@ -88,10 +91,12 @@ __global__ __launch_bounds__(256) void dw3x3_kernel(DwArgs a) {
 }
 
 int dw3x3_validate(const DwArgs& a) {
-    if (!a.x || !a.y || !a.wt) return DCVC_ERR_ARG;
+    if (!a.x || !a.wt || (!a.y && !a.y8)) return DCVC_ERR_ARG;
     if (a.C <= 0 || a.C % 8 || a.H <= 0 || a.W <= 0 || a.n_img <= 0) return DCVC_ERR_ARG;
-    if (a.ldx < a.C || a.ldy < a.C || a.ldx % 8 || a.ldy % 8) return DCVC_ERR_ARG;
-    if (((uintptr_t)a.x) % 16 || ((uintptr_t)a.y) % 16 || ((uintptr_t)a.wt) % 16) return DCVC_ERR_ARG;
+    if (a.ldx < a.C || a.ldx % 8) return DCVC_ERR_ARG;
+    if (((uintptr_t)a.x) % 16 || ((uintptr_t)a.wt) % 16) return DCVC_ERR_ARG;
+    if (a.y && (a.ldy < a.C || a.ldy % 8 || ((uintptr_t)a.y) % 16)) return DCVC_ERR_ARG;
+    if (a.y8 && (a.ldy8 < a.C || a.ldy8 % 8 || ((uintptr_t)a.y8) % 8)) return DCVC_ERR_ARG;
     return DCVC_OK;
 }
 

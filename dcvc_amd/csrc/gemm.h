@@ -22,13 +22,16 @@ enum GemmAMode {
     A_S1K3 = 3,   // 3x3 stride-1 pad-1 conv, K = 9 * cin, tap-major
 };
 
-// Operand precision of the product (activations stay fp16 in memory and are
-// converted as fragments leave LDS; weights are stored in the precision).
+// Operand precision of the product.  Weights are stored in the precision;
+// activations are fp16 in memory and converted as fragments leave LDS, except
+// under PREC_FP8A, where a producer has already stored them as e4m3.
 enum GemmPrec {
     PREC_F16 = 0,   // fp16 x fp16, v_mfma_f32_32x32x16_f16
     PREC_BF16 = 1,  // bf16 x bf16, v_mfma_f32_32x32x16_bf16
     PREC_FP8 = 2,   // OCP fp8 e4m3 x e4m3, v_mfma_scale_f32_32x32x64_f8f6f4; weights carry
                     // one fp32 scale per row (max |w| / 448), activations are taken at scale 1
+    PREC_FP8A = 3,  // PREC_FP8 with A already e4m3 bytes in memory (lda in bytes), as
+                    // fp8x8_from_f16x8 makes them: the same bits as PREC_FP8 on the fp16 A
 };
 
 enum GemmDMode {
@@ -55,8 +58,10 @@ struct GemmArgs {
     const int* qp = nullptr;    // device QP index (nullptr -> row 0)
     int q_round = 0;            // quant epilogues: round to fp16 before the q multiply,
                                 // as a separate fp16 multiply_with_broadcast would
-    f16* D = nullptr;
+    f16* D = nullptr;                // nullptr: no fp16 output (D8 only)
     int ldd = 0;
+    uint8_t* D8 = nullptr;           // optional e4m3 copy of the fp16 output (D_PLAIN, one problem),
+    int ldd8 = 0;                    // row stride in bytes: the A operand of a later PREC_FP8A product
     int prec = PREC_F16;             // GemmPrec; B is bf16 / fp8 bytes accordingly
     const float* wscale = nullptr;   // PREC_FP8: per B row dequantisation scale
     // `batch` independent problems of this shape in one launch (A_PLAIN /
@@ -70,6 +75,11 @@ struct GemmArgs {
 };
 
 int gemm_validate(const GemmArgs& g, int epi, int amode, int dmode);
+// Whether the codec layers in PREC_FP8 store e4m3 copies of their
+// activations for the next product (PREC_FP8A) instead of converting fp16
+// operands inside every product.  Same bits either way; default on.
+void set_fp8_activations(bool on);
+bool fp8_activations();
 hipError_t gemm_launch(const GemmArgs& g, int epi, int amode, int dmode, hipStream_t stream);
 // 0: automatic tile choice; 1-7: force one tile configuration (tests, tuning)
 int gemm_set_tile_policy(int policy);

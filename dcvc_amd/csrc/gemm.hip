@@ -41,6 +41,7 @@
 //   transposed_conv (2x2 s2).  Epilogue op order follows
 //   cutlass/cutlass_epilogue.h:79-131,175-224: (acc + b) [-> wsilu] [+ S1]
 //   [+ S2 | * q].
+#include <atomic>
 #include <mutex>
 #include <shared_mutex>
 #include <unordered_map>
@@ -48,6 +49,7 @@
 #include "common.h"
 #include "gemm.h"
 #include "gemm_dev.h"
+#include "fp8_dev.h"
 
 namespace dcvc {
 
@@ -63,7 +65,7 @@ std::shared_mutex g_tune_mu;
 std::unordered_map<uint64_t, int> g_tune;
 
 bool tune_key(int prec, int epi, int amode, int dmode, long long M, long long N, long long K, uint64_t* key) {
-    if (prec < 0 || prec > 2 || epi < 0 || epi > 15 || amode < 0 || amode > 3 || dmode < 0 || dmode > 1) return false;
+    if (prec < 0 || prec > 3 || epi < 0 || epi > 15 || amode < 0 || amode > 3 || dmode < 0 || dmode > 1) return false;
     if (M <= 0 || M >= (1LL << 24) || N <= 0 || N >= (1LL << 15) || K <= 0 || K >= (1LL << 16)) return false;
     *key = ((uint64_t)prec << 62) | ((uint64_t)epi << 58) | ((uint64_t)amode << 56) | ((uint64_t)dmode << 55) |
            ((uint64_t)M << 31) | ((uint64_t)N << 16) | (uint64_t)K;
@@ -73,7 +75,6 @@ bool tune_key(int prec, int epi, int amode, int dmode, long long M, long long N,
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef int i32x8 __attribute__((ext_vector_type(8)));
 typedef int i32x4 __attribute__((ext_vector_type(4)));
-typedef short s16x2 __attribute__((ext_vector_type(2)));
 
 // 8 fp16 -> 8 bf16 (through fp32, round to nearest even)
 DCVC_DEV bf16x8 to_bf16x8(f16x8 v) {
@@ -83,22 +84,14 @@ DCVC_DEV bf16x8 to_bf16x8(f16x8 v) {
     return o;
 }
 
-// 32 fp16 (k order) -> 32 OCP e4m3 bytes in the same order, saturated at
-// +-448 (the largest finite e4m3) before conversion
+// 32 fp16 (k order) -> 32 OCP e4m3 bytes in the same order (fp8x8_from_f16x8)
 DCVC_DEV i32x8 to_fp8x32(const f16x8 (&v)[4]) {
     i32x8 o;
-    const f16x2 lim = {(f16)448.f, (f16)448.f};
 #pragma unroll
-    for (int d = 0; d < 8; ++d) {
-        const f16x8& src = v[d >> 1];
-        const int e = (d & 1) * 4;
-        f16x2 lo = {src[e], src[e + 1]}, hi = {src[e + 2], src[e + 3]};
-        lo = __builtin_elementwise_max(__builtin_elementwise_min(lo, lim), -lim);
-        hi = __builtin_elementwise_max(__builtin_elementwise_min(hi, lim), -lim);
-        s16x2 t = {0, 0};
-        t = __builtin_amdgcn_cvt_scalef32_pk_fp8_f16(t, lo, 1.0f, false);
-        t = __builtin_amdgcn_cvt_scalef32_pk_fp8_f16(t, hi, 1.0f, true);
-        o[d] = __builtin_bit_cast(int, t);
+    for (int c = 0; c < 4; ++c) {
+        const int2 p = fp8x8_from_f16x8(v[c]);
+        o[2 * c] = p.x;
+        o[2 * c + 1] = p.y;
     }
     return o;
 }
@@ -109,26 +102,28 @@ template <int BM_, int BN_, int WM_, int WN_, int STAGES_, int EPI_, int AMODE_,
 struct Tile {
     static constexpr int BM = BM_, BN = BN_, WM = WM_, WN = WN_;
     static constexpr int PREC = PREC_;
-    static constexpr bool kFp8 = (PREC_ == PREC_FP8);
+    static constexpr bool kFp8 = (PREC_ == PREC_FP8 || PREC_ == PREC_FP8A);
+    static constexpr bool kA8 = (PREC_ == PREC_FP8A);  // A is e4m3 bytes in memory
     // K per ring tile: 32 (one 64-byte fp16 row), or 64 for fp8, whose B row
-    // is 64 fp8 bytes and whose A tile is two fp16 half tiles
+    // is 64 fp8 bytes and whose A tile is two fp16 half tiles (one 64-byte
+    // row of e4m3 bytes under kA8)
     static constexpr int kKT = kFp8 ? 64 : 32;
-    static constexpr int kAH = kFp8 ? 2 : 1;
+    static constexpr int kAH = (kFp8 && !kA8) ? 2 : 1;
     // LDS ring of STAGES K tiles, STAGES - 2 in flight ahead of the one being
     // read: the slot refilled at iteration kt is that of tile kt - 2, whose
     // fragment reads every wave consumed before this iteration's barrier.
     // STAGES == 3 keeps one tile ahead and refills the slot of tile kt - 1,
     // so its barrier also waits for the wave's own fragment reads (lgkmcnt).
-    // (fp8 converts every fragment inside its own iteration, so its reads are
-    // complete before the next barrier and STAGES - 1 tiles can be in flight)
+    // (fp8 reads a whole tile's fragments one iteration ahead of their
+    // MFMAs; its barrier waits for them, and STAGES - 1 tiles are in flight)
     static constexpr int kStages = STAGES_;
     static constexpr int kAhead = (STAGES_ >= 5 && !kFp8) ? STAGES_ - 2 : STAGES_ - 1;
-    static constexpr bool kWaitLds = !kFp8 && (kAhead == STAGES_ - 1);
+    static constexpr bool kWaitLds = kFp8 || (kAhead == STAGES_ - 1);
     // configurations meant to run 2 blocks per CU keep within 256 registers
     // per lane (architectural + accumulation) so two waves share each SIMD
     // (and 8-wave configurations run two waves on every SIMD of their CU)
     static constexpr int kWavesPerSimd =
-        (WM_ * WN_ >= 8 || (STAGES_ == 3 && PREC_ != PREC_FP8) || (BM_ * BN_ <= 128 * 128)) ? 2 : 1;
+        (WM_ * WN_ >= 8 || (STAGES_ == 3 && !kFp8) || (BM_ * BN_ <= 128 * 128)) ? 2 : 1;
     static constexpr int EPI = EPI_, AMODE = AMODE_, DMODE = DMODE_;
     static constexpr int kWaves = WM * WN;
     static constexpr int kThreads = kWaves * 64;
@@ -202,7 +197,9 @@ __global__ __launch_bounds__(T::kThreads, T::kWavesPerSimd) void gemm_kernel(Gem
 #pragma unroll
     for (int q = 0; q < T::AI; ++q) {
         const int m = m0 + wave * T::ARW + q * 16 + dr;
-        if constexpr (AMODE == A_PLAIN) {
+        if constexpr (AMODE == A_PLAIN && T::kA8) {
+            a_off[q] = (m < g.M) ? (uint32_t)((size_t)m * g.lda + kc * 16) : kOOB;
+        } else if constexpr (AMODE == A_PLAIN) {
             a_off[q] = (m < g.M) ? (uint32_t)(((size_t)m * g.lda + kc * 8) * 2) : kOOB;
         } else {
             const int hw = g.outH * g.outW;
@@ -265,7 +262,7 @@ __global__ __launch_bounds__(T::kThreads, T::kWavesPerSimd) void gemm_kernel(Gem
             if constexpr (AMODE == A_PLAIN) {
 #pragma unroll
                 for (int q = 0; q < T::AI; ++q)
-                    dma16(rA, sa + q * 16 * kRowBytes, (a_off[q] + (uint32_t)k0 * 2) | past);
+                    dma16(rA, sa + q * 16 * kRowBytes, (a_off[q] + (uint32_t)k0 * (T::kA8 ? 1 : 2)) | past);
             } else {
                 // cin % 64 == 0, so one K tile never straddles two taps
                 const int tap = k0 / g.cin;
@@ -341,24 +338,29 @@ __global__ __launch_bounds__(T::kThreads, T::kWavesPerSimd) void gemm_kernel(Gem
     // bytes (chunks 2h, 2h + 1 of the 64-byte row).  Any k order works as
     // long as A and B use the same one.
     const int a8_row = ((lane >> 5) * BM + wm * T::TM + lr) * kRowBytes;
+    // e4m3 row: the lane's 32 bytes are chunks 2h, 2h + 1 (h = lane >> 5)
+    auto read_row8 = [&](const char* rowp) {
+        const int c0 = 2 * (lane >> 5);
+        const i32x4 lo = *reinterpret_cast<const i32x4*>(rowp + ((c0 ^ swz(lr)) * 16));
+        const i32x4 hi = *reinterpret_cast<const i32x4*>(rowp + (((c0 + 1) ^ swz(lr)) * 16));
+        return __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+    };
     auto read_fp8 = [&](int st, i32x8* fa, i32x8* fb) {
         const char* base = smem + st * T::kStageBytes;
 #pragma unroll
         for (int i = 0; i < SM; ++i) {
-            f16x8 v[4];
+            if constexpr (T::kA8) {
+                fa[i] = read_row8(base + fa_row + i * 32 * kRowBytes);
+            } else {
+                f16x8 v[4];
 #pragma unroll
-            for (int c = 0; c < 4; ++c)
-                v[c] = *reinterpret_cast<const f16x8*>(base + a8_row + i * 32 * kRowBytes + ((c ^ swz(lr)) * 16));
-            fa[i] = to_fp8x32(v);
+                for (int c = 0; c < 4; ++c)
+                    v[c] = *reinterpret_cast<const f16x8*>(base + a8_row + i * 32 * kRowBytes + ((c ^ swz(lr)) * 16));
+                fa[i] = to_fp8x32(v);
+            }
         }
 #pragma unroll
-        for (int j = 0; j < SN; ++j) {
-            const char* rowp = base + fb_row + j * 32 * kRowBytes;
-            const int c0 = 2 * (lane >> 5);
-            const i32x4 lo = *reinterpret_cast<const i32x4*>(rowp + ((c0 ^ swz(lr)) * 16));
-            const i32x4 hi = *reinterpret_cast<const i32x4*>(rowp + (((c0 + 1) ^ swz(lr)) * 16));
-            fb[j] = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
-        }
+        for (int j = 0; j < SN; ++j) fb[j] = read_row8(base + fb_row + j * 32 * kRowBytes);
     };
     auto mfma_fp8 = [&](const i32x8* fa, const i32x8* fb) {
 #pragma unroll
@@ -426,11 +428,11 @@ __global__ __launch_bounds__(T::kThreads, T::kWavesPerSimd) void gemm_kernel(Gem
     for (int st = 0; st < T::kAhead; ++st) dma_tile(st, st);
     if constexpr (T::kFp8) {
         i32x8 pa[SM], pb[SN];
-        wait_tile_and_barrier<(T::kAhead - 1) * T::kDma, false>();
+        wait_tile_and_barrier<(T::kAhead - 1) * T::kDma, T::kWaitLds>();
         dma_tile(T::kAhead, T::kAhead);
         read_fp8(0, pa, pb);
         for (int kt = 1; kt < KT; ++kt) {
-            wait_tile_and_barrier<(T::kAhead - 1) * T::kDma, false>();
+            wait_tile_and_barrier<(T::kAhead - 1) * T::kDma, T::kWaitLds>();
             dma_tile(kt + T::kAhead, (kt + T::kAhead) % T::kStages);
             i32x8 na[SM], nb[SN];
             read_fp8(kt % T::kStages, na, nb);
@@ -547,8 +549,9 @@ __global__ __launch_bounds__(T::kThreads, T::kWavesPerSimd) void gemm_kernel(Gem
                 off = ((size_t)(img * 2 * g.outH + 2 * h + (sub >> 1)) * (2 * g.outW) + 2 * w + (sub & 1)) * g.ldd + ch;
             } else {
                 off = (size_t)m * g.ldd + n;
+                if (g.D8) *reinterpret_cast<int2*>(g.D8 + (size_t)m * g.ldd8 + n) = fp8x8_from_f16x8(o);
             }
-            *reinterpret_cast<f16x8*>(g.D + off) = o;
+            if (g.D) *reinterpret_cast<f16x8*>(g.D + off) = o;
         }
     }
 }
@@ -618,7 +621,7 @@ static int pick_tile(const GemmArgs& g, int epi, int amode, int dmode) {
 template <int EPI, int AMODE, int DMODE, int PREC = PREC_F16>
 static hipError_t dispatch_tiles(const GemmArgs& g, hipStream_t stream) {
     const int t = pick_tile(g, EPI, AMODE, DMODE);
-    if constexpr (PREC == PREC_FP8) {
+    if constexpr (PREC == PREC_FP8 || PREC == PREC_FP8A) {
         // fp8 K tiles are 48 KB (256 x 256) / 24 KB (128 x 128): 3 stages.  The
         // shortcut epilogues (HBM-bound) stay on 128 x 128, whose registers
         // also hold the shortcut tile.
@@ -683,8 +686,9 @@ hipError_t gemm_launch(const GemmArgs& g_in, int epi, int amode, int dmode, hipS
         a_rows = (size_t)(g.M / (g.outH * g.outW)) * g.inH * g.inW;
         a_last = g.cin;
     }
-    g.a_bytes = (uint32_t)(((a_rows - 1) * g.lda + a_last) * 2);
-    g.b_bytes = (uint32_t)((size_t)(epi == EPI_QUAD ? 4 * g.N : g.N) * g.K * (g.prec == PREC_FP8 ? 1 : 2));
+    const bool w8 = (g.prec == PREC_FP8 || g.prec == PREC_FP8A);
+    g.a_bytes = (uint32_t)(((a_rows - 1) * g.lda + a_last) * (g.prec == PREC_FP8A ? 1 : 2));
+    g.b_bytes = (uint32_t)((size_t)(epi == EPI_QUAD ? 4 * g.N : g.N) * g.K * (w8 ? 1 : 2));
     // implicit-conv A modes and the sub-pixel store come with the plain bias
     // epilogue only (conv_bias, transposed_conv)
     if (amode != A_PLAIN || dmode != D_PLAIN) {
@@ -705,6 +709,7 @@ hipError_t gemm_launch(const GemmArgs& g_in, int epi, int amode, int dmode, hipS
     case PREC_F16: return dispatch_plain<PREC_F16>(g, epi, stream);
     case PREC_BF16: return dispatch_plain<PREC_BF16>(g, epi, stream);
     case PREC_FP8: return dispatch_plain<PREC_FP8>(g, epi, stream);
+    case PREC_FP8A: return dispatch_plain<PREC_FP8A>(g, epi, stream);
     default: return hipErrorInvalidValue;
     }
 }
@@ -713,13 +718,18 @@ static bool aligned16(const void* p) { return ((uintptr_t)p) % 16 == 0; }
 
 // Host-side validation shared by every GEMM entry point.
 int gemm_validate(const GemmArgs& g, int epi, int amode, int dmode) {
-    if (!g.A || !g.B || !g.D || !g.bias) return DCVC_ERR_ARG;
+    if (!g.A || !g.B || !g.bias || (!g.D && !g.D8)) return DCVC_ERR_ARG;
     if ((amode != A_PLAIN || dmode != D_PLAIN) && (epi != EPI_BIAS || g.prec != PREC_F16)) return DCVC_ERR_ARG;
-    if (g.prec < PREC_F16 || g.prec > PREC_FP8 || (g.prec == PREC_FP8 && !g.wscale)) return DCVC_ERR_ARG;
+    const bool w8 = (g.prec == PREC_FP8 || g.prec == PREC_FP8A);
+    if (g.prec < PREC_F16 || g.prec > PREC_FP8A || (w8 && !g.wscale)) return DCVC_ERR_ARG;
     if (g.M <= 0 || g.N <= 0 || g.K <= 0) return DCVC_ERR_ARG;
     if (g.K % 64 != 0 || g.N % 8 != 0) return DCVC_ERR_ARG;
-    if (g.lda % 8 != 0 || g.lda < (amode == A_PLAIN ? g.K : g.cin)) return DCVC_ERR_ARG;
-    if (!aligned16(g.A) || !aligned16(g.B) || !aligned16(g.D) || g.ldd % 8 != 0) return DCVC_ERR_ARG;
+    // e4m3 A rows (lda in bytes) keep the 16-byte DMA alignment
+    if (g.lda % (g.prec == PREC_FP8A ? 16 : 8) != 0 || g.lda < (amode == A_PLAIN ? g.K : g.cin)) return DCVC_ERR_ARG;
+    if (!aligned16(g.A) || !aligned16(g.B)) return DCVC_ERR_ARG;
+    if (g.D && (!aligned16(g.D) || g.ldd % 8 != 0)) return DCVC_ERR_ARG;
+    if (g.D8 && (dmode != D_PLAIN || g.batch != 1 || g.ldd8 < g.N || g.ldd8 % 8 != 0 || ((uintptr_t)g.D8) % 8 != 0))
+        return DCVC_ERR_ARG;
     size_t a_rows = (size_t)g.M;
     if (amode != A_PLAIN) {
         const int taps = (amode == A_S2K2) ? 4 : 9;
@@ -732,13 +742,13 @@ int gemm_validate(const GemmArgs& g, int epi, int amode, int dmode) {
         a_rows = (size_t)(g.M / (g.outH * g.outW)) * g.inH * g.inW;
     }
     // buffer addressing uses 32-bit byte offsets below 2^31
-    if (a_rows * (size_t)g.lda * 2 >= kOOB) return DCVC_ERR_ARG;
+    if (a_rows * (size_t)g.lda * (g.prec == PREC_FP8A ? 1 : 2) >= kOOB) return DCVC_ERR_ARG;
     if ((size_t)(epi == EPI_QUAD ? 4 * g.N : g.N) * g.K * 2 >= kOOB) return DCVC_ERR_ARG;
     if (dmode == D_SUBPEL) {
         if (amode != A_PLAIN && amode != A_S1K3) return DCVC_ERR_ARG;
         if (g.N % 32 != 0 || g.outH <= 0 || g.outW <= 0 || g.M % (g.outH * g.outW) != 0) return DCVC_ERR_ARG;
         if (g.ldd < g.N / 4) return DCVC_ERR_ARG;
-    } else if (g.ldd < g.N) {
+    } else if (g.D && g.ldd < g.N) {
         return DCVC_ERR_ARG;
     }
     if (epi == EPI_SHORTCUT || epi == EPI_SHORTCUT2 || epi == EPI_SHORTCUT_QUANT) {
@@ -749,8 +759,9 @@ int gemm_validate(const GemmArgs& g, int epi, int amode, int dmode) {
     if (g.batch != 1) {
         // every problem's operands keep the 16-byte alignment of the first
         if (g.batch < 1 || g.batch > 65535 || g.a_div < 1 || amode != A_PLAIN || dmode != D_PLAIN) return DCVC_ERR_ARG;
+        if (g.prec == PREC_FP8A || !g.D) return DCVC_ERR_ARG;
         if (g.sA < 0 || g.sB < 0 || g.sBias < 0 || g.sS1 < 0 || g.sS2 < 0 || g.sD < 0) return DCVC_ERR_ARG;
-        const long wb = (g.prec == PREC_FP8) ? 1 : 2;
+        const long wb = w8 ? 1 : 2;
         if (g.sA % 8 || g.sD % 8 || g.sS1 % 8 || g.sS2 % 8 || (g.sB * wb) % 16) return DCVC_ERR_ARG;
     }
     return DCVC_OK;
@@ -805,6 +816,13 @@ hipError_t weights_to_fp8(const f16* w, int rows, int K, void* out, float* scale
                        scale);
     return hipGetLastError();
 }
+
+namespace {
+std::atomic<bool> g_fp8_act{true};
+}
+
+void set_fp8_activations(bool on) { g_fp8_act.store(on); }
+bool fp8_activations() { return g_fp8_act.load(); }
 
 int gemm_set_tile_policy(int policy) {
     if (policy < 0 || policy > kPolicies) return DCVC_ERR_ARG;

@@ -92,7 +92,7 @@ int dcvc_gemm_tune_set(const long long* entries, int n) { return gemm_tune_set(e
 
 int dcvc_gemm_tile_for(int prec, int epi, int amode, int dmode, long M, int N, int K) {
     if (M <= 0 || N <= 0 || K <= 0 || epi < EPI_BIAS || epi > EPI_QUAD || amode < A_PLAIN || amode > A_S1K3 ||
-        dmode < D_PLAIN || dmode > D_SUBPEL || prec < PREC_F16 || prec > PREC_FP8 || M > 0x7fffffffL)
+        dmode < D_PLAIN || dmode > D_SUBPEL || prec < PREC_F16 || prec > PREC_FP8A || M > 0x7fffffffL)
         return -DCVC_ERR_ARG;
     GemmArgs g;
     g.M = (int)M;
@@ -261,6 +261,35 @@ int dcvc_gemm_lowp(const void* A, int lda, int M, int K, const void* Wq, int N, 
     return run_gemm(g, epi, A_PLAIN, D_PLAIN, stream);
 }
 
+int dcvc_gemm_fp8_act(const void* A, int lda, int a_fp8, int M, int K, const void* Wq, int N, const float* scale,
+                      int prec, int epi, const float* bias, const void* S1, int lds1, const void* S2, int lds2,
+                      const void* q, int qstride, const int* qp, void* D, int ldd, void* D8, int ldd8, void* stream) {
+    if (epi < EPI_BIAS || epi > EPI_QUAD || prec < PREC_F16 || prec > PREC_FP8 || (a_fp8 != 0 && a_fp8 != 1))
+        return DCVC_ERR_ARG;
+    if (a_fp8 && prec != PREC_FP8) return DCVC_ERR_ARG;
+    GemmArgs g = plain_args(A, lda, M, K, Wq, N, bias, D, ldd);
+    g.prec = a_fp8 ? PREC_FP8A : prec;
+    g.wscale = scale;
+    g.S1 = H(S1);
+    g.lds1 = lds1;
+    g.S2 = H(S2);
+    g.lds2 = lds2;
+    g.q = H(q);
+    g.qstride = qstride;
+    g.qp = qp;
+    g.D8 = static_cast<uint8_t*>(D8);
+    g.ldd8 = ldd8;
+    return run_gemm(g, epi, A_PLAIN, D_PLAIN, stream);
+}
+
+int dcvc_set_fp8_activations(int on) {
+    if (on != 0 && on != 1) return DCVC_ERR_ARG;
+    set_fp8_activations(on != 0);
+    return DCVC_OK;
+}
+
+int dcvc_get_fp8_activations(void) { return fp8_activations() ? 1 : 0; }
+
 int dcvc_conv_bias(const void* X, int ldx, int n_img, int H_, int W_, int Cin, const void* Wt, int N,
                    const float* bias, int kernel, int stride, void* D, int ldd, void* stream) {
     int amode;
@@ -340,12 +369,20 @@ int dcvc_subpel_conv3x3(const void* X, int ldx, int n_img, int H_, int W_, int C
 
 int dcvc_d3x3(const void* X, int ldx, int n_img, int H_, int W_, int C, const void* Wt, void* Y, int ldy,
               void* stream) {
+    if (!Y) return DCVC_ERR_ARG;
+    return dcvc_d3x3_fp8_out(X, ldx, n_img, H_, W_, C, Wt, Y, ldy, nullptr, 0, stream);
+}
+
+int dcvc_d3x3_fp8_out(const void* X, int ldx, int n_img, int H_, int W_, int C, const void* Wt, void* Y, int ldy,
+                      void* Y8, int ldy8, void* stream) {
     DwArgs a;
     a.x = H(X);
     a.ldx = ldx;
     a.wt = H(Wt);
     a.y = Hm(Y);
     a.ldy = ldy;
+    a.y8 = static_cast<uint8_t*>(Y8);
+    a.ldy8 = ldy8;
     a.n_img = n_img;
     a.H = H_;
     a.W = W_;

@@ -113,6 +113,17 @@ struct Exec {
     int prec = PREC_F16;  // GemmPrec of the 1x1-conv products
 };
 
+// e4m3 operands of a product in the precision form (Exec::prec == PREC_FP8;
+// ignored otherwise): A read from an e4m3 copy of the activation, and an
+// e4m3 copy of the output stored beside, or instead of, the fp16 output.
+struct Fp8Io {
+    const uint8_t* a8 = nullptr;
+    int lda8 = 0;
+    uint8_t* d8 = nullptr;
+    int ldd8 = 0;
+    bool d16 = true;  // false: the e4m3 copy is the only output
+};
+
 struct Conv1x1 {
     f16* w = nullptr;
     float* b = nullptr;
@@ -127,14 +138,17 @@ struct Conv1x1 {
     // bf16 and fp8 (per-row scale) copies of the fp16 weight, on the null stream
     void make_low_precision(DevMem& mem);
     void run(const Exec& e, const Act& a, const Act& dst, int epi, const Act* s1 = nullptr,
-             const Act* s2 = nullptr, const QuantSel* q = nullptr) const;
+             const Act* s2 = nullptr, const QuantSel* q = nullptr, const Fp8Io* io = nullptr) const;
 };
 
 struct DW3x3 {
     f16* w = nullptr;  // [9][C]
     int C = 0;
     void load(DevMem& mem, const HostDict& d, const std::string& key);
-    void run(const Exec& e, const Act& in, const Act& out) const;
+    // y8: optional e4m3 copy of the output (ldy8 bytes per pixel); y16 false
+    // stores only that copy
+    void run(const Exec& e, const Act& in, const Act& out, uint8_t* y8 = nullptr, int ldy8 = 0,
+             bool y16 = true) const;
 };
 
 // DepthConvBlock (layers.py:128-159) as the proxy runs it

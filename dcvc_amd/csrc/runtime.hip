@@ -96,7 +96,7 @@ void Conv1x1::load(DevMem& mem, const HostDict& d, const std::string& prefix, bo
 }
 
 void Conv1x1::run(const Exec& e, const Act& a, const Act& dst, int epi, const Act* s1, const Act* s2,
-                  const QuantSel* q) const {
+                  const QuantSel* q, const Fp8Io* io) const {
     GemmArgs g;
     g.A = a.p;
     g.lda = a.ld;
@@ -115,6 +115,16 @@ void Conv1x1::run(const Exec& e, const Act& a, const Act& dst, int epi, const Ac
     } else if (e.prec == PREC_FP8) {
         g.B = static_cast<const f16*>(w_fp8);
         g.wscale = w_scale;
+        if (io && io->a8) {
+            g.A = reinterpret_cast<const f16*>(io->a8);
+            g.lda = io->lda8;
+            g.prec = PREC_FP8A;
+        }
+        if (io && io->d8) {
+            g.D8 = io->d8;
+            g.ldd8 = io->ldd8;
+            if (!io->d16) g.D = nullptr;
+        }
     }
     if (s1) {
         g.S1 = s1->p;
@@ -151,13 +161,15 @@ void DW3x3::load(DevMem& mem, const HostDict& d, const std::string& key) {
     w = upload_f16(mem, t);
 }
 
-void DW3x3::run(const Exec& e, const Act& in, const Act& out) const {
+void DW3x3::run(const Exec& e, const Act& in, const Act& out, uint8_t* y8, int ldy8, bool y16) const {
     DwArgs a;
     a.x = in.p;
     a.ldx = in.ld;
     a.wt = w;
-    a.y = out.p;
+    a.y = (y16 || !y8) ? out.p : nullptr;
     a.ldy = out.ld;
+    a.y8 = y8;
+    a.ldy8 = ldy8;
     a.n_img = in.n_img;
     a.H = in.H;
     a.W = in.W;
@@ -201,6 +213,9 @@ void DCB::load(DevMem& mem, const HostDict& d, const std::string& p, bool shortc
     cout = dc0.K;
     cmid = dc0.N;
     cin = has_adaptor ? ad.K : cout;
+    if (dw.C != cmid || dc3.K != cmid || dc3.N != cout || ffn0.K != cout || ffn0.N != cmid || ffn2.K != cmid ||
+        ffn2.N != cout)
+        throw std::invalid_argument(p + ": DepthConvBlock layer shapes do not chain");
 }
 
 void DCB::forward(const Exec& e, const Act& in, const Act& out, const QuantSel* q) const {
@@ -215,11 +230,47 @@ void DCB::forward(const Exec& e, const Act& in, const Act& out, const QuantSel* 
     }
     const Act t1 = make_act(sc.t1, in.n_img, in.H, in.W, cmid, cmid);
     const Act t2 = make_act(sc.t2, in.n_img, in.H, in.W, cmid, cmid);
+    const Act o = shortcut ? make_act(sc.so, in.n_img, in.H, in.W, cout, cout) : out;
+    if (shortcut && q) throw std::logic_error("DCB: quant tail with block shortcut is not used by the models");
+    if (e.prec == PREC_FP8 && fp8_activations()) {
+        // e4m3 activations between the products: the depthwise output and
+        // the quad output exist only as e4m3 (each feeds one product), dc.3's
+        // output also as an e4m3 copy (the quad's A; ffn.2 adds the fp16 one).
+        // t2's buffer holds the depthwise bytes, then dc.3's copy behind them
+        // (M * (cmid + cout) bytes fit in its M * max(cmid, cout) halves).
+        uint8_t* t2_8 = reinterpret_cast<uint8_t*>(sc.t2);
+        uint8_t* o8 = t2_8 + (size_t)in.pixels() * cmid;
+        uint8_t* t1_8 = reinterpret_cast<uint8_t*>(sc.t1);
+        dc0.run(e, x, t1, EPI_WSILU);
+        dw.run(e, t1, t2, t2_8, cmid, false);
+        Fp8Io io3;
+        io3.a8 = t2_8;
+        io3.lda8 = cmid;
+        io3.d8 = o8;
+        io3.ldd8 = cout;
+        dc3.run(e, t2, o, EPI_SHORTCUT, &x, nullptr, nullptr, &io3);
+        Fp8Io io0;
+        io0.a8 = o8;
+        io0.lda8 = cout;
+        io0.d8 = t1_8;
+        io0.ldd8 = cmid;
+        io0.d16 = false;
+        ffn0.run(e, o, t1, EPI_QUAD, nullptr, nullptr, nullptr, &io0);
+        Fp8Io io2;
+        io2.a8 = t1_8;
+        io2.lda8 = cmid;
+        if (shortcut) {
+            ffn2.run(e, t1, out, EPI_SHORTCUT2, &o, &x, nullptr, &io2);
+        } else if (q) {
+            ffn2.run(e, t1, out, EPI_SHORTCUT_QUANT, &o, nullptr, q, &io2);
+        } else {
+            ffn2.run(e, t1, out, EPI_SHORTCUT, &o, nullptr, nullptr, &io2);
+        }
+        return;
+    }
     dc0.run(e, x, t1, EPI_WSILU);
     dw.run(e, t1, t2);
-    const Act o = shortcut ? make_act(sc.so, in.n_img, in.H, in.W, cout, cout) : out;
     dc3.run(e, t2, o, EPI_SHORTCUT, &x);
-    if (shortcut && q) throw std::logic_error("DCB: quant tail with block shortcut is not used by the models");
     if (ffn_use((int)in.pixels(), cout, cmid, e.prec)) {
         // ffn.0 and ffn.2 in one launch, the hidden activation kept on chip
         FfnArgs f;

@@ -131,6 +131,38 @@ def conv1x1(a, w, bias, epi="bias", s1=None, s2=None, q=None, qp=None, out=None)
     return out
 
 
+def gemm_fp8_act(a, wq, scale, bias, epi="bias", a_fp8=True, s1=None, s2=None, q=None, qp=None, out=None,
+                 out8=None, fp16_out=True, prec="fp8"):
+    """gemm_lowp with e4m3 activations: `a` uint8 e4m3 bytes [M, K] when
+    a_fp8 (prec 'fp8' only), else fp16; out8 (uint8 [M, N]) receives an e4m3
+    copy of the output, which with fp16_out=False is the only output.
+    Returns (out or None, out8 or None)."""
+    _check(a, torch.uint8 if a_fp8 else torch.float16)
+    M, K = a.shape
+    N = wq.shape[0] // 4 if epi == "quad" else wq.shape[0]
+    if fp16_out and out is None:
+        out = torch.empty((M, N), dtype=torch.float16, device=a.device)
+    o = out if fp16_out else None
+    bias = bias.float().contiguous()
+    qs = 0 if q is None else (q.stride(0) if q.dim() == 2 else q.numel())
+    _lib_call("dcvc_gemm_fp8_act", _p(a), _ld(a), 1 if a_fp8 else 0, M, K, _p(wq.contiguous()), N, _p(scale),
+              PRECISIONS[prec], EPI_CODES[epi], _p(bias), _p(s1), _ld(s1) if s1 is not None else 0, _p(s2),
+              _ld(s2) if s2 is not None else 0, _p(q), qs, _p(qp), _p(o), _ld(o) if o is not None else 0,
+              _p(out8), _ld(out8) if out8 is not None else 0, _stream())
+    return o, out8
+
+
+def set_fp8_activations(on):
+    """Codec proxies in precision 'fp8': store e4m3 activations between
+    products (True, default) or convert fp16 inside each product (False).
+    Same bits either way; takes effect at the next graph capture."""
+    _lib_call("dcvc_set_fp8_activations", 1 if on else 0)
+
+
+def fp8_activations():
+    return bool(_lib.load().dcvc_get_fp8_activations())
+
+
 GROUPED_EPIS = {"bias": 0, "wsilu": 1, "shortcut": 2, "shortcut2": 3, "quad": 6}
 
 
@@ -235,6 +267,21 @@ def htl_quant_step(y, n_img, H, W, scales, means, qdec, ysf, yout, sym, cond, th
 def htl_restore_step(yq, n_img, H, W, C, means, qdec, ysf, yout, step):
     _lib_call("dcvc_htl_restore_step", _p(yq), n_img, H, W, C, _p(means), _ld(means), _p(qdec), _ld(qdec), _p(ysf),
               _ld(ysf), _p(yout), 0 if yout is None else _ld(yout), step, _stream())
+
+
+def d3x3_fp8_out(x, n_img, H, W, wt9c, out=None, out8=None, fp16_out=True):
+    """d3x3 that also (or only, fp16_out=False) stores an e4m3 copy of its
+    output into out8 (uint8 [n_img*H*W, C]); returns (out or None, out8)."""
+    C = wt9c.shape[1]
+    M = n_img * H * W
+    if fp16_out and out is None:
+        out = torch.empty((M, C), dtype=torch.float16, device=x.device)
+    if out8 is None:
+        out8 = torch.empty((M, C), dtype=torch.uint8, device=x.device)
+    o = out if fp16_out else None
+    _lib_call("dcvc_d3x3_fp8_out", _p(x), _ld(x), n_img, H, W, C, _p(wt9c.contiguous()), _p(o),
+              _ld(o) if o is not None else 0, _p(out8), _ld(out8), _stream())
+    return o, out8
 
 
 def d3x3(x, n_img, H, W, wt9c, out=None):

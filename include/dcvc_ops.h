@@ -41,7 +41,8 @@ int dcvc_gemm_set_tile_policy(int policy);
  * HINT_MAP_D3x3 (cutlass/d3x3.cu:7-441), which launch_cutlass_helper
  * consults (cutlass/cutlass_kernel.h:233-249).  dcvc_gemm_tune_set replaces
  * the GEMM table with n entries of 8 int64 {prec, epi, amode, dmode, M, N, K,
- * config (1-7)}: a GEMM of that key launches with that configuration, other
+ * config (1-7)} (prec as dcvc_gemm_lowp, or 3: prec 2 on e4m3 activations,
+ * dcvc_gemm_fp8_act): a GEMM of that key launches with that configuration, other
  * shapes keep the built-in heuristic (n = 0 clears).  dcvc_d3x3_tune_set does
  * the same for the depthwise strip length, entries {n_img, H, W, C, rows (4
  * or 8)}.  The *_for functions return the choice a launch makes now (or
@@ -142,6 +143,28 @@ int dcvc_weights_to_lowp(const void* W, int rows, int K, int prec, void* Wq, flo
 int dcvc_gemm_lowp(const void* A, int lda, int M, int K, const void* Wq, int N, const float* scale, int prec,
                    int epi, const float* bias, const void* S1, int lds1, const void* S2, int lds2, const void* q,
                    int qstride, const int* qp, void* D, int ldd, void* stream);
+
+/* fp8 activations in memory (precision form; no reference counterpart).
+ * dcvc_gemm_fp8_act is dcvc_gemm_lowp with two additions:
+ *   a_fp8 = 1: A holds e4m3 bytes (row stride lda in bytes, lda % 16 == 0)
+ *     as an earlier D8 / Y8 output stored them; requires prec 2, and gives
+ *     the same bits as prec 2 on the fp16 activations they were made from;
+ *   D8 (may be NULL): an e4m3 copy of the fp16 output [M x N] (row stride
+ *     ldd8 bytes, % 8 == 0), saturated at +-448, round to nearest even at
+ *     scale 1; D may then be NULL (e4m3 output only).
+ * dcvc_d3x3_fp8_out is dcvc_d3x3 with the same optional Y8 copy (Y may be
+ * NULL when Y8 is given).  dcvc_set_fp8_activations(0) makes the codec
+ * proxies in precision 2 convert fp16 activations inside every product
+ * instead of storing e4m3 copies (default 1; the bits are the same either
+ * way; takes effect at the next graph capture).  dcvc_get_fp8_activations
+ * returns the setting. */
+int dcvc_gemm_fp8_act(const void* A, int lda, int a_fp8, int M, int K, const void* Wq, int N, const float* scale,
+                      int prec, int epi, const float* bias, const void* S1, int lds1, const void* S2, int lds2,
+                      const void* q, int qstride, const int* qp, void* D, int ldd, void* D8, int ldd8, void* stream);
+int dcvc_d3x3_fp8_out(const void* X, int ldx, int n_img, int H, int W, int C, const void* Wt, void* Y, int ldy,
+                      void* Y8, int ldy8, void* stream);
+int dcvc_set_fp8_activations(int on);
+int dcvc_get_fp8_activations(void);
 
 /* conv_bias (cutlass/conv_bias.cu:131-151): k x k conv, stride s, pad (k-1)/2,
  * over n_img NHWC images of H x W x Cin.  (k, s) in {(2,2), (3,2), (3,1)}.

@@ -49,7 +49,7 @@ def test_gemm_table_rejects_malformed_entries():
     lib = _lib.load()
     h = lib.dcvc_gemm_tile_for(F16, EPI_QUAD, 0, 0, 8160, 768, 768)
     for bad in ([F16, EPI_QUAD, 0, 0, 8160, 768, 768, 0], [F16, EPI_QUAD, 0, 0, 8160, 768, 768, 8],
-                [3, EPI_QUAD, 0, 0, 8160, 768, 768, 1], [F16, EPI_QUAD, 0, 0, 0, 768, 768, 1],
+                [4, EPI_QUAD, 0, 0, 8160, 768, 768, 1], [F16, EPI_QUAD, 0, 0, 0, 768, 768, 1],
                 [F16, EPI_QUAD, 0, 0, 1 << 24, 768, 768, 1]):
         assert _set([[F16, EPI_QUAD, 0, 0, 8160, 768, 768, 2], bad]) == 1
     # a rejected table leaves the previous one in place
