@@ -20,8 +20,9 @@
 
 namespace dcvc {
 
-template <int TH>
-__global__ __launch_bounds__(256) void dw3x3_kernel(DwArgs a) {
+// OUT8: the output also (or, with y == nullptr, only) as e4m3 bytes
+template <int TH, bool OUT8>
+DCVC_DEV void dw3x3_body(const DwArgs& a) {
     const int CG = a.C >> 3;
     const long item = (long)blockIdx.x * 256 + threadIdx.x;
     const int strips = (a.H + TH - 1) / TH;
@@ -80,14 +81,28 @@ __global__ __launch_bounds__(256) void dw3x3_kernel(DwArgs a) {
             o[e] = (f16)s;
         }
         const size_t pix = pix0 + (size_t)h * a.W + w;
-        if (a.y) *reinterpret_cast<f16x8*>(a.y + pix * a.ldy + c0) = o;
-        if (a.y8) *reinterpret_cast<int2*>(a.y8 + pix * a.ldy8 + c0) = fp8x8_from_f16x8(o);
+        if constexpr (OUT8) {
+            if (a.y) *reinterpret_cast<f16x8*>(a.y + pix * a.ldy + c0) = o;
+            *reinterpret_cast<int2*>(a.y8 + pix * a.ldy8 + c0) = fp8x8_from_f16x8(o);
+        } else {
+            *reinterpret_cast<f16x8*>(a.y + pix * a.ldy + c0) = o;
+        }
 #pragma unroll
         for (int t = 0; t < 3; ++t) {
             r0[t] = r1[t];
             r1[t] = r2[t];
         }
     }
+}
+
+template <int TH>
+__global__ __launch_bounds__(256) void dw3x3_kernel(DwArgs a) {
+    dw3x3_body<TH, false>(a);
+}
+
+template <int TH>
+__global__ __launch_bounds__(256) void dw3x3_e4m3_kernel(DwArgs a) {
+    dw3x3_body<TH, true>(a);
 }
 
 int dw3x3_validate(const DwArgs& a) {
@@ -152,10 +167,18 @@ hipError_t dw3x3_launch(const DwArgs& a, hipStream_t stream) {
         const long total = (long)(a.C / 8) * a.W * strips * a.n_img;
         return (long)((total + 255) / 256);
     };
-    if (dw3x3_strip_for(a) == 4)
-        hipLaunchKernelGGL(dw3x3_kernel<4>, dim3((unsigned)blocks_for(4)), dim3(256), 0, stream, a);
-    else
-        hipLaunchKernelGGL(dw3x3_kernel<8>, dim3((unsigned)blocks_for(8)), dim3(256), 0, stream, a);
+    const int th = dw3x3_strip_for(a);
+    const dim3 grid((unsigned)blocks_for(th));
+    if (a.y8) {
+        if (th == 4)
+            hipLaunchKernelGGL(dw3x3_e4m3_kernel<4>, grid, dim3(256), 0, stream, a);
+        else
+            hipLaunchKernelGGL(dw3x3_e4m3_kernel<8>, grid, dim3(256), 0, stream, a);
+    } else if (th == 4) {
+        hipLaunchKernelGGL(dw3x3_kernel<4>, grid, dim3(256), 0, stream, a);
+    } else {
+        hipLaunchKernelGGL(dw3x3_kernel<8>, grid, dim3(256), 0, stream, a);
+    }
     return hipGetLastError();
 }
 

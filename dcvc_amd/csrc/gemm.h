@@ -60,7 +60,7 @@ struct GemmArgs {
                                 // as a separate fp16 multiply_with_broadcast would
     f16* D = nullptr;                // nullptr: no fp16 output (D8 only)
     int ldd = 0;
-    uint8_t* D8 = nullptr;           // optional e4m3 copy of the fp16 output (D_PLAIN, one problem),
+    uint8_t* D8 = nullptr;           // optional e4m3 copy of the fp16 output (fp8 products, D_PLAIN, one problem),
     int ldd8 = 0;                    // row stride in bytes: the A operand of a later PREC_FP8A product
     int prec = PREC_F16;             // GemmPrec; B is bf16 / fp8 bytes accordingly
     const float* wscale = nullptr;   // PREC_FP8: per B row dequantisation scale

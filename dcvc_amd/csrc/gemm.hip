@@ -549,9 +549,14 @@ __global__ __launch_bounds__(T::kThreads, T::kWavesPerSimd) void gemm_kernel(Gem
                 off = ((size_t)(img * 2 * g.outH + 2 * h + (sub >> 1)) * (2 * g.outW) + 2 * w + (sub & 1)) * g.ldd + ch;
             } else {
                 off = (size_t)m * g.ldd + n;
-                if (g.D8) *reinterpret_cast<int2*>(g.D8 + (size_t)m * g.ldd8 + n) = fp8x8_from_f16x8(o);
             }
-            if (g.D) *reinterpret_cast<f16x8*>(g.D + off) = o;
+            if constexpr (T::kFp8) {
+                // fp8 products may also (or only) store an e4m3 copy
+                if (g.D8) *reinterpret_cast<int2*>(g.D8 + (size_t)m * g.ldd8 + n) = fp8x8_from_f16x8(o);
+                if (g.D) *reinterpret_cast<f16x8*>(g.D + off) = o;
+            } else {
+                *reinterpret_cast<f16x8*>(g.D + off) = o;
+            }
         }
     }
 }
@@ -728,7 +733,8 @@ int gemm_validate(const GemmArgs& g, int epi, int amode, int dmode) {
     if (g.lda % (g.prec == PREC_FP8A ? 16 : 8) != 0 || g.lda < (amode == A_PLAIN ? g.K : g.cin)) return DCVC_ERR_ARG;
     if (!aligned16(g.A) || !aligned16(g.B)) return DCVC_ERR_ARG;
     if (g.D && (!aligned16(g.D) || g.ldd % 8 != 0)) return DCVC_ERR_ARG;
-    if (g.D8 && (dmode != D_PLAIN || g.batch != 1 || g.ldd8 < g.N || g.ldd8 % 8 != 0 || ((uintptr_t)g.D8) % 8 != 0))
+    if (g.D8 && (!w8 || dmode != D_PLAIN || g.batch != 1 || g.ldd8 < g.N || g.ldd8 % 8 != 0 ||
+                 ((uintptr_t)g.D8) % 8 != 0))
         return DCVC_ERR_ARG;
     size_t a_rows = (size_t)g.M;
     if (amode != A_PLAIN) {

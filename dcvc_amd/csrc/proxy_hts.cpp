@@ -385,9 +385,9 @@ void HTSProxy::decoder_and_recon(bool recon_all, bool reset_for_encoder) {
         const Act common = make_act(m_rh_common, 1, m_H8, m_W8, 512, 512);
         const Act ra = make_act(m_rh_a, 1, m_H8, m_W8, 256, 256);
         w.rh1[3].forward(e, fp, common);
-        w.rh2[7][0].forward(e, common, ra);
-        w.rh2[7][1].forward(e, ra, ra);
-        w.rh2[7][2].forward(e, ra, ra);
+        w.rh2[7][0].forward(e, common, ra, nullptr, false, true);
+        w.rh2[7][1].forward(e, ra, ra, nullptr, true, true);
+        w.rh2[7][2].forward(e, ra, ra, nullptr, true, false);
         w.rh_head[7].run(e, ra, make_act(m_feature_i, 1, m_H8, m_W8, 192, 192), EPI_BIAS);
     }
 }
@@ -403,9 +403,9 @@ void HTSProxy::recon_frames(const Exec& e, int first, int last, f16* common_buf,
     const Act fi = make_act(m_feature_i, 1, m_H8, m_W8, 192, 192);
     for (int i = first; i < last; ++i) {
         if (i == first || i % 2 == 0) w.rh1[i / 2].forward(e, fp, common);
-        w.rh2[i][0].forward(e, common, ra);
-        w.rh2[i][1].forward(e, ra, ra);
-        w.rh2[i][2].forward(e, ra, ra);
+        w.rh2[i][0].forward(e, common, ra, nullptr, false, true);
+        w.rh2[i][1].forward(e, ra, ra, nullptr, true, true);
+        w.rh2[i][2].forward(e, ra, ra, nullptr, true, false);
         const Act head = (i == kFrames - 1) ? fi : make_act(head_buf, 1, m_H8, m_W8, 192, 192);
         w.rh_head[i].run(e, ra, head, EPI_BIAS);
         hip_check(pixel_shuffle8_clamp(head.p, 192, 1, m_H8, m_W8, 3, m_frames[i], m_ld_out, e.stream), "shuffle");

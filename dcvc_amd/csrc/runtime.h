@@ -163,7 +163,12 @@ struct DCB {
     void load(DevMem& mem, const HostDict& d, const std::string& p, bool shortcut_ = false);
     // out may equal in unless the block has an adaptor; q multiplies the
     // block output per channel (conv1x1_bias_shortcut_with_quant).
-    void forward(const Exec& e, const Act& in, const Act& out, const QuantSel* q = nullptr) const;
+    // In precision fp8 with e4m3 activations, chain_out also stores an e4m3
+    // copy of the output in the Exec's scratch, and chain_in makes dc.0 read
+    // that copy: set it only when `in` is the output of the previous call on
+    // the same Exec, made with chain_out.  Both flags are ignored otherwise.
+    void forward(const Exec& e, const Act& in, const Act& out, const QuantSel* q = nullptr, bool chain_in = false,
+                 bool chain_out = false) const;
 };
 
 // A chain of DepthConvBlocks: block 0 reads `in`, intermediate results live in

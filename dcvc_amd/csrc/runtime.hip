@@ -218,7 +218,8 @@ void DCB::load(DevMem& mem, const HostDict& d, const std::string& p, bool shortc
         throw std::invalid_argument(p + ": DepthConvBlock layer shapes do not chain");
 }
 
-void DCB::forward(const Exec& e, const Act& in, const Act& out, const QuantSel* q) const {
+void DCB::forward(const Exec& e, const Act& in, const Act& out, const QuantSel* q, bool chain_in,
+                  bool chain_out) const {
     if (in.C != cin || out.C != cout) throw std::invalid_argument("DCB: channel mismatch");
     Scratch& sc = *e.sc;
     const size_t need_mid = (size_t)in.pixels() * cmid;
@@ -238,10 +239,20 @@ void DCB::forward(const Exec& e, const Act& in, const Act& out, const QuantSel* 
         // output also as an e4m3 copy (the quad's A; ffn.2 adds the fp16 one).
         // t2's buffer holds the depthwise bytes, then dc.3's copy behind them
         // (M * (cmid + cout) bytes fit in its M * max(cmid, cout) halves).
+        // A chained block input / output copy sits at the start of t2's
+        // buffer: read by dc.0 before the depthwise conv overwrites it, and
+        // written by ffn.2 after dc.3 and the quad have read their operands.
         uint8_t* t2_8 = reinterpret_cast<uint8_t*>(sc.t2);
         uint8_t* o8 = t2_8 + (size_t)in.pixels() * cmid;
         uint8_t* t1_8 = reinterpret_cast<uint8_t*>(sc.t1);
-        dc0.run(e, x, t1, EPI_WSILU);
+        if (chain_in && !has_adaptor) {
+            Fp8Io io;
+            io.a8 = t2_8;
+            io.lda8 = cin;
+            dc0.run(e, x, t1, EPI_WSILU, nullptr, nullptr, nullptr, &io);
+        } else {
+            dc0.run(e, x, t1, EPI_WSILU);
+        }
         dw.run(e, t1, t2, t2_8, cmid, false);
         Fp8Io io3;
         io3.a8 = t2_8;
@@ -259,6 +270,10 @@ void DCB::forward(const Exec& e, const Act& in, const Act& out, const QuantSel* 
         Fp8Io io2;
         io2.a8 = t1_8;
         io2.lda8 = cmid;
+        if (chain_out) {
+            io2.d8 = t2_8;
+            io2.ldd8 = cout;
+        }
         if (shortcut) {
             ffn2.run(e, t1, out, EPI_SHORTCUT2, &o, &x, nullptr, &io2);
         } else if (q) {
@@ -332,9 +347,9 @@ void DCBSeq::forward(const Exec& e, const Act& in, const Act& out, const Act& tm
         return;
     }
     Act t = channels(tmp, 0, blocks[0].cout);
-    blocks[0].forward(e, in, t);
-    for (int i = 1; i + 1 < n; ++i) blocks[i].forward(e, t, t);
-    blocks[n - 1].forward(e, t, out, q);
+    blocks[0].forward(e, in, t, nullptr, false, true);
+    for (int i = 1; i + 1 < n; ++i) blocks[i].forward(e, t, t, nullptr, true, true);
+    blocks[n - 1].forward(e, t, out, q, true, false);
 }
 
 // ---------------------------------------------------------------- ConvS2

@@ -149,9 +149,9 @@ int dcvc_gemm_lowp(const void* A, int lda, int M, int K, const void* Wq, int N, 
  *   a_fp8 = 1: A holds e4m3 bytes (row stride lda in bytes, lda % 16 == 0)
  *     as an earlier D8 / Y8 output stored them; requires prec 2, and gives
  *     the same bits as prec 2 on the fp16 activations they were made from;
- *   D8 (may be NULL): an e4m3 copy of the fp16 output [M x N] (row stride
- *     ldd8 bytes, % 8 == 0), saturated at +-448, round to nearest even at
- *     scale 1; D may then be NULL (e4m3 output only).
+ *   D8 (may be NULL; prec 2 only): an e4m3 copy of the fp16 output [M x N]
+ *     (row stride ldd8 bytes, % 8 == 0), saturated at +-448, round to nearest
+ *     even at scale 1; D may then be NULL (e4m3 output only).
  * dcvc_d3x3_fp8_out is dcvc_d3x3 with the same optional Y8 copy (Y may be
  * NULL when Y8 is given).  dcvc_set_fp8_activations(0) makes the codec
  * proxies in precision 2 convert fp16 activations inside every product

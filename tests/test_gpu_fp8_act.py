@@ -110,8 +110,8 @@ def test_fp8_activation_tile_configs_bit_identical(gpu, epi):
 def test_fp8_activation_arguments_rejected(gpu):
     from dcvc_amd import _lib
     lib = _lib.load()
-    a = torch.zeros((64, 64), dtype=torch.uint8, device=gpu)
-    w = torch.zeros((64, 64), dtype=torch.uint8, device=gpu)
+    a = torch.zeros((64, 128), dtype=torch.uint8, device=gpu)  # room for 64 x 64 fp16 activations too
+    w = torch.zeros((64, 128), dtype=torch.uint8, device=gpu)  # room for 64 x 64 fp16 weights too
     s = torch.ones(64, dtype=torch.float32, device=gpu)
     d = torch.zeros((64, 64), dtype=torch.float16, device=gpu)
     d8 = torch.zeros((64, 64), dtype=torch.uint8, device=gpu)
@@ -127,6 +127,8 @@ def test_fp8_activation_arguments_rejected(gpu):
     assert call(D=None, D8=None) == 1  # no output at all
     assert call(ldd8=60) == 1
     assert call(a_fp8=2) == 1
+    assert call(a_fp8=0, prec=0, lda=64) == 1  # e4m3 output copies come from fp8 products only
+    assert call(a_fp8=0, prec=0, lda=64, D8=None) == 0
 
 
 @pytest.mark.parametrize("model", ["hts", "htl", "ld"])
