@@ -272,7 +272,17 @@ def precision_forms(sdp, p_enc_ref, x_hat_i, chunks, H, W, pad_b, pad_r, skip, d
         out["fps_loop_y_symbols"][prec] = ns
         out["fps_loop_bytes_per_chunk"][prec] = nb
         rec_e, rec_d = layer_trace(e, d, chunks[0], 32, pad_b, pad_r, H, W)
-        out["traced_kernel_ms"][prec] = round(float(np.concatenate([rec_e, rec_d])[:, 7].sum()) * 1e-6, 3)
+        recs = np.concatenate([rec_e, rec_d])
+        out["traced_kernel_ms"][prec] = round(float(recs[:, 7].sum()) * 1e-6, 3)
+        # quad GEMMs (ffn.0): algorithmic FLOPs over event-timed duration,
+        # against the dense MFMA peak of the operand type
+        quad = (recs[:, 0] == 0) & (recs[:, 1] == 6)
+        if quad.any():
+            tf = float(recs[quad, 5].sum()) / float(recs[quad, 7].sum()) * 1e-3  # FLOP/ns -> TFLOP/s
+            peak = {"fp16": 2500.0, "bf16": 2500.0, "fp8": 5000.0}[prec]
+            out.setdefault("quad_gemm", {})[prec] = {
+                "launches": int(quad.sum()), "achieved_tflops": round(tf, 1), "peak_tflops": peak,
+                "frac": round(tf / peak, 4), "operand_prec_codes": sorted({int(v) for v in recs[quad, 10]})}
         del e, d
         # a continuing stream from the anchor: per-chunk coded y symbols
         e, d = make(prec)
@@ -300,7 +310,7 @@ def precision_forms(sdp, p_enc_ref, x_hat_i, chunks, H, W, pad_b, pad_r, skip, d
         ref_syms = None
         for prec in ("fp16", "bf16", "fp8"):
             e, d = make(prec, ref_l)
-            nbytes, psnrs, syms = 0, [], []
+            nbytes, psnrs, syms, first = 0, [], [], None
             for c in range(n_chunks):
                 bits, ec = e.proxy.compress(chunks[c], qp, False, pad_b, pad_r)
                 syms.append(e.proxy.debug_copy(5, P16 * 256 * 2, np.int16))
@@ -312,15 +322,25 @@ def precision_forms(sdp, p_enc_ref, x_hat_i, chunks, H, W, pad_b, pad_r, skip, d
                     x = rec[f].float()[:, :, :H, :W]
                     mse = float(((x - src[:, 3 * f:3 * f + 3]) ** 2).mean())
                     psnrs.append(10 * np.log10(1.0 / max(mse, 1e-12)))
+                if c == 0:
+                    first = (len(bits), float(np.mean(psnrs)))
             if prec == "fp16":
                 ref_syms = syms
             match = float(np.mean([np.mean(a == b) for a, b in zip(syms, ref_syms)]))
             res[prec] = {"bpp": round(8.0 * nbytes / (4 * n_chunks * H * W), 5),
-                         "psnr_db": round(float(np.mean(psnrs)), 4), "symbol_match_vs_fp16": round(match, 5)}
+                         "psnr_db": round(float(np.mean(psnrs)), 4), "symbol_match_vs_fp16": round(match, 5),
+                         "first_chunk_bpp": round(8.0 * first[0] / (4 * H * W), 5),
+                         "first_chunk_psnr_db": round(first[1], 4)}
             del e, d
         for prec in ("bf16", "fp8"):
             res[prec]["psnr_delta_db"] = round(res[prec]["psnr_db"] - res["fp16"]["psnr_db"], 4)
             res[prec]["bpp_delta_pct"] = round(100 * (res[prec]["bpp"] / res["fp16"]["bpp"] - 1), 3)
+            # one chunk from the common anchor: the products' own error, before
+            # the stream's temporal state carries it into the next chunks
+            res[prec]["first_chunk_psnr_delta_db"] = round(res[prec]["first_chunk_psnr_db"]
+                                                           - res["fp16"]["first_chunk_psnr_db"], 4)
+            res[prec]["first_chunk_bpp_delta_pct"] = round(100 * (res[prec]["first_chunk_bpp"]
+                                                                  / res["fp16"]["first_chunk_bpp"] - 1), 3)
         out["per_qp"][str(qp)] = res
     out["qp_weights"] = "per-QP ladders (weights.qp_ladder: encoder side 0.5 -> 2 over qp, decoder side inverse)"
     # first chunk of the stream at qp 32 against the FP32 oracle (skip 0.15)
@@ -495,7 +515,7 @@ def batching_form(sdi, sdp, rounds=2):
     """16 concurrent streams of uneven resolution (4x 854x480, 6x 1280x720,
     4x 1920x1080, 2x 3840x2160), one 4-frame chunk each per round at a random
     qp.  The plain path (FP16, one stream at a time) against all streams at
-    once in FP16 and in BF16 (the batching form's precision).  Also the
+    once in FP16, in BF16 (the batching form's precision) and in FP8.  Also the
     bound batching could reach: every stream at the per-pixel rate of the
     best-utilised resolution in the plain path."""
     import torch
@@ -527,6 +547,15 @@ def batching_form(sdi, sdp, rounds=2):
         batching.code_concurrently(setup2.contexts, [c[1 + r] for c in setup2.chunks], setup2.qps)
     t_con = time.perf_counter() - t0
     del setup2
+    # FP8 products on e4m3 activations (the fastest precision of this build)
+    setup3, _ = batching.build_streams(sdi, sdp, batching.MIX_16, n_chunks=rounds + 1, seed=0, precision="fp8")
+    batching.code_concurrently(setup3.contexts, [c[0] for c in setup3.chunks], setup3.qps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r in range(rounds):
+        batching.code_concurrently(setup3.contexts, [c[1 + r] for c in setup3.chunks], setup3.qps)
+    t_con8 = time.perf_counter() - t0
+    del setup3
     # per-pixel cost of each resolution alone (plain path), and the bound
     px = {k: ((k[0] + 15) // 16 * 16) * ((k[1] + 15) // 16 * 16) for k in per_res}
     us_per_mpx = {f"{k[1]}x{k[0]}": round(1e6 * float(np.mean(v)) / (px[k] / 1e6), 1) for k, v in per_res.items()}
@@ -535,7 +564,9 @@ def batching_form(sdi, sdp, rounds=2):
     out.update({"plain_fp16_sequential_frames_per_s": round(frames / t_seq, 2),
                 "concurrent_fp16_frames_per_s": round(frames / t_con16, 2),
                 "concurrent_bf16_frames_per_s": round(frames / t_con, 2),
+                "concurrent_fp8_frames_per_s": round(frames / t_con8, 2),
                 "speedup_bf16_vs_plain": round(t_seq / t_con, 3), "speedup_fp16_vs_plain": round(t_seq / t_con16, 3),
+                "speedup_fp8_vs_plain": round(t_seq / t_con8, 3),
                 "target_speedup": 3.0,
                 "plain_us_per_megapixel_step": us_per_mpx,
                 "bound_frames_per_s_at_best_per_pixel_rate": round(frames / t_bound, 2),
