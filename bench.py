@@ -713,6 +713,9 @@ def main():
     ap.add_argument("--no-htl", action="store_true", help="skip the HT-L chunk-model section")
     ap.add_argument("--no-ld", action="store_true", help="skip the low-delay model section")
     ap.add_argument("--no-skip0", action="store_true", help="skip the HT-S skip_thres 0 section")
+    ap.add_argument("--precision", default="fp16", choices=("fp16", "bf16", "fp8"),
+                    help="product precision of the headline stream: fp16 is the measured workload, the others "
+                         "are for profiling the precision form (--profile-step)")
     ap.add_argument("--serving-seconds", type=float, default=60.0,
                     help="length of the Poisson serving trace (0 skips it)")
     args = ap.parse_args()
@@ -734,6 +737,9 @@ def main():
     i_dec = codec.IntraCodec(sdi, args.skip_thres)
     p_enc = codec.VideoCodecHTS(sdp, args.skip_thres)
     p_dec = codec.VideoCodecHTS(share_from=p_enc)
+    if args.precision != "fp16":
+        p_enc.proxy.set_precision(args.precision)
+        p_dec.proxy.set_precision(args.precision)
     log("synthesising frames")
     n_chunks = 8
     frames = synthetic.motion_sequence_yuv420(H, W, 1 + 4 * n_chunks, seed=0)
@@ -794,7 +800,8 @@ def main():
     out = {
         "metric": METRIC, "value": round(fps, 3), "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "fp16", "data": "synthetic",
+        "vs_baseline": None, "dtype": "fp16" if args.precision == "fp16" else f"{args.precision} products, fp16 storage",
+        "data": "synthetic",
         "config": {"workload": "DCVC-UF HT-S chunk encode+decode, one stream, 4 frames 1920x1080 YUV420 "
                                "padded to the native 8-frame chunk, qp 32", "height": H, "width": W,
                    "chunks_in_stream": n_chunks,
