@@ -556,6 +556,20 @@ def batching_form(sdi, sdp, rounds=2):
         batching.code_concurrently(setup3.contexts, [c[1 + r] for c in setup3.chunks], setup3.qps)
     t_con8 = time.perf_counter() - t0
     del setup3
+
+    # one batch proxy pair per resolution (one launch per layer over the
+    # group's streams), the four groups on concurrent host threads
+    def time_groups(precision):
+        gs, _ = batching.build_groups(sdi, sdp, batching.MIX_16, n_chunks=rounds + 1, seed=0, precision=precision)
+        batching.code_groups_concurrently(gs.groups, [ch[0] for ch in gs.chunks], gs.qps)  # graph capture
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for r in range(rounds):
+            batching.code_groups_concurrently(gs.groups, [ch[1 + r] for ch in gs.chunks], gs.qps)
+        dt = time.perf_counter() - t0
+        del gs
+        return dt
+    t_grp = {p: time_groups(p) for p in ("fp16", "bf16", "fp8")}
     # per-pixel cost of each resolution alone (plain path), and the bound
     px = {k: ((k[0] + 15) // 16 * 16) * ((k[1] + 15) // 16 * 16) for k in per_res}
     us_per_mpx = {f"{k[1]}x{k[0]}": round(1e6 * float(np.mean(v)) / (px[k] / 1e6), 1) for k, v in per_res.items()}
@@ -567,6 +581,10 @@ def batching_form(sdi, sdp, rounds=2):
                 "concurrent_fp8_frames_per_s": round(frames / t_con8, 2),
                 "speedup_bf16_vs_plain": round(t_seq / t_con, 3), "speedup_fp16_vs_plain": round(t_seq / t_con16, 3),
                 "speedup_fp8_vs_plain": round(t_seq / t_con8, 3),
+                "grouped_frames_per_s": {p: round(frames / t, 2) for p, t in t_grp.items()},
+                "grouped_speedup_vs_plain": {p: round(t_seq / t, 3) for p, t in t_grp.items()},
+                "groups": "one batch proxy pair per resolution: 4 x 854x480, 6 x 1280x720, 4 x 1920x1080, "
+                          "2 x 3840x2160; every layer one launch over the group",
                 "target_speedup": 3.0,
                 "plain_us_per_megapixel_step": us_per_mpx,
                 "bound_frames_per_s_at_best_per_pixel_rate": round(frames / t_bound, 2),

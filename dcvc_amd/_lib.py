@@ -110,6 +110,10 @@ SIGNATURES = {
     "dcvc_hts_device_bytes": [P],
     "dcvc_hts_debug_copy": [P, I, P, L],
     "dcvc_hts_last_symbol_count": [P],
+    "dcvc_hts_prepare_batch": [P, I, I, I, P, I],
+    "dcvc_hts_compress_batch": [P, P, I, I, I, P, I, I, I, P, P, P, P],
+    "dcvc_hts_decompress_batch": [P, P, P, P, I, I, P, I, P],
+    "dcvc_hts_batch_symbol_count": [P, I],
     "dcvc_htl_create": [],
     "dcvc_htl_destroy": [P],
     "dcvc_htl_set_param": [P, I, P, P, P, P, F],
@@ -143,7 +147,7 @@ _RESTYPE = {"dcvc_last_error": C.c_char_p, "dcvc_rans_encoder_create": C.c_void_
             "dcvc_rans_decoder_create": C.c_void_p, "dcvc_rans_encoder_destroy": None,
             "dcvc_rans_decoder_destroy": None, "dcvc_dmci_create": C.c_void_p, "dcvc_hts_create": C.c_void_p,
             "dcvc_dmci_destroy": None, "dcvc_hts_destroy": None, "dcvc_hts_last_symbol_count": C.c_long,
-            "dcvc_hts_device_bytes": C.c_long,
+            "dcvc_hts_device_bytes": C.c_long, "dcvc_hts_batch_symbol_count": C.c_long,
             "dcvc_dmci_step_symbol_count": C.c_long, "dcvc_htl_create": C.c_void_p, "dcvc_htl_destroy": None,
             "dcvc_htl_device_bytes": C.c_long, "dcvc_htl_step_symbol_count": C.c_long,
             "dcvc_ld_create": C.c_void_p, "dcvc_ld_destroy": None, "dcvc_ld_device_bytes": C.c_long,

@@ -9,6 +9,12 @@ own HIP streams, so the kernels of different streams overlap on the GPU and
 the CPU entropy coding of one stream overlaps the others' GPU work.
 Results are bit-identical to coding each stream alone: every kernel is
 batch- and tile-invariant and a stream's state is its own.
+
+``BatchGroup`` goes further for streams of one resolution: one encoder and
+one decoder proxy code the whole group as a batch (``DMCHTSBatchProxy``), so
+every layer is one launch over the group's pictures, each stream keeping its
+own QP, temporal context and bitstream.  ``code_groups_concurrently`` runs
+the groups of a mix side by side, one worker thread per group.
 """
 from __future__ import annotations
 
@@ -21,6 +27,7 @@ import numpy as np
 import torch
 
 from . import codec, synthetic
+from .inference_extensions_cuda import DMCHTSBatchProxy
 
 
 @dataclass
@@ -136,3 +143,107 @@ def build_streams(sdi, sdp, resolutions=MIX_16, n_chunks=2, seed=0, precision="f
                        for c in range(n_chunks)])
         qps.append(int(rng.integers(0, 64)))
     return BatchingSetup(contexts, chunks, qps, ib), shared
+
+
+class BatchGroup:
+    """Streams of one resolution coded as one batch: an encoder and a decoder
+    DMCHTSBatchProxy sharing `shared`'s weights.  The streams advance together,
+    one chunk each per call."""
+
+    def __init__(self, sids, height: int, width: int, shared: codec.VideoCodecHTS, precision="fp16"):
+        self.sids = list(sids)
+        self.H, self.W = height, width
+        n = len(self.sids)
+        self.enc = DMCHTSBatchProxy(n)
+        self.dec = DMCHTSBatchProxy(n)
+        self.enc.share_weights_from(shared.proxy)
+        self.dec.share_weights_from(shared.proxy)
+        self.enc.set_precision(precision)
+        self.dec.set_precision(precision)
+        self.pad_r, self.pad_b = synthetic.get_padding_size(height, width, 16)
+        self.lock = threading.Lock()
+
+    def start(self, intra_enc: codec.IntraCodec, intra_dec: codec.IntraCodec, frames, qp=32):
+        """Code each stream's I frame (one intra proxy pair) and seed the
+        group's temporal contexts; `frames` holds one [1, 3, H, W] input per
+        stream.  Returns the I-frame sizes."""
+        enc_hat, dec_hat, sizes = [], [], []
+        for f in frames:
+            e = intra_enc.compress(f, qp, self.pad_b, self.pad_r)
+            d = intra_dec.decompress(e["bit_stream"], {"height": self.H, "width": self.W}, qp, e["ec_parallel"])
+            enc_hat.append(e["x_hat"].clone())
+            dec_hat.append(d["x_hat"].clone())
+            sizes.append(len(e["bit_stream"]))
+        self.enc.add_ref_feature_from_frame(_stack(enc_hat))
+        self.dec.add_ref_feature_from_frame(_stack(dec_hat), False)
+        torch.cuda.synchronize()
+        return sizes
+
+    def code_chunk(self, xs, qps):
+        """Encode then decode one chunk of every stream (`xs`: one [1, 24, H, W]
+        input per stream); returns one ChunkResult per stream, with the
+        group's encode / decode time."""
+        with self.lock:
+            x = _stack(xs)
+            t0 = time.perf_counter()
+            bits, ecs = self.enc.compress_batch(x, qps, False, self.pad_b, self.pad_r)
+            t1 = time.perf_counter()
+            outs = self.dec.decompress_batch(bits, qps, self.H, self.W, ecs, False)
+            frames = [[o[i:i + 1].clone() for o in outs] for i in range(len(self.sids))]
+            torch.cuda.current_stream().synchronize()
+            t2 = time.perf_counter()
+            return [ChunkResult(sid, int(q), b.tobytes(), f, t1 - t0, t2 - t1)
+                    for sid, q, b, f in zip(self.sids, qps, bits, frames)]
+
+
+def _stack(ts):
+    return torch.cat(list(ts), 0).contiguous(memory_format=torch.channels_last)
+
+
+def code_group(group: BatchGroup, xs, qps):
+    with torch.cuda.stream(_worker_stream()):
+        return group.code_chunk(xs, qps)
+
+
+def code_groups_concurrently(groups, inputs, qps):
+    """Every group at once, one worker thread per group; `inputs` / `qps` hold
+    one list per group.  Returns the ChunkResults of all streams, by group."""
+    with ThreadPoolExecutor(max_workers=len(groups)) as pool:
+        futs = [pool.submit(code_group, g, x, q) for g, x, q in zip(groups, inputs, qps)]
+        return [r for f in futs for r in f.result()]
+
+
+@dataclass
+class GroupSetup:
+    groups: list
+    chunks: list               # per group: per chunk index, one input per stream
+    qps: list                  # per group: one qp per stream
+    intra_bytes: list = field(default_factory=list)
+
+
+def build_groups(sdi, sdp, resolutions=MIX_16, n_chunks=2, seed=0, precision="fp16", skip_thres=0.15,
+                 device="cuda"):
+    """The streams of build_streams (same content and qps, stream for
+    stream), grouped by resolution into BatchGroups."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    qps = [int(rng.integers(0, 64)) for _ in resolutions]
+    intra_e = codec.IntraCodec(sdi, skip_thres)
+    intra_d = codec.IntraCodec(sdi, skip_thres)
+    shared = codec.VideoCodecHTS(sdp, skip_thres)
+    by_res = {}
+    for sid, hw in enumerate(resolutions):
+        by_res.setdefault(hw, []).append(sid)
+    groups, chunks, gqps, ib = [], [], [], []
+    for (H, W), sids in by_res.items():
+        g = BatchGroup(sids, H, W, shared, precision)
+        firsts, per_chunk = [], [[] for _ in range(n_chunks)]
+        for sid in sids:
+            frames = synthetic.motion_sequence_yuv420(H, W, 1 + 4 * n_chunks, seed=seed + 17 * sid)
+            firsts.append(codec.to_input(synthetic.frame_to_input(frames[0]), device))
+            for c in range(n_chunks):
+                per_chunk[c].append(codec.to_input(synthetic.chunk_to_input(frames[1 + 4 * c:5 + 4 * c], 8), device))
+        ib.extend(g.start(intra_e, intra_d, firsts))
+        groups.append(g)
+        chunks.append(per_chunk)
+        gqps.append([qps[sid] for sid in sids])
+    return GroupSetup(groups, chunks, gqps, ib), shared

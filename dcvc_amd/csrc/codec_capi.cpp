@@ -117,6 +117,43 @@ int dcvc_hts_prepare(void* h, int Hp, int Wp, void* const* frames, int ld_out) {
     });
 }
 
+int dcvc_hts_prepare_batch(void* h, int n_img, int Hp, int Wp, void* const* frames, int ld_out) {
+    return guarded([&] {
+        if (!frames || ld_out < 3) throw std::invalid_argument("bad output buffers");
+        f16* f[HTSProxy::kFrames];
+        for (int i = 0; i < HTSProxy::kFrames; ++i) {
+            if (!frames[i]) throw std::invalid_argument("null output frame");
+            f[i] = static_cast<f16*>(frames[i]);
+        }
+        checked<HTSProxy>(h)->prepare(Hp, Wp, f, ld_out, n_img);
+    });
+}
+
+int dcvc_hts_compress_batch(void* h, const void* x, int ldx, int H, int W, const int* qps, int reset_feature_memory,
+                            int pad_b, int pad_r, void* stream, const uint8_t** bits, long* nbytes,
+                            int* ec_parallel) {
+    return guarded([&] {
+        if (!x || !qps || !bits || !nbytes || !ec_parallel) throw std::invalid_argument("null argument");
+        checked<HTSProxy>(h)->compress_batch(static_cast<const f16*>(x), ldx, H, W, qps, reset_feature_memory != 0,
+                                             pad_b, pad_r, S(stream), bits, nbytes, ec_parallel);
+    });
+}
+
+int dcvc_hts_decompress_batch(void* h, const uint8_t* const* bits, const long* nbytes, const int* qps, int height,
+                              int width, const int* ec_parallel, int reset_feature_memory, void* stream) {
+    return guarded([&] {
+        if (!bits || !nbytes || !qps || !ec_parallel) throw std::invalid_argument("null argument");
+        checked<HTSProxy>(h)->decompress_batch(bits, nbytes, qps, height, width, ec_parallel,
+                                               reset_feature_memory != 0, S(stream));
+    });
+}
+
+long dcvc_hts_batch_symbol_count(void* h, int img) {
+    if (!h) return -1;
+    const HTSProxy* p = static_cast<HTSProxy*>(h);
+    return (img >= 0 && img < p->images()) ? p->last_symbol_count(img) : -1;
+}
+
 int dcvc_hts_add_ref_feature_from_frame(void* h, const void* frame, int ld, int Hp, int Wp, int apply_adaptor,
                                         void* stream) {
     return guarded([&] {

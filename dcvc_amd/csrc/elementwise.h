@@ -23,8 +23,10 @@ hipError_t pad_unshuffle(const f16* x, int ldx, int n_img, int H, int W, int Cin
                          int padR, f16* y, int ldy, hipStream_t s);
 hipError_t pixel_shuffle8_clamp(const f16* x, int ldx, int n_img, int H8, int W8, int C, f16* y,
                                 int ldy, hipStream_t s);
+// y = x * q[qp[m / hw]][c]: hw pixels per image, one device QP index per
+// image (hw <= 0: one image of M pixels)
 hipError_t mul_channel(const f16* x, int ldx, long M, int C, const f16* q, int qstride,
-                       const int* qp, f16* y, int ldy, hipStream_t s);
+                       const int* qp, f16* y, int ldy, hipStream_t s, long hw = 0);
 hipError_t round_z(const f16* z, int ldz, long M, int C, f16* zhat, int ldh, int8_t* zi8,
                    hipStream_t s);
 hipError_t i8_to_f16(const int8_t* zi8, long M, int C, f16* zhat, int ldh, hipStream_t s);

@@ -230,13 +230,13 @@ hipError_t pixel_shuffle8_clamp(const f16* x, int ldx, int n_img, int H8, int W8
 // multiply_with_broadcast_cuda (stream.cu:484-546): y = x * q[qp][c]
 // x and y may be the same buffer (in place): each lane reads its 8 elements before writing them
 __global__ void mul_channel_kernel(const f16* x, int ldx, long M, int C, const f16* __restrict__ q, int qstride,
-                                   const int* __restrict__ qp, f16* y, int ldy) {
+                                   const int* __restrict__ qp, long hw, f16* y, int ldy) {
     const int CG = C >> 3;
     const long item = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (item >= M * CG) return;
     const long m = item / CG;
     const int c0 = (int)(item % CG) * 8;
-    const f16* qr = q + (size_t)(qp ? *qp : 0) * qstride + c0;
+    const f16* qr = q + (size_t)(qp ? qp[hw < M ? (int)(m / hw) : 0] : 0) * qstride + c0;
     f16x8 a = *reinterpret_cast<const f16x8*>(x + m * ldx + c0);
     f16x8 o;
 #pragma unroll
@@ -245,8 +245,9 @@ __global__ void mul_channel_kernel(const f16* x, int ldx, long M, int C, const f
 }
 
 hipError_t mul_channel(const f16* x, int ldx, long M, int C, const f16* q, int qstride, const int* qp, f16* y,
-                       int ldy, hipStream_t s) {
-    hipLaunchKernelGGL(mul_channel_kernel, grid1d(M * (C / 8)), dim3(256), 0, s, x, ldx, M, C, q, qstride, qp, y,
+                       int ldy, hipStream_t s, long hw) {
+    if (hw <= 0) hw = M;
+    hipLaunchKernelGGL(mul_channel_kernel, grid1d(M * (C / 8)), dim3(256), 0, s, x, ldx, M, C, q, qstride, qp, hw, y,
                        ldy);
     return hipGetLastError();
 }

@@ -55,7 +55,7 @@ struct GemmArgs {
     int lds2 = 0;
     const f16* q = nullptr;     // per-QP table, row stride qstride
     int qstride = 0;
-    const int* qp = nullptr;    // device QP index (nullptr -> row 0)
+    const int* qp = nullptr;    // device QP index per image of outH x outW pixels (nullptr -> row 0)
     int q_round = 0;            // quant epilogues: round to fp16 before the q multiply,
                                 // as a separate fp16 multiply_with_broadcast would
     f16* D = nullptr;                // nullptr: no fp16 output (D8 only)

@@ -468,14 +468,14 @@ __global__ __launch_bounds__(T::kThreads, T::kWavesPerSimd) void gemm_kernel(Gem
     // ---- epilogue ----
     // bias (and activation) in registers; accumulator lane layout of a 32x32
     // tile: column lane & 31, row (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3).
-    const f16* q = nullptr;
-    if constexpr (EPI == EPI_SHORTCUT_QUANT || EPI == EPI_QUANT) {
-        const int qp = g.qp ? *g.qp : 0;
-        q = g.q + (size_t)qp * g.qstride;
-    }
     float* E = reinterpret_cast<float*>(smem);
     constexpr int EP = T::kEPitch;
     const int hwo = g.outH * g.outW;
+    // per-QP row of pixel m: one QP index per image of outH x outW pixels
+    auto qrow = [&](int m) -> const f16* {
+        const int img = (hwo < g.M) ? m / hwo : 0;
+        return g.q + (size_t)(g.qp ? g.qp[img] : 0) * g.qstride;
+    };
 #pragma unroll
     for (int i = 0; i < SM; ++i) {
         if (i > 0) __syncthreads();  // previous pass fully read back
@@ -530,6 +530,7 @@ __global__ __launch_bounds__(T::kThreads, T::kWavesPerSimd) void gemm_kernel(Gem
 #pragma unroll
                     for (int e = 0; e < 8; ++e) x[e] = (float)(f16)x[e];
                 }
+                const f16* q = qrow(m);
 #pragma unroll
                 for (int e = 0; e < 8; ++e) x[e] *= (float)q[n + e];
             }

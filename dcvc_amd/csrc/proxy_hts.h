@@ -32,22 +32,38 @@ struct HTSWeights {
     void load(const HostDict& d, float skip);
 };
 
+// One proxy codes n_img streams of one resolution as a batch (n_img = 1 is
+// the reference's proxy): every layer is one launch over the n_img images,
+// each image keeps its own QP, temporal state and bitstream, and its bits
+// equal those of a proxy of its own.
 class HTSProxy {
 public:
     static constexpr int kFrames = 8;
+    static constexpr int kMaxImages = DeviceInts::kMax;
     HTSProxy();
     ~HTSProxy();
     void set_param(const HostDict& d, float skip_thres);
     void share_weights(const std::shared_ptr<HTSWeights>& w);
     std::shared_ptr<HTSWeights> weights() const { return m_w; }
-    // Allocate buffers for a padded Hp x Wp picture; `frames` are 8 caller-owned
-    // NHWC fp16 [Hp, Wp, 3] reconstruction outputs (pixel stride ld_out).
-    void prepare(int Hp, int Wp, f16* const frames[kFrames], int ld_out);
+    // Allocate buffers for n_img pictures of padded size Hp x Wp; `frames`
+    // are 8 caller-owned NHWC fp16 [n_img, Hp, Wp, 3] reconstruction outputs
+    // (pixel stride ld_out).
+    void prepare(int Hp, int Wp, f16* const frames[kFrames], int ld_out, int n_img = 1);
+    int images() const { return m_nimg; }
+    // frame: n_img NHWC pictures [n_img, Hp, Wp] (pixel stride ld)
     void add_ref_feature_from_frame(const f16* frame, int ld, int Hp, int Wp, bool apply_adaptor, hipStream_t s);
+    // one image (n_img == 1)
     int compress(const f16* x, int ldx, int H, int W, int qp, bool reset, int pad_b, int pad_r, hipStream_t s,
                  const uint8_t** bits, long* nbytes);
     void decompress(const uint8_t* bits, long nbytes, int qp, int height, int width, int ec_parallel, bool reset,
                     hipStream_t s);
+    // all n_img images: x is [n_img, H, W] pixels of 24 channels; qps, bits,
+    // nbytes, ec_parallel have n_img entries (bits stay valid until the next
+    // compress)
+    void compress_batch(const f16* x, int ldx, int H, int W, const int* qps, bool reset, int pad_b, int pad_r,
+                        hipStream_t s, const uint8_t** bits, long* nbytes, int* ec_parallel);
+    void decompress_batch(const uint8_t* const* bits, const long* nbytes, const int* qps, int height, int width,
+                          const int* ec_parallel, bool reset, hipStream_t s);
     void set_use_graphs(bool on) { m_use_graphs = on; reset_graphs(); }
     // GemmPrec of every 1x1-conv product (precision form); recaptures the graphs
     void set_precision(int prec) {
@@ -58,7 +74,9 @@ public:
     }
     size_t device_bytes() const { return m_buf.bytes(); }
     // diagnostics for tests: last compacted symbol count and y / symbols on host
-    long last_symbol_count() const { return m_last_count; }
+    long last_symbol_count(int img = 0) const {
+        return (img >= 0 && img < (int)m_last_counts.size()) ? m_last_counts[img] : 0;
+    }
     void copy_debug(int what, void* host, long bytes);
 
 private:
@@ -87,10 +105,15 @@ private:
     Exec exec_side() { return Exec{m_side, m_d_qp, &m_scratch2, m_prec}; }
     void fork_side();
     void join_side();
+    void set_coder_cdfs();
+    void set_qps(const int* qps);
+    void check_qps(const int* qps) const;
 
     std::shared_ptr<HTSWeights> m_w;
-    RansEncoder m_enc;
-    RansDecoder m_dec;
+    int m_nimg = 1;
+    RansEncoder m_enc;                                  // images coded one after another
+    std::vector<std::vector<uint8_t>> m_bits;           // per image: the last compress's bitstream
+    std::vector<std::unique_ptr<RansDecoder>> m_decs;  // per image: z and y decode of one call
     hipStream_t m_stream = nullptr, m_copy = nullptr, m_side = nullptr;
     hipEvent_t m_ev_in = nullptr, m_ev_out = nullptr, m_ev_sym = nullptr, m_ev_fork = nullptr, m_ev_join = nullptr;
     bool m_use_graphs = true;
@@ -101,7 +124,8 @@ private:
     int m_Hp = 0, m_Wp = 0, m_H8 = 0, m_W8 = 0, m_H16 = 0, m_W16 = 0, m_H16p = 0, m_W16p = 0;
     int m_H64 = 0, m_W64 = 0;
     bool m_pad4 = false;
-    long m_ny = 0, m_nz = 0;
+    long m_ny = 0, m_nz = 0;  // per image
+    int m_offs_stride = 0;    // compaction block-offset slots per image
 
     // device buffers
     DevMem m_buf;
@@ -131,8 +155,7 @@ private:
     bool m_memory_ready = false;     // encoder side: memory/ctx hold the adaptor output
     bool m_memory_has_value = false; // decoder side (dmc_hts_proxy.cpp:617, 707)
     bool m_has_ref = false;  // add_ref_feature_from_frame ran since prepare(): m_feature_i holds a frame
-    long m_last_count = 0;
-    int m_qp = 0;
+    std::vector<long> m_last_counts;
 };
 
 }  // namespace dcvc

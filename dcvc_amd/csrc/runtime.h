@@ -253,6 +253,13 @@ private:
 };
 
 void set_device_int(int* dst, int v, hipStream_t s);
+// dst[0..n) = v[0..n), n <= 16, ordered on stream s (a kernel, so graph-safe)
+struct DeviceInts {
+    static constexpr int kMax = 16;
+    int n = 0;
+    int v[kMax] = {};
+};
+void set_device_ints(int* dst, const int* v, int n, hipStream_t s);
 
 // Launch trace: one record per GEMM / conv / depthwise launch issued by the
 // layer classes while tracing is on (used to build the per-chunk layer table).

@@ -286,7 +286,8 @@ void DCB::forward(const Exec& e, const Act& in, const Act& out, const QuantSel* 
     dc0.run(e, x, t1, EPI_WSILU);
     dw.run(e, t1, t2);
     dc3.run(e, t2, o, EPI_SHORTCUT, &x);
-    if (ffn_use((int)in.pixels(), cout, cmid, e.prec)) {
+    // (the fused FFN reads one QP per launch: batches of images take the two GEMMs)
+    if (ffn_use((int)in.pixels(), cout, cmid, e.prec) && (!q || in.n_img == 1)) {
         // ffn.0 and ffn.2 in one launch, the hidden activation kept on chip
         FfnArgs f;
         f.A = o.p;
@@ -588,6 +589,20 @@ __global__ void set_int_kernel(int* dst, int v) { *dst = v; }
 void set_device_int(int* dst, int v, hipStream_t s) {
     hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, s, dst, v);
     hip_check(hipGetLastError(), "set_device_int");
+}
+
+// the values travel as a kernel argument, so no host buffer must outlive the call
+__global__ void set_ints_kernel(int* dst, DeviceInts v) {
+    if ((int)threadIdx.x < v.n) dst[threadIdx.x] = v.v[threadIdx.x];
+}
+
+void set_device_ints(int* dst, const int* v, int n, hipStream_t s) {
+    if (n < 1 || n > DeviceInts::kMax) throw std::invalid_argument("set_device_ints: 1..16 values");
+    DeviceInts a;
+    a.n = n;
+    for (int i = 0; i < n; ++i) a.v[i] = v[i];
+    hipLaunchKernelGGL(set_ints_kernel, dim3(1), dim3(64), 0, s, dst, a);
+    hip_check(hipGetLastError(), "set_device_ints");
 }
 
 

@@ -181,7 +181,7 @@ class _ChunkProxy:
 
     def share_weights_from(self, other):
         """Use `other`'s weights (one HBM copy for many stream contexts)."""
-        if type(other) is not type(self):
+        if not isinstance(other, _ChunkProxy) or other._PREFIX != self._PREFIX:
             raise TypeError("weights can only be shared between proxies of the same model structure")
         self._device = other._device
         self._call("share_weights", other._h)
@@ -249,6 +249,80 @@ class DMCHTSProxy(_ChunkProxy):
 
     def last_symbol_count(self):
         return int(self._lib.dcvc_hts_last_symbol_count(self._h))
+
+
+class DMCHTSBatchProxy(DMCHTSProxy):
+    """n HT-S streams of one resolution coded as one batch (batching form; no
+    reference counterpart: the reference holds one DMCHTSProxy per stream,
+    dmc_hts_proxy.cpp:892-1000).  Every layer is one launch over the n
+    pictures; each stream keeps its own QP, temporal context and bitstream,
+    and gets the same bits as from a DMCHTSProxy of its own.  The streams move
+    in lockstep: one chunk of every stream per compress_batch /
+    decompress_batch call."""
+    MAX_STREAMS = 16
+
+    def __init__(self, n):
+        if not 1 <= int(n) <= self.MAX_STREAMS:
+            raise ValueError(f"1 to {self.MAX_STREAMS} streams per batch")
+        super().__init__()
+        self.n = int(n)
+
+    def _prepare(self, Hp, Wp):
+        if self._size != (Hp, Wp):
+            self._frames = [torch.empty((self.n, 3, Hp, Wp), dtype=torch.float16, device=self._device,
+                                        memory_format=torch.channels_last) for _ in range(self.FRAMES)]
+            ptrs = (C.c_void_p * self.FRAMES)(*[f.data_ptr() for f in self._frames])
+            _lib.call("dcvc_hts_prepare_batch", self._h, self.n, Hp, Wp, C.cast(ptrs, C.c_void_p), 3)
+            self._size = (Hp, Wp)
+
+    def add_ref_feature_from_frame(self, frames, apply_adaptor=True):
+        """frames: [n,3,Hp,Wp], one decoded I frame per stream."""
+        frames = _nhwc(frames)
+        n, c, Hp, Wp = frames.shape
+        if n != self.n or c != 3 or Hp % 16 or Wp % 16:
+            raise ValueError(f"reference frames must be [{self.n},3,H,W] with H, W multiples of 16")
+        self._prepare(Hp, Wp)
+        self._call("add_ref_feature_from_frame", frames.data_ptr(), 3, Hp, Wp, int(bool(apply_adaptor)), _stream())
+
+    def compress(self, *args, **kwargs):
+        raise TypeError("a batch proxy codes all its streams at once: use compress_batch")
+
+    def decompress(self, *args, **kwargs):
+        raise TypeError("a batch proxy codes all its streams at once: use decompress_batch")
+
+    def compress_batch(self, x, qps, reset_feature_memory, padding_b, padding_r):
+        """x: [n,24,H,W], one chunk per stream; qps: n QPs.
+        -> (n np.uint8 bitstreams, n ec_parallel values)"""
+        x = _nhwc(x)
+        n, c, H, W = x.shape
+        if n != self.n or c != 24:
+            raise ValueError(f"chunk input must be [{self.n},24,H,W]")
+        if len(qps) != n:
+            raise ValueError("one qp per stream")
+        q = (C.c_int * n)(*[int(v) for v in qps])
+        bits = (C.c_void_p * n)()
+        nb = (C.c_long * n)()
+        ec = (C.c_int * n)()
+        _lib.call("dcvc_hts_compress_batch", self._h, x.data_ptr(), 24, H, W, q, int(bool(reset_feature_memory)),
+                  int(padding_b), int(padding_r), _stream(), bits, nb, ec)
+        return [_bits_to_numpy(C.c_void_p(bits[i]), C.c_long(nb[i])) for i in range(n)], list(ec)
+
+    def decompress_batch(self, bit_streams, qps, height, width, entropy_coder_parallel, reset_feature_memory):
+        """-> list of 8 tensors [n,3,Hp,Wp] (frame i of every stream)"""
+        n = self.n
+        if len(bit_streams) != n or len(qps) != n or len(entropy_coder_parallel) != n:
+            raise ValueError("one bitstream, qp and ec_parallel per stream")
+        arrs = [np.ascontiguousarray(np.asarray(b, dtype=np.uint8)) for b in bit_streams]
+        ptrs = (C.c_void_p * n)(*[a.ctypes.data if a.size else None for a in arrs])
+        sizes = (C.c_long * n)(*[a.size for a in arrs])
+        q = (C.c_int * n)(*[int(v) for v in qps])
+        ec = (C.c_int * n)(*[int(v) for v in entropy_coder_parallel])
+        _lib.call("dcvc_hts_decompress_batch", self._h, ptrs, sizes, q, int(height), int(width), ec,
+                  int(bool(reset_feature_memory)), _stream())
+        return list(self._frames)
+
+    def last_symbol_count(self, stream=0):
+        return int(self._lib.dcvc_hts_batch_symbol_count(self._h, int(stream)))
 
 
 class DMCHTLProxy(_ChunkProxy):

@@ -127,6 +127,26 @@ int dcvc_hts_debug_copy(void* h, int what, void* host, long bytes);
 int dcvc_dmci_debug_copy(void* h, int what, void* host, long bytes);
 /* Symbols handed to the entropy coder by the last call (per step for DMCI). */
 long dcvc_hts_last_symbol_count(void* h);
+/* Batches of streams (batching form; no reference counterpart: the
+ * reference holds one proxy per stream, dmc_hts_proxy.cpp:892-1000).  One
+ * proxy codes n_img (1-16) streams of one resolution together: every layer
+ * is one launch over the n_img pictures, each stream keeps its own QP,
+ * temporal context and bitstream, and each stream's bits equal those of a
+ * proxy of its own.  dcvc_hts_prepare_batch is dcvc_hts_prepare with the 8
+ * outputs of shape [n_img, Hp, Wp, 3]; add_ref_feature_from_frame then takes
+ * [n_img, Hp, Wp, 3]; compress_batch takes x [n_img, H, W, 24] and n_img
+ * entries in qps (in), bits / nbytes / ec_parallel (out; the bitstreams stay
+ * valid until the next compress); decompress_batch takes n_img bitstreams,
+ * QPs and ec_parallel values.  The single-stream compress / decompress
+ * refuse a proxy prepared for more than one stream.  batch_symbol_count is
+ * last_symbol_count of one stream (-1 for a bad index). */
+int dcvc_hts_prepare_batch(void* h, int n_img, int Hp, int Wp, void* const* frames, int ld_out);
+int dcvc_hts_compress_batch(void* h, const void* x, int ldx, int H, int W, const int* qps, int reset_feature_memory,
+                            int pad_b, int pad_r, void* stream, const uint8_t** bits, long* nbytes,
+                            int* ec_parallel);
+int dcvc_hts_decompress_batch(void* h, const uint8_t* const* bits, const long* nbytes, const int* qps, int height,
+                              int width, const int* ec_parallel, int reset_feature_memory, void* stream);
+long dcvc_hts_batch_symbol_count(void* h, int img);
 /* DMCHTLProxy (dmc_htl_proxy.h; bind.cpp exports it as DMCHTLProxy).  Same
  * call sequence and buffer contracts as the HT-S entry points above; the y
  * latent is coded in four entropy round trips (dmc_htl_proxy.cpp:718-915). */
