@@ -593,7 +593,7 @@ def batching_form(sdi, sdp, rounds=2):
 
 
 def serving_form(sdi, sdp, duration=60.0, rate=120.0, n_streams=64, slots=8, sweep=(200.0, 280.0, 360.0),
-                 sweep_seconds=10.0, p99_bound_ms=100.0):
+                 sweep_seconds=10.0, p99_bound_ms=100.0, sweep_fp8=(280.0, 360.0, 440.0)):
     """Poisson arrivals (encode or decode of one 4-frame chunk of one of 64
     persistent streams, resolutions 480p 30 %, 720p 40 %, 1080p 25 %,
     2160p 5 %) served by the continuous-batching loop, HBM budget 200 GB,
@@ -601,7 +601,8 @@ def serving_form(sdi, sdp, duration=60.0, rate=120.0, n_streams=64, slots=8, swe
     (warm start).  Then the same load on the plain path (one request at a
     time), and the capacity: the arrival rates of `sweep` for
     `sweep_seconds` each, the highest one whose p99 stays under
-    `p99_bound_ms` and whose requests are all served."""
+    `p99_bound_ms` and whose requests are all served; then the load and a
+    capacity sweep with FP8 products."""
     from dcvc_amd import serving
     res = serving.resolution_mix(n_streams, seed=0)
     ex = serving.GpuExecutor(sdi, sdp, res, chunks_per_stream=4, seed=0, keep_results=False)
@@ -630,6 +631,21 @@ def serving_form(sdi, sdp, duration=60.0, rate=120.0, n_streams=64, slots=8, swe
           if c["p99_ms"] is not None and c["p99_ms"] <= p99_bound_ms and c["served"] == c["requests"]]
     out["capacity"] = {"p99_bound_ms": p99_bound_ms, "sweep": cap,
                        "max_rate_within_bound_per_s": max(ok) if ok else None}
+    # the same streams with FP8 products (e4m3 activations): the load at
+    # `rate` and the capacity sweep, on a fresh executor
+    del ex
+    ex = serving.GpuExecutor(sdi, sdp, res, chunks_per_stream=4, seed=0, keep_results=False, precision="fp8")
+    ex.prewarm()
+    f8 = serve(rate, sweep_seconds, slots, 3)
+    cap8 = []
+    for r in sweep_fp8:
+        st = serve(r, sweep_seconds, slots, 4)
+        cap8.append({k: st[k] for k in ("rate_per_s", "served", "requests", "p50_ms", "p99_ms", "frames_per_s")})
+    ok8 = [c["rate_per_s"] for c in [f8] + cap8
+           if c["p99_ms"] is not None and c["p99_ms"] <= p99_bound_ms and c["served"] == c["requests"]]
+    out["fp8"] = {"at_rate": {k: f8[k] for k in ("rate_per_s", "duration_s", "served", "requests", "p50_ms", "p99_ms",
+                                                 "frames_per_s")},
+                  "capacity": {"sweep": cap8, "max_rate_within_bound_per_s": max(ok8) if ok8 else None}}
     return out
 
 

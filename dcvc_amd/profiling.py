@@ -20,6 +20,7 @@ Usage (after the rocprofv3 runs of `bench.py --profile-step`):
   python -m dcvc_amd.profiling traffic <fetch.csv> <write.csv> <kernel> <copy_bytes>
   python -m dcvc_amd.profiling counters <pmc.csv> <kernel> <counter,...>
   python -m dcvc_amd.profiling launch <epi> <M> <K> <N> <times>
+  python -m dcvc_amd.profiling launch_dw <H> <W> <C> <times>
 A <kernel> is a short name as printed by `stats` (e.g.
 gemm_kernel<128,256,2,2,3,6,0,0,0>) or a kernel class (e.g. conv1x1_quad).
 `launch` runs one 1x1-conv GEMM shape (N weight rows; epi bias, wsilu,
@@ -181,6 +182,21 @@ def launch(epi: str, M: int, K: int, N: int, times: int):
     torch.cuda.synchronize()
 
 
+def launch_dw(H: int, W: int, C: int, times: int):
+    """The depthwise 3x3 conv on one H x W x C picture, `times` times."""
+    import numpy as np
+    import torch
+    from . import kernels
+    dev = torch.device("cuda:0")
+    r = np.random.Generator(np.random.PCG64(0))
+    x = torch.from_numpy(r.standard_normal((H * W, C)).astype(np.float16)).to(dev)
+    wt = torch.from_numpy((r.standard_normal((9, C)) * 0.3).astype(np.float16)).to(dev)
+    y = torch.empty_like(x)
+    for _ in range(times):
+        kernels.d3x3(x, 1, H, W, wt, out=y)
+    torch.cuda.synchronize()
+
+
 def main(argv):
     if argv[0] == "stats":
         print(window_stats_markdown(argv[1], argv[2], " ".join(argv[3:])), end="")
@@ -188,6 +204,8 @@ def main(argv):
         print(json.dumps(traffic_per_launch(argv[1], argv[2], argv[3], int(argv[4])), indent=1))
     elif argv[0] == "launch":
         launch(argv[1], int(argv[2]), int(argv[3]), int(argv[4]), int(argv[5]))
+    elif argv[0] == "launch_dw":
+        launch_dw(int(argv[1]), int(argv[2]), int(argv[3]), int(argv[4]))
     elif argv[0] == "counters":
         print(json.dumps(counters_per_launch(argv[1], argv[2], argv[3].split(",")), indent=1))
     else:
