@@ -21,6 +21,7 @@ Usage (after the rocprofv3 runs of `bench.py --profile-step`):
   python -m dcvc_amd.profiling counters <pmc.csv> <kernel> <counter,...>
   python -m dcvc_amd.profiling launch <epi> <M> <K> <N> <times>
   python -m dcvc_amd.profiling launch_dw <H> <W> <C> <times>
+  python -m dcvc_amd.profiling launch_hts_batch <n> <H> <W> <chunks>
 A <kernel> is a short name as printed by `stats` (e.g.
 gemm_kernel<128,256,2,2,3,6,0,0,0>) or a kernel class (e.g. conv1x1_quad).
 `launch` runs one 1x1-conv GEMM shape (N weight rows; epi bias, wsilu,
@@ -197,6 +198,25 @@ def launch_dw(H: int, W: int, C: int, times: int):
     torch.cuda.synchronize()
 
 
+def launch_hts_batch(n: int, H: int, W: int, steps: int):
+    """n HT-S streams of H x W coded as one batch (DMCHTSBatchProxy pair),
+    `steps` chunks between two marker launches after two warm-up chunks: a
+    --kernel-trace of it shows one launch per layer for the n streams."""
+    import torch
+    from . import batching, codec
+    sdi, sdp = codec.make_state_dicts(seed=0)
+    gs, _ = batching.build_groups(sdi, sdp, [(H, W)] * n, n_chunks=2 + steps, seed=0)
+    g = gs.groups[0]
+    for c in range(2):
+        g.code_chunk(gs.chunks[0][c], gs.qps[0])
+    torch.cuda.synchronize()
+    torch.cuda._sleep(1000)  # window marker
+    for c in range(steps):
+        g.code_chunk(gs.chunks[0][2 + c], gs.qps[0])
+    torch.cuda._sleep(1000)
+    torch.cuda.synchronize()
+
+
 def main(argv):
     if argv[0] == "stats":
         print(window_stats_markdown(argv[1], argv[2], " ".join(argv[3:])), end="")
@@ -206,6 +226,8 @@ def main(argv):
         launch(argv[1], int(argv[2]), int(argv[3]), int(argv[4]), int(argv[5]))
     elif argv[0] == "launch_dw":
         launch_dw(int(argv[1]), int(argv[2]), int(argv[3]), int(argv[4]))
+    elif argv[0] == "launch_hts_batch":
+        launch_hts_batch(int(argv[1]), int(argv[2]), int(argv[3]), int(argv[4]))
     elif argv[0] == "counters":
         print(json.dumps(counters_per_launch(argv[1], argv[2], argv[3].split(",")), indent=1))
     else:
