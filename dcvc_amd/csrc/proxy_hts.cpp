@@ -11,6 +11,10 @@
 // encoder (dmc_hts_proxy.cpp:516-576) and dec_0[memory] / dec_1 / dec_2 /
 // dec_3 on the decoder (:611-704).  The QP is a device scalar read by the
 // kernels, so one graph serves all 64 QPs.
+// A proxy prepared for n_img pictures holds them one after another in every
+// buffer (the rows of picture b follow those of picture b - 1), with one
+// device QP per picture; only the compaction, the host coding and the
+// symbol buffers are handled picture by picture.
 #include "proxy_hts.h"
 
 #include <cstring>
