@@ -570,6 +570,22 @@ def batching_form(sdi, sdp, rounds=2):
         del gs
         return dt
     t_grp = {p: time_groups(p) for p in ("fp16", "bf16", "fp8")}
+
+    # 8 streams of 1920x1080 (the headline's size) as two batches of 4 on
+    # two host threads: aggregate 1080p frames/s against the one-stream path
+    def time_1080p(precision, n=8, batch=4):
+        gs, _ = batching.build_groups(sdi, sdp, [(1080, 1920)] * n, n_chunks=rounds + 1, seed=1,
+                                      precision=precision, max_batch=batch)
+        batching.code_groups_concurrently(gs.groups, [ch[0] for ch in gs.chunks], gs.qps)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for r in range(rounds):
+            batching.code_groups_concurrently(gs.groups, [ch[1 + r] for ch in gs.chunks], gs.qps)
+        dt = time.perf_counter() - t0
+        del gs
+        return round(4 * n * rounds / dt, 2)
+    out["aggregate_1080p"] = {"streams": "8 x 1920x1080, two batches of 4 on two host threads, random qp",
+                              "frames_per_s": {p: time_1080p(p) for p in ("fp16", "fp8")}}
     # per-pixel cost of each resolution alone (plain path), and the bound
     px = {k: ((k[0] + 15) // 16 * 16) * ((k[1] + 15) // 16 * 16) for k in per_res}
     us_per_mpx = {f"{k[1]}x{k[0]}": round(1e6 * float(np.mean(v)) / (px[k] / 1e6), 1) for k, v in per_res.items()}

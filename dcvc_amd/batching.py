@@ -222,9 +222,10 @@ class GroupSetup:
 
 
 def build_groups(sdi, sdp, resolutions=MIX_16, n_chunks=2, seed=0, precision="fp16", skip_thres=0.15,
-                 device="cuda"):
+                 device="cuda", max_batch=DMCHTSBatchProxy.MAX_STREAMS):
     """The streams of build_streams (same content and qps, stream for
-    stream), grouped by resolution into BatchGroups."""
+    stream), grouped by resolution into BatchGroups of at most max_batch
+    streams."""
     rng = np.random.Generator(np.random.PCG64(seed))
     qps = [int(rng.integers(0, 64)) for _ in resolutions]
     intra_e = codec.IntraCodec(sdi, skip_thres)
@@ -233,8 +234,12 @@ def build_groups(sdi, sdp, resolutions=MIX_16, n_chunks=2, seed=0, precision="fp
     by_res = {}
     for sid, hw in enumerate(resolutions):
         by_res.setdefault(hw, []).append(sid)
+    parts = []
+    for hw, sids in by_res.items():
+        for i in range(0, len(sids), max_batch):
+            parts.append((hw, sids[i:i + max_batch]))
     groups, chunks, gqps, ib = [], [], [], []
-    for (H, W), sids in by_res.items():
+    for (H, W), sids in parts:
         g = BatchGroup(sids, H, W, shared, precision)
         firsts, per_chunk = [], [[] for _ in range(n_chunks)]
         for sid in sids:
