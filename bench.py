@@ -160,6 +160,34 @@ def roofline_of(instances, classes):
     return out
 
 
+def mfma_ceiling(dev, blocks=512, iters=2000):
+    """Sustained fp16 MFMA rate on operands held in registers
+    (dcvc_mfma_calibrate, 2 waves per SIMD as in the GEMMs): random N(0, 1)
+    operands, the rate the matrix cores hold on real data, and all-zero
+    operands.  TFLOP/s, best of 3 launches."""
+    import torch
+    from dcvc_amd import _lib
+    n = blocks * 2048 * 8
+    sink = torch.empty(blocks * 256, dtype=torch.float32, device=dev)
+    gen = torch.Generator(device="cpu").manual_seed(0)
+    ops = {"random": torch.randn(n, generator=gen).half().to(dev), "zero": torch.zeros(n, dtype=torch.float16, device=dev)}
+    st = torch.cuda.current_stream()
+    flop = blocks * 4 * 16 * iters * 2.0 * 32 * 32 * 16
+    out = {}
+    for name, t in ops.items():
+        _lib.call("dcvc_mfma_calibrate", t.data_ptr(), sink.data_ptr(), blocks, 50, st.cuda_stream)
+        best = 1e9
+        for _ in range(3):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(st)
+            _lib.call("dcvc_mfma_calibrate", t.data_ptr(), sink.data_ptr(), blocks, iters, st.cuda_stream)
+            e1.record(st)
+            torch.cuda.synchronize()
+            best = min(best, e0.elapsed_time(e1))
+        out[name + "_operands_tflops"] = round(flop / (best * 1e-3) / 1e12, 1)
+    return out
+
+
 def gemm_layers(sd):
     """Every 1x1-conv GEMM layer of a state dict with the epilogue it runs
     with (dc.0: WSiLU, ffn.0: the quad GEMM, the rest: bias; shortcut adds
@@ -884,6 +912,12 @@ def main():
         instances = kernel_instances(recs)
         out["kernel_instances"] = dict(list(instances.items())[:8])
         out["roofline"] = roofline_of(instances, classes)
+        if out["roofline"]["bound"] == "mfma":
+            # the nominal peak is quoted at minimum switching activity; on
+            # real operands the matrix cores sustain less
+            mc = mfma_ceiling(dev)
+            out["roofline"]["mfma_sustained"] = mc
+            out["roofline"]["frac_of_sustained"] = round(out["roofline"]["achieved"] / mc["random_operands_tflops"], 4)
     if not args.no_tuning_ab:
         log("tuned vs heuristic tiles")
         out["tuning"] = tuning_ab(sdp, enc_i["x_hat"], dec_i["x_hat"], chunks, H, W, pad_b, pad_r, args.skip_thres,

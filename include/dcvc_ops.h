@@ -53,6 +53,15 @@ int dcvc_gemm_tile_for(int prec, int epi, int amode, int dmode, long M, int N, i
 int dcvc_d3x3_tune_set(const long long* entries, int n);
 int dcvc_d3x3_strip_for(int n_img, int H, int W, int C);
 
+/* Calibration kernel, no counterpart in the reference: measures the rate the
+ * matrix cores sustain on given operand values, so a GEMM's MFMA fraction can
+ * be read against it as well as against the nominal peak.  `blocks`
+ * workgroups of 4 waves each issue 16 * `iters` v_mfma_f32_32x32x16_f16 (2 *
+ * 32 * 32 * 16 FLOP each) on operands held in registers: `operands` holds
+ * blocks * 2048 16-byte fp16 vectors (blocks * 32 KiB, 16-byte aligned);
+ * `sink` receives blocks * 256 floats.  Asynchronous on `stream`. */
+int dcvc_mfma_calibrate(const void* operands, float* sink, int blocks, int iters, void* stream);
+
 /* ---------------- 1x1-conv GEMMs (MFMA, fp32 accumulate) ----------------
  * A [M x K] fp16 (row stride lda), W [N x K] fp16 (PyTorch [N,K,1,1]),
  * bias [N] fp32, D [M x N] fp16 (row stride ldd).  K % 64 == 0,
