@@ -160,32 +160,58 @@ def roofline_of(instances, classes):
     return out
 
 
-def mfma_ceiling(dev, blocks=512, iters=2000):
-    """Sustained fp16 MFMA rate on operands held in registers
-    (dcvc_mfma_calibrate, 2 waves per SIMD as in the GEMMs): random N(0, 1)
-    operands, the rate the matrix cores hold on real data, and all-zero
-    operands.  TFLOP/s, best of 3 launches."""
+def mfma_ceiling(dev, prec="fp16", blocks=512, iters=2000):
+    """Sustained MFMA rate on operands held in registers (dcvc_mfma_calibrate,
+    2 waves per SIMD as in the GEMMs), for the operand type of `prec`: random
+    N(0, 1) operands, the rate the matrix cores hold on real data, and
+    all-zero operands.  TFLOP/s, best of 3 launches."""
     import torch
     from dcvc_amd import _lib
-    n = blocks * 2048 * 8
-    sink = torch.empty(blocks * 256, dtype=torch.float32, device=dev)
+    code, dtype, k = {"fp16": (0, torch.float16, 16), "bf16": (1, torch.bfloat16, 16),
+                      "fp8": (2, torch.float8_e4m3fn, 64)}[prec]
+    nbytes = blocks * 64 * 1024
     gen = torch.Generator(device="cpu").manual_seed(0)
-    ops = {"random": torch.randn(n, generator=gen).half().to(dev), "zero": torch.zeros(n, dtype=torch.float16, device=dev)}
+    esize = torch.tensor([], dtype=dtype).element_size()
+    rnd = torch.randn(nbytes // esize, generator=gen).to(dtype).view(torch.uint8).to(dev)
+    ops = {"random": rnd, "zero": torch.zeros(nbytes, dtype=torch.uint8, device=dev)}
+    sink = torch.empty(blocks * 256, dtype=torch.float32, device=dev)
     st = torch.cuda.current_stream()
-    flop = blocks * 4 * 16 * iters * 2.0 * 32 * 32 * 16
+    flop = blocks * 4 * 16 * iters * 2.0 * 32 * 32 * k
     out = {}
     for name, t in ops.items():
-        _lib.call("dcvc_mfma_calibrate", t.data_ptr(), sink.data_ptr(), blocks, 50, st.cuda_stream)
+        _lib.call("dcvc_mfma_calibrate", t.data_ptr(), sink.data_ptr(), blocks, 50, code, st.cuda_stream)
         best = 1e9
         for _ in range(3):
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record(st)
-            _lib.call("dcvc_mfma_calibrate", t.data_ptr(), sink.data_ptr(), blocks, iters, st.cuda_stream)
+            _lib.call("dcvc_mfma_calibrate", t.data_ptr(), sink.data_ptr(), blocks, iters, code, st.cuda_stream)
             e1.record(st)
             torch.cuda.synchronize()
             best = min(best, e0.elapsed_time(e1))
         out[name + "_operands_tflops"] = round(flop / (best * 1e-3) / 1e12, 1)
     return out
+
+
+def copy_bandwidth(dev, mib=1024):
+    """Device-to-device copy of `mib` MiB (torch's copy kernel), read + write
+    bytes over the best of 5 event-timed copies: the rate a plain streaming
+    kernel reaches, against which the HBM-bound classes can also be read."""
+    import torch
+    n = mib << 20
+    a = torch.empty(n, dtype=torch.uint8, device=dev).fill_(1)
+    b = torch.empty_like(a)
+    st = torch.cuda.current_stream()
+    b.copy_(a)
+    best = 1e9
+    for _ in range(5):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        b.copy_(a)
+        e1.record(st)
+        torch.cuda.synchronize()
+        best = min(best, e0.elapsed_time(e1))
+    del a, b
+    return round(2.0 * n / (best * 1e-3) / 1e9, 1)
 
 
 def gemm_layers(sd):
@@ -308,9 +334,11 @@ def precision_forms(sdp, p_enc_ref, x_hat_i, chunks, H, W, pad_b, pad_r, skip, d
         if quad.any():
             tf = float(recs[quad, 5].sum()) / float(recs[quad, 7].sum()) * 1e-3  # FLOP/ns -> TFLOP/s
             peak = {"fp16": 2500.0, "bf16": 2500.0, "fp8": 5000.0}[prec]
+            sus = mfma_ceiling(dev, prec)["random_operands_tflops"]
             out.setdefault("quad_gemm", {})[prec] = {
                 "launches": int(quad.sum()), "achieved_tflops": round(tf, 1), "peak_tflops": peak,
-                "frac": round(tf / peak, 4), "operand_prec_codes": sorted({int(v) for v in recs[quad, 10]})}
+                "frac": round(tf / peak, 4), "sustained_tflops": sus, "frac_of_sustained": round(tf / sus, 4),
+                "operand_prec_codes": sorted({int(v) for v in recs[quad, 10]})}
         del e, d
         # a continuing stream from the anchor: per-chunk coded y symbols
         e, d = make(prec)
@@ -918,6 +946,12 @@ def main():
             mc = mfma_ceiling(dev)
             out["roofline"]["mfma_sustained"] = mc
             out["roofline"]["frac_of_sustained"] = round(out["roofline"]["achieved"] / mc["random_operands_tflops"], 4)
+        # a plain 1 GiB device copy: the streaming rate the HBM-bound classes
+        # can be read against
+        cb = copy_bandwidth(dev)
+        out["roofline"]["copy_gbs"] = cb
+        for v in out["roofline"]["hbm_bound_classes"].values():
+            v["frac_of_copy"] = round(v["achieved_gbs"] / cb, 4)
     if not args.no_tuning_ab:
         log("tuned vs heuristic tiles")
         out["tuning"] = tuning_ab(sdp, enc_i["x_hat"], dec_i["x_hat"], chunks, H, W, pad_b, pad_r, args.skip_thres,

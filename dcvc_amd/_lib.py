@@ -25,7 +25,7 @@ SIGNATURES = {
     "dcvc_gemm_tile_for": [I, I, I, I, L, I, I],
     "dcvc_d3x3_tune_set": [P, I],
     "dcvc_d3x3_strip_for": [I, I, I, I],
-    "dcvc_mfma_calibrate": [P, P, I, I, P],
+    "dcvc_mfma_calibrate": [P, P, I, I, I, P],
     "dcvc_conv1x1_bias": [P, I, I, I, P, I, P, P, I, P],
     "dcvc_conv1x1_bias_wsilu": [P, I, I, I, P, I, P, P, I, P],
     "dcvc_conv1x1_bias_shortcut": [P, I, I, I, P, I, P, P, I, P, I, P],

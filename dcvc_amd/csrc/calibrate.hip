@@ -1,6 +1,6 @@
-// Calibration kernel for the bench's roofline: the sustained rate of
-// v_mfma_f32_32x32x16_f16 on operands held in registers, with no memory
-// traffic in the timed loop.  Each wave keeps 16 independent 32x32
+// Calibration kernel for the bench's roofline: the sustained rate of the
+// fp16, bf16 and fp8 MFMAs the GEMMs use, on operands held in registers, with
+// no memory traffic in the timed loop.  Each wave keeps 16 independent 32x32
 // accumulators (4 A and 4 B fragments) and issues 16 MFMAs per iteration.
 // Fed random operands it gives the rate the matrix cores sustain on real
 // data; fed zeros, the rate at minimum switching activity.
@@ -9,12 +9,19 @@
 
 namespace dcvc {
 
-__global__ __launch_bounds__(256) void mfma_rate_kernel(const f16x8* __restrict__ ops, float* __restrict__ sink,
+typedef __bf16 bf16x8c __attribute__((ext_vector_type(8)));
+typedef int i32x8c __attribute__((ext_vector_type(8)));
+
+// PREC 0: v_mfma_f32_32x32x16_f16, 1: v_mfma_f32_32x32x16_bf16 (the same
+// operand bits read as bf16), 2: v_mfma_scale_f32_32x32x64_f8f6f4 on OCP e4m3
+// (scale 2^0).  Each wave reads 8 fragments of 32 bytes per lane.
+template <int PREC>
+__global__ __launch_bounds__(256) void mfma_rate_kernel(const i32x8c* __restrict__ ops, float* __restrict__ sink,
                                                         int iters) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const f16x8* src = ops + ((size_t)blockIdx.x * 4 + wave) * 8 * 64;
-    f16x8 a[4], b[4];
+    const i32x8c* src = ops + ((size_t)blockIdx.x * 4 + wave) * 8 * 64;
+    i32x8c a[4], b[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         a[i] = src[i * 64 + lane];
@@ -31,8 +38,20 @@ __global__ __launch_bounds__(256) void mfma_rate_kernel(const f16x8* __restrict_
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[i], b[j], acc[i][j], 0, 0, 0);
+            for (int j = 0; j < 4; ++j) {
+                if constexpr (PREC == 0) {
+                    const f16x8 x = __builtin_bit_cast(f16x8, __builtin_shufflevector(a[i], a[i], 0, 1, 2, 3));
+                    const f16x8 y = __builtin_bit_cast(f16x8, __builtin_shufflevector(b[j], b[j], 0, 1, 2, 3));
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(x, y, acc[i][j], 0, 0, 0);
+                } else if constexpr (PREC == 1) {
+                    const bf16x8c x = __builtin_bit_cast(bf16x8c, __builtin_shufflevector(a[i], a[i], 0, 1, 2, 3));
+                    const bf16x8c y = __builtin_bit_cast(bf16x8c, __builtin_shufflevector(b[j], b[j], 0, 1, 2, 3));
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x, y, acc[i][j], 0, 0, 0);
+                } else {
+                    acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a[i], b[j], acc[i][j], 0, 0, 0, 127,
+                                                                                0, 127);
+                }
+            }
     }
     float s = 0.f;
 #pragma unroll
@@ -46,10 +65,18 @@ __global__ __launch_bounds__(256) void mfma_rate_kernel(const f16x8* __restrict_
 
 }  // namespace dcvc
 
-extern "C" int dcvc_mfma_calibrate(const void* operands, float* sink, int blocks, int iters, void* stream) {
-    if (!operands || !sink || blocks <= 0 || blocks > 65535 || iters <= 0 || ((uintptr_t)operands) % 16)
+extern "C" int dcvc_mfma_calibrate(const void* operands, float* sink, int blocks, int iters, int prec, void* stream) {
+    if (!operands || !sink || blocks <= 0 || blocks > 65535 || iters <= 0 || prec < 0 || prec > 2 ||
+        ((uintptr_t)operands) % 32)
         return DCVC_ERR_ARG;
-    hipLaunchKernelGGL(dcvc::mfma_rate_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream,
-                       static_cast<const f16x8*>(operands), sink, iters);
+    const dim3 grid(blocks), block(256);
+    const auto* ops = static_cast<const dcvc::i32x8c*>(operands);
+    const hipStream_t s = (hipStream_t)stream;
+    if (prec == 0)
+        hipLaunchKernelGGL(dcvc::mfma_rate_kernel<0>, grid, block, 0, s, ops, sink, iters);
+    else if (prec == 1)
+        hipLaunchKernelGGL(dcvc::mfma_rate_kernel<1>, grid, block, 0, s, ops, sink, iters);
+    else
+        hipLaunchKernelGGL(dcvc::mfma_rate_kernel<2>, grid, block, 0, s, ops, sink, iters);
     return hipGetLastError() == hipSuccess ? DCVC_OK : DCVC_ERR_HIP;
 }

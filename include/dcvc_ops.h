@@ -56,11 +56,14 @@ int dcvc_d3x3_strip_for(int n_img, int H, int W, int C);
 /* Calibration kernel, no counterpart in the reference: measures the rate the
  * matrix cores sustain on given operand values, so a GEMM's MFMA fraction can
  * be read against it as well as against the nominal peak.  `blocks`
- * workgroups of 4 waves each issue 16 * `iters` v_mfma_f32_32x32x16_f16 (2 *
- * 32 * 32 * 16 FLOP each) on operands held in registers: `operands` holds
- * blocks * 2048 16-byte fp16 vectors (blocks * 32 KiB, 16-byte aligned);
- * `sink` receives blocks * 256 floats.  Asynchronous on `stream`. */
-int dcvc_mfma_calibrate(const void* operands, float* sink, int blocks, int iters, void* stream);
+ * workgroups of 4 waves each issue 16 * `iters` MFMAs on operands held in
+ * registers: prec 0 v_mfma_f32_32x32x16_f16 and 1 v_mfma_f32_32x32x16_bf16
+ * (2 * 32 * 32 * 16 FLOP each, operands the first 16 bytes of each 32-byte
+ * fragment), prec 2 v_mfma_scale_f32_32x32x64_f8f6f4 on OCP e4m3 (2 * 32 *
+ * 32 * 64 FLOP each).  `operands` holds blocks * 2048 fragments of 32 bytes
+ * (blocks * 64 KiB, 32-byte aligned); `sink` receives blocks * 256 floats.
+ * Asynchronous on `stream`. */
+int dcvc_mfma_calibrate(const void* operands, float* sink, int blocks, int iters, int prec, void* stream);
 
 /* ---------------- 1x1-conv GEMMs (MFMA, fp32 accumulate) ----------------
  * A [M x K] fp16 (row stride lda), W [N x K] fp16 (PyTorch [N,K,1,1]),
