@@ -36,6 +36,12 @@ def test_invalid_arguments_are_rejected_on_the_host():
     assert lib.dcvc_conv_bias(None, 64, 1, 5, 5, 64, None, 8, None, 5, 2, None, 8, None) == 1
     assert lib.dcvc_conditional_index(None, 3, None, 10, None, None, None, None) == 1
     assert lib.dcvc_hts_set_param(None, 0, None, None, None, None, 0.0) == 1
+    # calibration kernel: null or misaligned operands, bad sizes, unknown operand type
+    assert lib.dcvc_mfma_calibrate(None, None, 4, 10, 0, None) == 1
+    assert lib.dcvc_mfma_calibrate(32, 64, 0, 10, 0, None) == 1
+    assert lib.dcvc_mfma_calibrate(32, 64, 4, 0, 0, None) == 1
+    assert lib.dcvc_mfma_calibrate(32, 64, 4, 10, 3, None) == 1
+    assert lib.dcvc_mfma_calibrate(48, 64, 4, 10, 0, None) == 1
 
 
 def test_stream_container_layout_and_round_trip():

@@ -604,3 +604,25 @@ def test_conv1x1_grouped_equals_per_problem(gpu, epi, a_div):
         ref = kernels.conv1x1(a[i // a_div], w[i], b[i], epi=epi, s1=s1[i], s2=s2[i])
         torch.cuda.synchronize()
         assert torch.equal(got[i].view(torch.int16), ref.view(torch.int16)), f"problem {i}"
+
+
+@pytest.mark.parametrize("prec,k", [(0, 16), (1, 16), (2, 64)])
+def test_mfma_calibrate_closed_form(gpu, prec, k):
+    """The calibration kernel's MFMAs on all-ones operands: every element of
+    each of a wave's 16 accumulators gains K per MFMA, so every lane's sink is
+    exactly 16 * 16 * K * iters (fp16 / bf16 K = 16, e4m3 K = 64), and all-zero
+    operands give zero."""
+    from dcvc_amd import _lib
+    blocks, iters = 6, 25
+    one = {0: torch.float16, 1: torch.bfloat16, 2: torch.float8_e4m3fn}[prec]
+    esize = torch.tensor([], dtype=one).element_size()
+    ops = torch.ones(blocks * 64 * 1024 // esize, dtype=one).view(torch.uint8).to(gpu)
+    sink = torch.full((blocks * 256,), -1.0, dtype=torch.float32, device=gpu)
+    st = torch.cuda.current_stream().cuda_stream
+    _lib.call("dcvc_mfma_calibrate", ops.data_ptr(), sink.data_ptr(), blocks, iters, prec, st)
+    torch.cuda.synchronize()
+    assert torch.all(sink == float(16 * 16 * k * iters))
+    ops.zero_()
+    _lib.call("dcvc_mfma_calibrate", ops.data_ptr(), sink.data_ptr(), blocks, iters, prec, st)
+    torch.cuda.synchronize()
+    assert torch.all(sink == 0.0)
