@@ -340,3 +340,136 @@ def test_hts_1080p_chunks_bitexact_and_within_the_floor(gpu):
     n_o = _coded_bytes(sdp, [runs["fp32"].sym16], runs["fp32"].z_int8, 32, ec)
     print(f"HT-S 1080p bytes gpu {len(bits)} fp32 {n_o}")
     assert abs(len(bits) - n_o) <= 0.005 * n_o
+
+
+# ------------------------------------------------------------- 1920x1080
+H1080, W1080 = 1080, 1920
+
+
+def _check_floor_2(name, gpu, o32, o16):
+    """The 1080p form of _check_floor: FP32 and fp16-storage oracle runs only
+    (the float64-summation run would double the oracle's time at this size)."""
+    a = _agree(gpu, o32)
+    f1 = _agree(o16, o32)
+    n = sum(x.size for x in gpu)
+    print(f"{name} 1080p (N={n}) [symbols, indexes]: GPU~fp32 {a} | floor fp16~fp32 {f1}", flush=True)
+    for i, what in enumerate(("symbols", "indexes")):
+        assert 1 - a[i] <= 2 * (1 - f1[i]) + 5.0 / n, f"{name} {what}: GPU~fp32 {a[i]} below the floor {f1[i]}"
+
+
+def _psnr_cropped(x_hat, src):
+    return _psnr(x_hat[..., :src.shape[-2], :src.shape[-1]], src)
+
+
+@pytest.fixture(scope="module")
+def intra_1080p(gpu):
+    """A 1920x1080 DMCI frame (padded to 1088 rows) on the GPU, encoder and
+    decoder, and the oracle's FP32 and fp16-storage runs of it."""
+    from dcvc_amd.inference_extensions_cuda import DMCIProxy
+    pad_r, pad_b = synthetic.get_padding_size(H1080, W1080, 16)
+    print(f"1080p intra: padding bottom {pad_b} right {pad_r}", flush=True)
+    sdi = _state("dmci", 0)
+    frames = synthetic.motion_sequence_yuv420(H1080, W1080, 9, seed=11)
+    frame = synthetic.frame_to_input(frames[0])
+    enc, dec = DMCIProxy(), DMCIProxy()
+    enc.set_param(sdi, 0.0)
+    dec.set_param(sdi, 0.0)
+    bits, x_hat_e, ec = enc.compress(_gpu_nchw(frame, gpu), 32, pad_b, pad_r)
+    x_hat_e = x_hat_e.clone()
+    x_hat_d = dec.decompress(bits, 32, H1080, W1080, ec).clone()
+    Hp, Wp = H1080 + pad_b, W1080 + pad_r
+    nq = (Hp // 16) * (Wp // 16) * 64
+    sym = [enc.debug_copy(k, nq * 2, np.int16).copy() for k in range(4)]
+    torch.cuda.synchronize()
+    sdo = _f32(sdi)
+    xin = torch.from_numpy(frame)[None].half().float()
+    runs = {}
+    for m in ("fp32", "fp16"):
+        with torch.inference_mode(), O.storage_mode(**MODES[m]):
+            runs[m] = O.dmci_compress(sdo, xin, 32)
+    return dict(sdi=sdi, frames=frames, frame=frame, bits=bits, ec=ec, x_hat_e=x_hat_e, x_hat_d=x_hat_d, sym=sym,
+                runs=runs, pad_b=pad_b, pad_r=pad_r, Hp=Hp, Wp=Wp)
+
+
+def test_dmci_1080p_within_the_fp16_storage_floor(intra_1080p):
+    """The intra codec at 1920x1080: encoder and decoder agree bit for bit,
+    the four quarter-step symbol arrays sit within the fp16 storage floor of
+    the FP32 oracle, and PSNR / bytes stay within 0.02 dB / 0.5 %."""
+    t = intra_1080p
+    r = t["runs"]
+    assert torch.equal(t["x_hat_e"], t["x_hat_d"])
+    _check_floor_2("DMCI", t["sym"], r["fp32"].steps_dense, r["fp16"].steps_dense)
+    src = torch.from_numpy(t["frame"])[None]
+    pg, po = _psnr_cropped(t["x_hat_e"].float().cpu(), src), _psnr_cropped(r["fp32"].x_hat, src)
+    n_o = _coded_bytes(t["sdi"], r["fp32"].steps_sym16[::-1], r["fp32"].z_int8, 32, t["ec"])
+    print(f"DMCI 1080p PSNR gpu {pg:.4f} fp32 {po:.4f}; bytes gpu {len(t['bits'])} fp32 {n_o}")
+    assert abs(pg - po) <= 0.02
+    assert abs(len(t["bits"]) - n_o) <= 0.005 * n_o
+
+
+@pytest.mark.parametrize("model", ["htl", "ld"])
+def test_htl_ld_1080p_within_the_fp16_storage_floor(gpu, intra_1080p, model):
+    """HT-L (one chunk of 4 real frames padded to 8) and LD (one frame) at
+    1920x1080 from the intra frame above: encoder and decoder agree bit for
+    bit, the symbols sit within the fp16 storage floor of the FP32 oracle,
+    and PSNR / bytes stay within 0.02 dB / 0.5 % (HT-L bytes: or within twice
+    the fp16 run's own distance, as at 128x192)."""
+    from dcvc_amd.inference_extensions_cuda import DMCHTLProxy, DMCLDProxy
+    t = intra_1080p
+    pad_b, pad_r, Hp, Wp = t["pad_b"], t["pad_r"], t["Hp"], t["Wp"]
+    x_hat_i = t["x_hat_e"]
+    sdp = _state(model, 2 if model == "htl" else 3)
+    enc, dec = (DMCHTLProxy(), DMCHTLProxy()) if model == "htl" else (DMCLDProxy(), DMCLDProxy())
+    enc.set_param(sdp, 0.0)
+    dec.set_param(sdp, 0.0)
+    enc.add_ref_feature_from_frame(x_hat_i, True)
+    dec.add_ref_feature_from_frame(t["x_hat_d"], False)
+    P16 = (Hp // 16) * (Wp // 16)
+    if model == "htl":
+        x = synthetic.chunk_to_input(t["frames"][1:5], 8)
+    else:
+        x = synthetic.frame_to_input(t["frames"][1])
+    print(f"{model} 1080p: input {x.shape}", flush=True)
+    bits, ec = enc.compress(_gpu_nchw(x, gpu), 32, False, pad_b, pad_r)
+    if model == "htl":
+        sym = [enc.debug_copy(5 + k, P16 * 64 * 2, np.int16).copy() for k in range(4)]
+        yhat_e = enc.debug_copy(0, P16 * 256 * 2, np.float16).copy()
+        outs = [o.float().cpu() for o in dec.decompress(bits, 32, H1080, W1080, ec, False)]
+        yhat_d = dec.debug_copy(0, P16 * 256 * 2, np.float16)
+    else:
+        sym = [enc.debug_copy(5, P16 * 128 * 2, np.int16).copy()]
+        yhat_e = enc.debug_copy(0, P16 * 512 * 2, np.float16).reshape(P16, 512)[:, :128].copy()
+        outs = [dec.decompress(bits, 32, H1080, W1080, ec, False).float().cpu()]
+        yhat_d = dec.debug_copy(0, P16 * 512 * 2, np.float16).reshape(P16, 512)[:, :128]
+    torch.cuda.synchronize()
+    assert np.array_equal(yhat_e.view(np.uint16), yhat_d.view(np.uint16)), "decoder y_hat differs"
+    sdo = _f32(sdp)
+    ref = x_hat_i.float().cpu().contiguous()
+    xin = torch.from_numpy(x)[None].half().float()
+    runs = {}
+    for m in ("fp32", "fp16"):
+        with torch.inference_mode(), O.storage_mode(**MODES[m]):
+            if model == "htl":
+                st = O.HTSState()
+                O.htl_add_ref_feature_from_frame(sdo, st, ref, True)
+                runs[m] = O.htl_compress(sdo, st, xin, 32, False, recon=True)
+            else:
+                st = O.LDState()
+                O.ld_add_ref_feature_from_frame(sdo, st, ref, True)
+                runs[m] = O.ld_compress(sdo, st, xin, 32, False, recon=True)
+    if model == "htl":
+        _check_floor_2("HT-L", sym, runs["fp32"].steps_dense, runs["fp16"].steps_dense)
+        nb = {m: _coded_bytes(sdp, runs[m].steps_sym16[::-1], runs[m].z_int8, 32, ec) for m in runs}
+        srcs = [torch.from_numpy(x[3 * i:3 * i + 3])[None] for i in range(8)]
+        dp = max(abs(_psnr_cropped(outs[i], srcs[i]) - _psnr_cropped(runs["fp32"].x_hat[i], srcs[i]))
+                 for i in range(8))
+        tol = max(0.005 * nb["fp32"], 2 * abs(nb["fp16"] - nb["fp32"]))
+    else:
+        _check_floor_2("LD", sym, [runs["fp32"].sym16_dense], [runs["fp16"].sym16_dense])
+        nb = {m: _coded_bytes(sdp, [runs[m].sym16], runs[m].z_int8, 32, ec) for m in runs}
+        src = torch.from_numpy(x)[None]
+        dp = abs(_psnr_cropped(outs[0], src) - _psnr_cropped(runs["fp32"].x_hat, src))
+        tol = 0.005 * nb["fp32"]
+    print(f"{model} 1080p bytes gpu {len(bits)} {nb}; max PSNR delta {dp:.4f} dB", flush=True)
+    assert dp <= 0.02
+    assert abs(len(bits) - nb["fp32"]) <= tol
