@@ -30,6 +30,9 @@
 //     finishes 8 consecutive channels of one pixel (shortcut, second
 //     shortcut, per-channel quant) with 16-byte loads and stores, so every
 //     output row segment is written whole;
+//   * configurations 8-11 take 64-deep K tiles (128-byte LDS rows, one cache
+//     line per row and tile): half the line requests per byte, for the
+//     same per-element accumulation order;
 //   * XCD-aware tile order: hardware hands block b to XCD b % 8, so
 //     consecutive tiles (the N tiles of one M tile, which share A) are
 //     assigned to blocks of the same XCD and meet in its L2.
@@ -45,6 +48,7 @@
 #include <mutex>
 #include <shared_mutex>
 #include <unordered_map>
+#include <utility>
 
 #include "common.h"
 #include "gemm.h"
@@ -98,7 +102,8 @@ DCVC_DEV i32x8 to_fp8x32(const f16x8 (&v)[4]) {
 
 }  // namespace
 
-template <int BM_, int BN_, int WM_, int WN_, int STAGES_, int EPI_, int AMODE_, int DMODE_, int PREC_ = PREC_F16>
+template <int BM_, int BN_, int WM_, int WN_, int STAGES_, int EPI_, int AMODE_, int DMODE_, int PREC_ = PREC_F16,
+          int KB_ = 32>
 struct Tile {
     static constexpr int BM = BM_, BN = BN_, WM = WM_, WN = WN_;
     static constexpr int PREC = PREC_;
@@ -107,7 +112,16 @@ struct Tile {
     // K per ring tile: 32 (one 64-byte fp16 row), or 64 for fp8, whose B row
     // is 64 fp8 bytes and whose A tile is two fp16 half tiles (one 64-byte
     // row of e4m3 bytes under kA8)
-    static constexpr int kKT = kFp8 ? 64 : 32;
+    // fp16 / bf16 rings take K tiles of KB_ = 32 (64-byte LDS rows, 16 rows
+    // per DMA wave instruction) or 64 (128-byte rows, 8 rows per
+    // instruction).  A 128-byte row is one whole cache line per row and K
+    // tile, so each DMA instruction requests half as many lines; the plain
+    // load stream of the quad GEMM runs about 1.45x faster that way.
+    static constexpr int kKT = kFp8 ? 64 : KB_;
+    static constexpr int kRB = kFp8 ? 64 : 2 * KB_;    // LDS row bytes
+    static constexpr int kLPR = kRB / 16;              // DMA lanes per row
+    static constexpr int kRPI = 64 / kLPR;             // rows per DMA instruction
+    static constexpr int kSteps = kFp8 ? 1 : KB_ / 16;  // k16 MFMA steps per K tile
     static constexpr int kAH = (kFp8 && !kA8) ? 2 : 1;
     // LDS ring of STAGES K tiles, STAGES - 2 in flight ahead of the one being
     // read: the slot refilled at iteration kt is that of tile kt - 2, whose
@@ -134,10 +148,10 @@ struct Tile {
     static constexpr int TNO = kQuad ? TN / 4 : TN;  // output columns per wave
     static constexpr int ARW = BM / kWaves;           // A rows each wave loads per K tile
     static constexpr int BRW = BN / kWaves;
-    static constexpr int AI = ARW / 16;               // DMA instructions (16 rows each)
-    static constexpr int BI = BRW / 16;
+    static constexpr int AI = ARW / kRPI;             // DMA instructions (kRPI rows each)
+    static constexpr int BI = BRW / kRPI;
     static constexpr int kDma = AI * kAH + BI;        // per lane per K tile
-    static constexpr int kStageBytes = (BM * kAH + BN) * kRowBytes;
+    static constexpr int kStageBytes = (BM * kAH + BN) * kRB;
     static constexpr int kEPitch = BNO + 4;           // epilogue fp32 row pitch
     static constexpr int kERows = WM * 32;            // rows staged per epilogue pass
     static constexpr int kEChunks = kERows * BNO / 8;
@@ -146,10 +160,13 @@ struct Tile {
                                             : (size_t)kERows * kEPitch * 4;
     static_assert(TM % 32 == 0 && TN % 32 == 0, "wave tile");
     static_assert(!kQuad || SN % 4 == 0, "quad GEMM needs the 4 B slabs of an output in one wave");
-    static_assert(ARW % 16 == 0 && BRW % 16 == 0, "DMA split");
+    static_assert(ARW % kRPI == 0 && BRW % kRPI == 0 && AI > 0 && BI > 0, "DMA split");
+    static_assert(KB_ == 32 || (KB_ == 64 && !kFp8 && AMODE_ == A_PLAIN), "64-deep K tiles: plain fp16 / bf16 A");
     static_assert(kEChunks % kThreads == 0, "epilogue split");
     static_assert(kAhead * kDma < 64, "vmcnt range");
-    static_assert(kStages >= 3 && kAhead >= 2, "ring depth");
+    // two slots (64-deep K tiles only): the slot refilled at iteration kt is
+    // that of tile kt - 1, whose fragment reads the barrier's lgkmcnt(0) retired
+    static_assert((kStages >= 3 && kAhead >= 2) || (KB_ == 64 && kStages == 2 && kAhead == 1), "ring depth");
     static_assert(kLdsBytes <= 160 * 1024, "LDS");
 };
 
@@ -185,18 +202,22 @@ __global__ __launch_bounds__(T::kThreads, T::kWavesPerSimd) void gemm_kernel(Gem
 
     const __amdgpu_buffer_rsrc_t rA = make_rsrc(g.A, g.a_bytes);
     const __amdgpu_buffer_rsrc_t rB = make_rsrc(g.B, g.b_bytes);
-    // DMA lane geometry: a 16-row group is 1 KB of LDS; lane -> row dr,
-    // 16-byte position dp, holding K chunk kc (row bases are multiples of 16,
-    // so the swizzle of the row equals that of dr)
-    const int dr = lane >> 2;
-    const int kc = (lane & 3) ^ swz(dr);
+    // DMA lane geometry: one wave instruction fills kRPI consecutive LDS rows
+    // (1 KB); lane -> row dr of the group and 16-byte position dp, which
+    // holds K chunk dp ^ swizzle(row of the operand's block)
+    constexpr int RB = T::kRB;
+    const int dr = lane / T::kLPR;
+    const int dp = lane % T::kLPR;
 
     // ---- A rows of this lane's DMAs (fixed over the K loop) ----
     uint32_t a_off[T::AI];
-    int a_pix[T::AI], a_h[T::AI], a_w[T::AI];
+    int a_kc[T::AI], a_pix[T::AI], a_h[T::AI], a_w[T::AI];
 #pragma unroll
     for (int q = 0; q < T::AI; ++q) {
-        const int m = m0 + wave * T::ARW + q * 16 + dr;
+        const int r = wave * T::ARW + q * T::kRPI + dr;
+        const int m = m0 + r;
+        const int kc = dp ^ swz_rows<RB>(r);
+        a_kc[q] = kc;
         if constexpr (AMODE == A_PLAIN && T::kA8) {
             a_off[q] = (m < g.M) ? (uint32_t)((size_t)m * g.lda + kc * 16) : kOOB;
         } else if constexpr (AMODE == A_PLAIN) {
@@ -226,7 +247,8 @@ __global__ __launch_bounds__(T::kThreads, T::kWavesPerSimd) void gemm_kernel(Gem
     uint32_t b_off[T::BI];
 #pragma unroll
     for (int q = 0; q < T::BI; ++q) {
-        const int row = wave * T::BRW + q * 16 + dr;
+        const int row = wave * T::BRW + q * T::kRPI + dr;
+        const int kc = dp ^ swz_rows<RB>(row);
         int grow;
         bool valid;
         if constexpr (T::kQuad) {
@@ -254,15 +276,15 @@ __global__ __launch_bounds__(T::kThreads, T::kWavesPerSimd) void gemm_kernel(Gem
     auto dma_tile = [&](int kt, int st) {
         const uint32_t past = (kt < KT) ? 0u : kOOB;
         const int kbase = (kt < KT) ? kt * T::kKT : 0;
-        char* sb = smem + st * T::kStageBytes + (BM * T::kAH + wave * T::BRW) * kRowBytes;
+        char* sb = smem + st * T::kStageBytes + (BM * T::kAH + wave * T::BRW) * RB;
 #pragma unroll
         for (int h = 0; h < T::kAH; ++h) {
             const int k0 = kbase + h * kBK;
-            char* sa = smem + st * T::kStageBytes + (h * BM + wave * T::ARW) * kRowBytes;
+            char* sa = smem + st * T::kStageBytes + (h * BM + wave * T::ARW) * RB;
             if constexpr (AMODE == A_PLAIN) {
 #pragma unroll
                 for (int q = 0; q < T::AI; ++q)
-                    dma16(rA, sa + q * 16 * kRowBytes, (a_off[q] + (uint32_t)k0 * (T::kA8 ? 1 : 2)) | past);
+                    dma16(rA, sa + q * T::kRPI * RB, (a_off[q] + (uint32_t)k0 * (T::kA8 ? 1 : 2)) | past);
             } else {
                 // cin % 64 == 0, so one K tile never straddles two taps
                 const int tap = k0 / g.cin;
@@ -280,14 +302,14 @@ __global__ __launch_bounds__(T::kThreads, T::kWavesPerSimd) void gemm_kernel(Gem
                     const int hi = a_h[q] + dy, wi = a_w[q] + dx;
                     const bool ok = (unsigned)hi < (unsigned)g.inH && (unsigned)wi < (unsigned)g.inW;
                     const uint32_t off =
-                        ok ? (uint32_t)(((size_t)(a_pix[q] + hi * g.inW + wi) * g.lda + c0 + kc * 8) * 2) : kOOB;
-                    dma16(rA, sa + q * 16 * kRowBytes, off | past);
+                        ok ? (uint32_t)(((size_t)(a_pix[q] + hi * g.inW + wi) * g.lda + c0 + a_kc[q] * 8) * 2) : kOOB;
+                    dma16(rA, sa + q * T::kRPI * RB, off | past);
                 }
             }
         }
         const uint32_t kb = T::kFp8 ? (uint32_t)kbase : (uint32_t)kbase * 2;
 #pragma unroll
-        for (int q = 0; q < T::BI; ++q) dma16(rB, sb + q * 16 * kRowBytes, (b_off[q] + kb) | past);
+        for (int q = 0; q < T::BI; ++q) dma16(rB, sb + q * T::kRPI * RB, (b_off[q] + kb) | past);
     };
 
     f32x16 acc[SM][SN];
@@ -300,19 +322,23 @@ __global__ __launch_bounds__(T::kThreads, T::kWavesPerSimd) void gemm_kernel(Gem
 
     // fragment addresses: lane reads row lr, K chunk 2 * ks + (lane >> 5)
     const int lr = lane & 31;
-    const int fa_row = (wm * T::TM + lr) * kRowBytes;
-    const int fb_row = (BM * T::kAH + wn * T::TN + lr) * kRowBytes;
-    const int pos0 = ((0 + (lane >> 5)) ^ swz(lr)) * 16;
-    const int pos1 = ((2 + (lane >> 5)) ^ swz(lr)) * 16;
-    f16x8 f0a[SM], f0b[SN], f1a[SM], f1b[SN];
+    // (fragment rows sit at multiples of 32 in their operand block, so the
+    // swizzle of the row is that of lr)
+    const int fa_row = (wm * T::TM + lr) * RB;
+    const int fb_row = (BM * T::kAH + wn * T::TN + lr) * RB;
+    int pos[T::kSteps];
+#pragma unroll
+    for (int s = 0; s < T::kSteps; ++s) pos[s] = ((2 * s + (lane >> 5)) ^ swz_rows<RB>(lr)) * 16;
+    // two fragment sets, alternating k16 steps
+    f16x8 fra[2][SM], frb[2][SN];
     auto read_frags = [&](int st, int pos, f16x8* fa, f16x8* fb) {
         const char* base = smem + st * T::kStageBytes + pos;
 #pragma unroll
         for (int i = 0; i < SM; ++i)
-            fa[i] = *reinterpret_cast<const f16x8*>(base + fa_row + i * 32 * kRowBytes);
+            fa[i] = *reinterpret_cast<const f16x8*>(base + fa_row + i * 32 * RB);
 #pragma unroll
         for (int j = 0; j < SN; ++j)
-            fb[j] = *reinterpret_cast<const f16x8*>(base + fb_row + j * 32 * kRowBytes);
+            fb[j] = *reinterpret_cast<const f16x8*>(base + fb_row + j * 32 * RB);
     };
     auto mfma_all = [&](const f16x8* fa, const f16x8* fb) {
         if constexpr (T::PREC == PREC_BF16) {
@@ -337,7 +363,7 @@ __global__ __launch_bounds__(T::kThreads, T::kWavesPerSimd) void gemm_kernel(Gem
     // tile lane >> 5, all four chunks), converted; B: the matching 32 fp8
     // bytes (chunks 2h, 2h + 1 of the 64-byte row).  Any k order works as
     // long as A and B use the same one.
-    const int a8_row = ((lane >> 5) * BM + wm * T::TM + lr) * kRowBytes;
+    const int a8_row = ((lane >> 5) * BM + wm * T::TM + lr) * RB;
     // e4m3 row: the lane's 32 bytes are chunks 2h, 2h + 1 (h = lane >> 5)
     auto read_row8 = [&](const char* rowp) {
         const int c0 = 2 * (lane >> 5);
@@ -350,17 +376,17 @@ __global__ __launch_bounds__(T::kThreads, T::kWavesPerSimd) void gemm_kernel(Gem
 #pragma unroll
         for (int i = 0; i < SM; ++i) {
             if constexpr (T::kA8) {
-                fa[i] = read_row8(base + fa_row + i * 32 * kRowBytes);
+                fa[i] = read_row8(base + fa_row + i * 32 * RB);
             } else {
                 f16x8 v[4];
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
-                    v[c] = *reinterpret_cast<const f16x8*>(base + a8_row + i * 32 * kRowBytes + ((c ^ swz(lr)) * 16));
+                    v[c] = *reinterpret_cast<const f16x8*>(base + a8_row + i * 32 * RB + ((c ^ swz(lr)) * 16));
                 fa[i] = to_fp8x32(v);
             }
         }
 #pragma unroll
-        for (int j = 0; j < SN; ++j) fb[j] = read_row8(base + fb_row + j * 32 * kRowBytes);
+        for (int j = 0; j < SN; ++j) fb[j] = read_row8(base + fb_row + j * 32 * RB);
     };
     auto mfma_fp8 = [&](const i32x8* fa, const i32x8* fb) {
 #pragma unroll
@@ -444,24 +470,35 @@ __global__ __launch_bounds__(T::kThreads, T::kWavesPerSimd) void gemm_kernel(Gem
         }
         mfma_fp8(pa, pb);
     } else {
+        // k16 step s of a tile reads into fragment set s & 1 and runs its
+        // MFMAs one step later, so the fragment reads of a step overlap the
+        // MFMAs of the one before; the last step of tile kt - 1 runs after
+        // the barrier of tile kt.  The tile's DMAs are spread over its steps.
+        constexpr int NS = T::kSteps;
+        static_assert(NS % 2 == 0, "the last step of a tile lands in set 1");
         wait_tile_and_barrier<(T::kAhead - 1) * T::kDma, T::kWaitLds>();
-        dma_tile(T::kAhead, T::kAhead);
-        read_frags(0, pos0, f0a, f0b);
-        read_frags(0, pos1, f1a, f1b);
-        mfma_all(f0a, f0b);
+        dma_tile(T::kAhead, T::kAhead % T::kStages);
+        read_frags(0, pos[0], fra[0], frb[0]);
+#pragma unroll
+        for (int s = 1; s < NS; ++s) {
+            read_frags(0, pos[s], fra[s & 1], frb[s & 1]);
+            mfma_all(fra[(s - 1) & 1], frb[(s - 1) & 1]);
+        }
         for (int kt = 1; kt < KT; ++kt) {
             // this wave's DMAs of tile kt have landed; after the barrier, every wave's
             wait_tile_and_barrier<(T::kAhead - 1) * T::kDma, T::kWaitLds>();
             const int st = kt % T::kStages;
             dma_tile(kt + T::kAhead, (kt + T::kAhead) % T::kStages);
-            read_frags(st, pos0, f0a, f0b);
-            mfma_all(f1a, f1b);  // second k16 step of tile kt - 1
-            interleave<NDS, NMF, T::kDma>();
-            read_frags(st, pos1, f1a, f1b);
-            mfma_all(f0a, f0b);
-            interleave<NDS, NMF, 0>();
+            static_for(std::make_integer_sequence<int, NS>{}, [&](auto sc) {
+                constexpr int s = decltype(sc)::value;
+                read_frags(st, pos[s], fra[s & 1], frb[s & 1]);
+                mfma_all(fra[(s + 1) & 1], frb[(s + 1) & 1]);  // step s - 1 (of tile kt - 1 for s = 0)
+                // 32-deep tiles issue all their DMAs in the first step
+                interleave<NDS, NMF, NS == 2 ? (s == 0 ? T::kDma : 0)
+                                             : (T::kDma * (s + 1)) / NS - (T::kDma * s) / NS>();
+            });
         }
-        mfma_all(f1a, f1b);
+        mfma_all(fra[1], frb[1]);
     }
     __syncthreads();  // every wave is done with the ring before the epilogue reuses it
 
@@ -577,11 +614,17 @@ __global__ __launch_bounds__(T::kThreads, T::kWavesPerSimd) void gemm_kernel(Gem
 //   quad (B rows; 4 B rows per output): 1: 256 x 256 5 st, 2: 128 x 256 5 st,
 //   3: 128 x 128 waves 4 x 1, 4: 128 x 256 3 st, 5: 256 x 256 4 st,
 //   6 / 7: 256 x 256 8 waves, 4 / 3 st
-constexpr int kPolicies = 7;
+//   64-deep K tiles (128-byte LDS rows; plain 1x1 products in fp16 / bf16,
+//   other launches fall back to the automatic choice):
+//   8: 256 x 256, wave 128 x 128, 2 stages (1 block / CU)
+//   9: 256 x 256, 8 waves (4 x 2, wave 64 x 128), 2 stages
+//   10: 256 x 128 (quad: 128 x 256), 3 stages (1 block / CU)
+//   11: 128 x 128 (quad: waves 4 x 1), 2 stages (2 blocks / CU)
+constexpr int kPolicies = 11;
 
-template <int BM, int BN, int WM, int WN, int ST, int EPI, int AMODE, int DMODE, int PREC = PREC_F16>
+template <int BM, int BN, int WM, int WN, int ST, int EPI, int AMODE, int DMODE, int PREC = PREC_F16, int KB = 32>
 static hipError_t launch_tile(GemmArgs g, hipStream_t stream) {
-    using T = Tile<BM, BN, WM, WN, ST, EPI, AMODE, DMODE, PREC>;
+    using T = Tile<BM, BN, WM, WN, ST, EPI, AMODE, DMODE, PREC, KB>;
     const int tiles_m = ceil_div(g.M, BM);
     const int tiles_n = ceil_div(g.N, T::BNO);
     auto kern = gemm_kernel<T>;
@@ -626,6 +669,8 @@ static int pick_tile(const GemmArgs& g, int epi, int amode, int dmode) {
 
 template <int EPI, int AMODE, int DMODE, int PREC = PREC_F16>
 static hipError_t dispatch_tiles(const GemmArgs& g, hipStream_t stream) {
+    // 64-deep K tiles (configurations 8-11) serve plain 1x1 products only
+    constexpr bool kPlain = (AMODE == A_PLAIN && DMODE == D_PLAIN);
     const int t = pick_tile(g, EPI, AMODE, DMODE);
     if constexpr (PREC == PREC_FP8 || PREC == PREC_FP8A) {
         // fp8 K tiles are 48 KB (256 x 256) / 24 KB (128 x 128): 3 stages.  The
@@ -651,6 +696,14 @@ static hipError_t dispatch_tiles(const GemmArgs& g, hipStream_t stream) {
         case 5: return launch_tile<256, 256, 2, 2, 4, EPI, AMODE, DMODE, PREC>(g, stream);
         case 6: return launch_tile<256, 256, 4, 2, 4, EPI, AMODE, DMODE, PREC>(g, stream);
         case 7: return launch_tile<256, 256, 4, 2, 3, EPI, AMODE, DMODE, PREC>(g, stream);
+        case 8: if constexpr (kPlain) return launch_tile<256, 256, 2, 2, 2, EPI, AMODE, DMODE, PREC, 64>(g, stream);
+        [[fallthrough]];
+        case 9: if constexpr (kPlain) return launch_tile<256, 256, 4, 2, 2, EPI, AMODE, DMODE, PREC, 64>(g, stream);
+        [[fallthrough]];
+        case 10: if constexpr (kPlain) return launch_tile<128, 256, 2, 2, 3, EPI, AMODE, DMODE, PREC, 64>(g, stream);
+        [[fallthrough]];
+        case 11: if constexpr (kPlain) return launch_tile<128, 128, 4, 1, 2, EPI, AMODE, DMODE, PREC, 64>(g, stream);
+        [[fallthrough]];
         default: return launch_tile<128, 128, 4, 1, 5, EPI, AMODE, DMODE, PREC>(g, stream);
         }
     } else {
@@ -661,6 +714,14 @@ static hipError_t dispatch_tiles(const GemmArgs& g, hipStream_t stream) {
         case 5: return launch_tile<128, 256, 2, 2, 3, EPI, AMODE, DMODE, PREC>(g, stream);
         case 6: return launch_tile<256, 256, 4, 2, 4, EPI, AMODE, DMODE, PREC>(g, stream);
         case 7: return launch_tile<256, 256, 4, 2, 3, EPI, AMODE, DMODE, PREC>(g, stream);
+        case 8: if constexpr (kPlain) return launch_tile<256, 256, 2, 2, 2, EPI, AMODE, DMODE, PREC, 64>(g, stream);
+        [[fallthrough]];
+        case 9: if constexpr (kPlain) return launch_tile<256, 256, 4, 2, 2, EPI, AMODE, DMODE, PREC, 64>(g, stream);
+        [[fallthrough]];
+        case 10: if constexpr (kPlain) return launch_tile<256, 128, 2, 2, 3, EPI, AMODE, DMODE, PREC, 64>(g, stream);
+        [[fallthrough]];
+        case 11: if constexpr (kPlain) return launch_tile<128, 128, 2, 2, 2, EPI, AMODE, DMODE, PREC, 64>(g, stream);
+        [[fallthrough]];
         default: return launch_tile<128, 128, 2, 2, 5, EPI, AMODE, DMODE, PREC>(g, stream);
         }
     }

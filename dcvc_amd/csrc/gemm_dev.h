@@ -3,6 +3,9 @@
 // instruction interleave pattern of the K loop.
 #pragma once
 
+#include <type_traits>
+#include <utility>
+
 #include "common.h"
 
 namespace dcvc {
@@ -28,6 +31,18 @@ DCVC_DEV void dma16(__amdgpu_buffer_rsrc_t r, char* dst, uint32_t off) {
 // XOR swizzle of the 16-byte K chunk within a 64-byte LDS row.
 DCVC_DEV int swz(int row) { return ((row >> 2) ^ (row >> 1)) & 3; }
 
+// The same for an LDS row of RB bytes (64: 4 chunks, swz; 128: 8 chunks).  A
+// 128-byte row spans 32 of the 64 banks, so the 16 rows one quarter-wave
+// fragment read touches are spread by giving each row pair its own chunk
+// permutation: bank group = (row & 1) * 8 + (chunk ^ ((row >> 1) & 7)).
+template <int RB>
+DCVC_DEV int swz_rows(int row) {
+    if constexpr (RB == 64)
+        return swz(row);
+    else
+        return (row >> 1) & 7;
+}
+
 // s_waitcnt vmcnt(VM) [lgkmcnt(0) if LGKM] (gfx9 encoding), then a workgroup
 // barrier.  The compiler fences keep LDS accesses on their side of the
 // barrier; no memory-model fence is used because it would also wait for the
@@ -40,6 +55,13 @@ DCVC_DEV void wait_tile_and_barrier() {
     __builtin_amdgcn_s_waitcnt(imm);
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");
+}
+
+// f(std::integral_constant<int, I>) for I in the sequence, in order (a loop
+// whose index is a template argument).
+template <int... I, class F>
+DCVC_DEV void static_for(std::integer_sequence<int, I...>, F&& f) {
+    (f(std::integral_constant<int, I>{}), ...);
 }
 
 // Scheduling pattern for one k16 step: NDS fragment reads, NMF MFMAs and NVM

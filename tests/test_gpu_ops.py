@@ -138,7 +138,9 @@ def test_gemm_batch_invariance(gpu):
                                  "s1k3", "subpel", "subpel3"])
 def test_gemm_tile_configs_bit_identical(gpu, epi):
     """Every tile configuration gives the same bits (same per-element accumulation
-    order), including ragged M / N tails, implicit convs and the sub-pixel store."""
+    order), including ragged M / N tails, implicit convs and the sub-pixel store.
+    Configurations 8-11 (64-deep K tiles, 128-byte LDS rows) serve plain 1x1
+    products; the other launches fall back to the automatic choice."""
     from dcvc_amd import _lib, kernels
     lib = _lib.load()
     r = rng(21)
@@ -171,10 +173,37 @@ def test_gemm_tile_configs_bit_identical(gpu, epi):
 
     outs = []
     try:
-        for policy in (1, 2, 3, 4, 5, 6, 7):
+        for policy in range(1, 12):
             assert lib.dcvc_gemm_set_tile_policy(policy) == 0
             r0 = rng(5)
             outs.append(run())
+    finally:
+        lib.dcvc_gemm_set_tile_policy(0)
+    torch.cuda.synchronize()
+    assert torch.isfinite(outs[0].float()).all()
+    for o in outs[1:]:
+        assert torch.equal(o.view(torch.int16), outs[0].view(torch.int16))
+
+
+@pytest.mark.parametrize("epi", ["wsilu", "shortcut", "quad"])
+def test_bf16_tile_configs_bit_identical(gpu, epi):
+    """The BF16 products give the same bits in every tile configuration,
+    the 64-deep K tiles included."""
+    from dcvc_amd import _lib, kernels
+    lib = _lib.load()
+    r = rng(23)
+    M, K, N = 3001, 768, 136
+    rows = 4 * N if epi == "quad" else N
+    a = f16(r.standard_normal((M, K)), gpu)
+    w = f16(r.standard_normal((rows, K)) / 25, gpu)
+    s1 = f16(r.standard_normal((M, N)), gpu)
+    wq, scale = kernels.weights_to_lowp(w, "bf16")
+    bias = torch.full((rows,), 0.05, device=gpu)
+    outs = []
+    try:
+        for policy in range(1, 12):
+            assert lib.dcvc_gemm_set_tile_policy(policy) == 0
+            outs.append(kernels.gemm_lowp(a, wq, scale, "bf16", bias, epi=epi, s1=s1 if epi == "shortcut" else None))
     finally:
         lib.dcvc_gemm_set_tile_policy(0)
     torch.cuda.synchronize()
