@@ -14,8 +14,12 @@ this file) that the drop-in proxies install at ``set_param``
 (``ensure_loaded``), before any graph is captured.  Tile choice never
 changes results, so the database only moves time.
 
+The HT-S batches of the batching form (streams of one resolution coded as
+one batch, M = streams x pixels) are tuned the same way.
+
 Usage on a GPU:  python -m dcvc_amd.tuning [--out PATH] [--models hts,htl,ld]
                                             [--res 1080x1920,...] [--reps 3]
+                                            [--batches 4x1080x1920,...]
 """
 from __future__ import annotations
 
@@ -34,6 +38,9 @@ from . import _lib
 
 DB_PATH = Path(__file__).resolve().with_name("tuning_db.json")
 RESOLUTIONS = [(2160, 3840), (1080, 1920), (720, 1280), (480, 832), (480, 854), (240, 416)]
+# HT-S batches (streams, H, W): the resolution groups of the batching form's
+# 16-stream mix (batching.MIX_16), each layer one launch over the group
+BATCHES = [(4, 480, 854), (6, 720, 1280), (4, 1080, 1920), (2, 2160, 3840)]
 # 6 and 7 hold one block per CU: faster alone on some shapes, slower inside
 # the codecs, whose concurrent streams need a second block to share the CU
 GEMM_CANDIDATES = (1, 2, 3, 4, 5)
@@ -132,11 +139,40 @@ def _pass_fn(model, H, W, prec, seed=0):
     return run
 
 
-def tune_one(model, H, W, prec="fp16", reps=3):
-    """Per-shape timings of one (model, resolution): returns (gemm, d3x3) dicts
-    key -> {candidate: total microseconds}, plus the heuristic's choice."""
+def _batch_pass_fn(n, H, W, prec, seed=0):
+    """One chunk encode + decode of `n` HT-S streams of H x W coded as one
+    batch (batching.BatchGroup: every layer one launch over the n pictures,
+    M = n x pixels), graphs off.  The streams' I frames are coded once, before
+    the timed passes; returns a callable that codes the next chunk."""
     import torch
-    run = _pass_fn(model, H, W, prec)
+    from . import batching, codec, synthetic
+    dev = torch.device("cuda:0")
+    sdi, sdp = codec.make_state_dicts(seed=seed, video="hts")
+    shared = codec.VideoCodecHTS(sdp, 0.15)
+    group = batching.BatchGroup(range(n), H, W, shared, prec)
+    group.enc.set_use_graphs(False)
+    group.dec.set_use_graphs(False)
+    ie, idec = codec.IntraCodec(sdi, 0.15), codec.IntraCodec(sdi, 0.15)
+    firsts, xs = [], []
+    for i in range(n):
+        frames = synthetic.motion_sequence_yuv420(H, W, 5, seed=seed + 1 + 17 * i)
+        firsts.append(codec.to_input(synthetic.frame_to_input(frames[0]), dev))
+        xs.append(codec.to_input(synthetic.chunk_to_input(frames[1:5], 8), dev))
+    group.start(ie, idec, firsts)
+    qps = [32] * n
+
+    def run():
+        group.code_chunk(xs, qps)
+
+    return run
+
+
+def tune_one(model, H, W, prec="fp16", reps=3, batch=1):
+    """Per-shape timings of one (model, resolution), or with batch > 1 of an
+    HT-S batch of that many streams: returns (gemm, d3x3) dicts key ->
+    {candidate: total microseconds}, plus the heuristic's choice."""
+    import torch
+    run = _pass_fn(model, H, W, prec) if batch == 1 else _batch_pass_fn(batch, H, W, prec)
     run()  # allocation, first-launch costs
     torch.cuda.synchronize()
     clear()
@@ -195,29 +231,32 @@ def choose(entry, margin=0.03):
     return best
 
 
-def tune(models=("hts", "htl", "ld"), resolutions=RESOLUTIONS, precs=("fp16",), reps=3, log=print):
+def tune(models=("hts", "htl", "ld"), resolutions=RESOLUTIONS, precs=("fp16",), reps=3, batches=BATCHES, log=print):
+    """Tunes every (model, precision, resolution), then the HT-S batches
+    (streams, H, W) of the batching form."""
     gemm, d3 = {}, {}
     t_start = time.time()
-    for model in models:
-        for prec in precs:
-            for (H, W) in resolutions:
-                t0 = time.time()
-                g, d = tune_one(model, H, W, prec=prec, reps=reps)
-                for k, e in g.items():
-                    acc = gemm.setdefault(k, {"heuristic": e["heuristic"], "us": {}})
-                    for c, v in e["us"].items():
-                        acc["us"][c] = acc["us"].get(c, 0.0) + v
-                for k, e in d.items():
-                    acc = d3.setdefault(k, {"heuristic": e["heuristic"], "us": {}})
-                    for c, v in e["us"].items():
-                        acc["us"][c] = acc["us"].get(c, 0.0) + v
-                log(f"tuned {model} {prec} {H}x{W}: {len(g)} GEMM shapes, {len(d)} depthwise shapes, "
-                    f"{time.time() - t0:.1f} s", flush=True)
+    jobs = [(model, prec, H, W, 1) for model in models for prec in precs for (H, W) in resolutions]
+    jobs += [("hts", prec, H, W, n) for prec in precs for (n, H, W) in batches]
+    for model, prec, H, W, n in jobs:
+        t0 = time.time()
+        g, d = tune_one(model, H, W, prec=prec, reps=reps, batch=n)
+        for k, e in g.items():
+            acc = gemm.setdefault(k, {"heuristic": e["heuristic"], "us": {}})
+            for c, v in e["us"].items():
+                acc["us"][c] = acc["us"].get(c, 0.0) + v
+        for k, e in d.items():
+            acc = d3.setdefault(k, {"heuristic": e["heuristic"], "us": {}})
+            for c, v in e["us"].items():
+                acc["us"][c] = acc["us"].get(c, 0.0) + v
+        what = f"{model} {prec} {H}x{W}" + (f" batch of {n}" if n > 1 else "")
+        log(f"tuned {what}: {len(g)} GEMM shapes, {len(d)} depthwise shapes, {time.time() - t0:.1f} s", flush=True)
     import torch
     db = {"device": torch.cuda.get_device_name(0), "created_by": "python -m dcvc_amd.tuning",
           "models": list(models), "precisions": list(precs),
-          "resolutions": [f"{W}x{H}" for H, W in resolutions], "tuning_seconds": round(time.time() - t_start, 1),
-          "gemm": [], "d3x3": []}
+          "resolutions": [f"{W}x{H}" for H, W in resolutions],
+          "hts_batches": [f"{n}x{W}x{H}" for n, H, W in batches],
+          "tuning_seconds": round(time.time() - t_start, 1), "gemm": [], "d3x3": []}
     for (prec, epi, amode, dmode, M, N, K), e in sorted(gemm.items()):
         db["gemm"].append({"prec": prec, "epi": epi, "amode": amode, "dmode": dmode, "M": M, "N": N, "K": K,
                            "cfg": int(choose(e)), "heuristic": e["heuristic"],
@@ -247,11 +286,14 @@ def main(argv=None):
     ap.add_argument("--precs", default="fp16")
     ap.add_argument("--res", default=",".join(f"{h}x{w}" for h, w in RESOLUTIONS), help="HxW list")
     ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--batches", default=",".join(f"{n}x{h}x{w}" for n, h, w in BATCHES),
+                    help="HT-S batches, NxHxW list ('' for none)")
     a = ap.parse_args(argv)
     import torch
     torch.cuda.set_device(0)
     res = [tuple(int(v) for v in r.split("x")) for r in a.res.split(",") if r]
-    db = tune(models=a.models.split(","), resolutions=res, precs=a.precs.split(","), reps=a.reps)
+    batches = [tuple(int(v) for v in b.split("x")) for b in a.batches.split(",") if b]
+    db = tune(models=a.models.split(","), resolutions=res, precs=a.precs.split(","), reps=a.reps, batches=batches)
     with open(a.out, "w") as f:
         json.dump(db, f, indent=1)
     print(json.dumps(summary(db)))

@@ -48,7 +48,7 @@ def test_gemm_table_overrides_the_heuristic_and_clears():
 def test_gemm_table_rejects_malformed_entries():
     lib = _lib.load()
     h = lib.dcvc_gemm_tile_for(F16, EPI_QUAD, 0, 0, 8160, 768, 768)
-    for bad in ([F16, EPI_QUAD, 0, 0, 8160, 768, 768, 0], [F16, EPI_QUAD, 0, 0, 8160, 768, 768, 8],
+    for bad in ([F16, EPI_QUAD, 0, 0, 8160, 768, 768, 0], [F16, EPI_QUAD, 0, 0, 8160, 768, 768, 12],
                 [4, EPI_QUAD, 0, 0, 8160, 768, 768, 1], [F16, EPI_QUAD, 0, 0, 0, 768, 768, 1],
                 [F16, EPI_QUAD, 0, 0, 1 << 24, 768, 768, 1]):
         assert _set([[F16, EPI_QUAD, 0, 0, 8160, 768, 768, 2], bad]) == 1
@@ -121,3 +121,17 @@ def test_tuning_pass_and_bitstreams_independent_of_the_table(gpu):
         finally:
             tuning.clear()
     assert out[0] == out[1]
+
+
+@pytest.mark.gpu
+def test_tuning_pass_of_an_hts_batch(gpu):
+    """The tuner's batch pass (streams of one resolution coded as one batch)
+    times the batched launch shapes: the 1/8-resolution GEMMs carry M = streams
+    x pixels."""
+    n, H, W = 2, 64, 96
+    g, d = tuning.tune_one("hts", H, W, reps=1, batch=n)
+    assert g and d
+    ms = {k[4] for k in g}
+    assert n * (H // 8) * (W // 8) in ms
+    for e in g.values():
+        assert set(e["us"]) == set(tuning.GEMM_CANDIDATES) and all(v > 0 for v in e["us"].values())
