@@ -37,8 +37,8 @@ import sys
 from collections import defaultdict
 
 EPI_NAMES = {0: "bias", 1: "wsilu", 2: "shortcut", 3: "shortcut2", 4: "shortcut_quant", 5: "quant", 6: "quad"}
-# Tile<BM, BN, WM, WN, STAGES, EPI, AMODE, DMODE[, PREC]>
-_TILE = re.compile(r"Tile<(\d+), (\d+), (\d+), (\d+), (\d+), (\d+), (\d+), (\d+)(?:, (\d+))?>")
+# Tile<BM, BN, WM, WN, STAGES, EPI, AMODE, DMODE[, PREC[, KB]]>
+_TILE = re.compile(r"Tile<(\d+), (\d+), (\d+), (\d+), (\d+), (\d+), (\d+), (\d+)(?:, (\d+))?(?:, (\d+))?>")
 
 
 # FfnCfg<C, CM, EPI>: the fused DepthConvBlock FFN
@@ -61,7 +61,11 @@ def gemm_class(kernel_name: str) -> str | None:
 def short_name(kernel_name: str) -> str:
     m = _TILE.search(kernel_name)
     if m:
-        return f"gemm_kernel<{','.join(x for x in m.groups() if x is not None)}> [{gemm_class(kernel_name)}]"
+        # the K tile depth is shown only when it is not the default 32
+        g = [x for x in m.groups()[:9] if x is not None]
+        kb = m.group(10)
+        tag = f",k{kb}" if kb is not None and kb != "32" else ""
+        return f"gemm_kernel<{','.join(g)}{tag}> [{gemm_class(kernel_name)}]"
     f = _FFN.search(kernel_name)
     if f:
         return f"ffn_kernel<{','.join(f.groups())}> [{gemm_class(kernel_name)}]"

@@ -88,3 +88,16 @@ def test_synthetic_inputs_are_deterministic_and_shaped():
     assert np.array_equal(x[9:12], x[21:24])
     # occlusion re-noises ~10 % of luma pixels relative to a pure shift
     assert synthetic.get_padding_size(1080, 1920, 16) == (0, 8)
+
+
+def test_profile_kernel_names():
+    """rocprofv3 kernel names map to the short names and classes the profile
+    summaries and bench.py use, with or without the K tile depth parameter."""
+    from dcvc_amd import profiling as P
+    quad = "void dcvc::gemm_kernel<dcvc::Tile<128, 256, 2, 2, 3, 6, 0, 0, 0, 32> >(dcvc::GemmArgs, int)"
+    assert P.short_name(quad) == "gemm_kernel<128,256,2,2,3,6,0,0,0> [conv1x1_quad]"
+    assert P.short_name(quad.replace(", 0, 32>", ">")) == P.short_name(quad)
+    k64 = "void dcvc::gemm_kernel<dcvc::Tile<256, 256, 4, 2, 2, 1, 0, 0, 1, 64> >(dcvc::GemmArgs, int)"
+    assert P.short_name(k64) == "gemm_kernel<256,256,4,2,2,1,0,0,1,k64> [conv1x1_wsilu]"
+    conv = "void dcvc::gemm_kernel<dcvc::Tile<256, 128, 2, 2, 3, 0, 1, 0, 0, 32> >(dcvc::GemmArgs, int)"
+    assert P.gemm_class(conv) == "conv_s2_bias"
