@@ -96,7 +96,7 @@ def test_profile_kernel_names():
     from dcvc_amd import profiling as P
     quad = "void dcvc::gemm_kernel<dcvc::Tile<128, 256, 2, 2, 3, 6, 0, 0, 0, 32> >(dcvc::GemmArgs, int)"
     assert P.short_name(quad) == "gemm_kernel<128,256,2,2,3,6,0,0,0> [conv1x1_quad]"
-    assert P.short_name(quad.replace(", 0, 32>", ">")) == P.short_name(quad)
+    assert P.short_name(quad.replace(", 32>", ">")) == P.short_name(quad)
     k64 = "void dcvc::gemm_kernel<dcvc::Tile<256, 256, 4, 2, 2, 1, 0, 0, 1, 64> >(dcvc::GemmArgs, int)"
     assert P.short_name(k64) == "gemm_kernel<256,256,4,2,2,1,0,0,1,k64> [conv1x1_wsilu]"
     conv = "void dcvc::gemm_kernel<dcvc::Tile<256, 128, 2, 2, 3, 0, 1, 0, 0, 32> >(dcvc::GemmArgs, int)"
