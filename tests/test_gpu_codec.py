@@ -197,3 +197,41 @@ def test_hts_decoder_recovers_after_a_corrupt_bitstream(gpu, models):
     torch.cuda.synchronize()
     yhat_d = dec.debug_copy(0, P16 * 512 * 2, np.float16).reshape(P16, 512)[:, :256]
     assert np.array_equal(yhat_e.view(np.uint16), yhat_d.view(np.uint16))
+
+
+@pytest.mark.parametrize("model", ["hts", "htl", "ld"])
+def test_resolution_change_equals_fresh_proxies(gpu, model):
+    """One intra and one P proxy pair that switch resolution between calls
+    (buffers and graphs prepared again, as the reference's tensor pool is
+    cleared on a resolution change, memory_pool.h:76-84) code every
+    resolution bit for bit as fresh proxies do: I frame, P chunk (or frame)
+    and the decoder's reconstructions."""
+    from dcvc_amd import codec
+    sdi, sdp = codec.make_state_dicts(seed=3, video=model)
+    cls = codec.video_codec(model)
+    n = 1 + cls.FRAME_DELAY if cls.FRAME_DELAY > 1 else 2
+
+    def code(ie, idec, pe, pd, H, W, seed):
+        pad_r, pad_b = synthetic.get_padding_size(H, W, 16)
+        frames = synthetic.motion_sequence_yuv420(H, W, n, seed=seed)
+        e = ie.compress(codec.to_input(synthetic.frame_to_input(frames[0]), gpu), 32, pad_b, pad_r)
+        d = idec.decompress(e["bit_stream"], {"height": H, "width": W}, 32, e["ec_parallel"])
+        pe.add_ref_feature_from_frame(e["x_hat"])
+        pd.add_ref_feature_from_frame(d["x_hat"], apply_feature_adaptor=False)
+        src = synthetic.frame_to_input(frames[1]) if cls.FRAME_DELAY == 1 else \
+            synthetic.chunk_to_input(frames[1:5], cls.FRAME_DELAY)
+        c = pe.compress(codec.to_input(src, gpu), 21, 0, pad_b, pad_r)
+        outs = pd.decompress(c["bit_stream"], {"height": H, "width": W}, 21, c["ec_parallel"], 0)["x_hat"]
+        outs = outs if isinstance(outs, list) else [outs]
+        torch.cuda.synchronize()
+        return e["bit_stream"], d["x_hat"].clone(), c["bit_stream"], [o.clone() for o in outs]
+
+    reused = (codec.IntraCodec(sdi, 0.15), codec.IntraCodec(sdi, 0.15), cls(sdp, 0.15), cls(sdp, 0.15))
+    for step, (H, W) in enumerate([(128, 192), (96, 160), (136, 200), (128, 192)]):
+        print(f"{model}: step {step} at {H}x{W}", flush=True)
+        got = code(*reused, H, W, seed=40 + step)
+        fresh = code(codec.IntraCodec(sdi, 0.15), codec.IntraCodec(sdi, 0.15), cls(sdp, 0.15), cls(sdp, 0.15), H, W,
+                     seed=40 + step)
+        assert got[0] == fresh[0] and got[2] == fresh[2], f"{H}x{W}: bitstreams differ"
+        assert torch.equal(got[1], fresh[1]), f"{H}x{W}: I-frame reconstruction differs"
+        assert len(got[3]) == len(fresh[3]) and all(torch.equal(a, b) for a, b in zip(got[3], fresh[3]))
