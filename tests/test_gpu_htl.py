@@ -196,3 +196,32 @@ def test_htl_shared_weights_and_precision_forms(gpu, models):
         yd = dec.debug_copy(0, P16 * 256 * 2, np.float16)
         assert np.array_equal(ye.view(np.uint16), yd.view(np.uint16)), prec
         assert all(torch.isfinite(o.float()).all() for o in outs)
+
+
+def test_htl_decoder_recovers_after_a_corrupt_bitstream(gpu, models):
+    """As for HT-S and LD: a truncated chunk bitstream raises in the host z
+    decode with the temporal-context work already queued on the side stream;
+    after both sides re-anchor on the intra frame, the next chunk decodes bit
+    for bit."""
+    from dcvc_amd.inference_extensions_cuda import DMCHTLProxy
+    sdi, sdp = models
+    H, W = 128, 192
+    frames = synthetic.motion_sequence_yuv420(H, W, 17, seed=6)
+    x_hat_i, x_hat_id = _intra(sdi, synthetic.frame_to_input(frames[0]), gpu, H, W, 0, 0)
+    enc, dec = DMCHTLProxy(), DMCHTLProxy()
+    enc.set_param(sdp, 0.15)
+    dec.set_param(sdp, 0.15)
+    enc.add_ref_feature_from_frame(x_hat_i, True)
+    dec.add_ref_feature_from_frame(x_hat_id, False)
+    bits, ec = enc.compress(_to_gpu_nchw(synthetic.chunk_to_input(frames[1:9]), gpu), 32, False, 0, 0)
+    with pytest.raises(Exception):
+        dec.decompress(np.ascontiguousarray(bits[:3]), 32, H, W, ec, False)
+    enc.add_ref_feature_from_frame(x_hat_i, True)
+    dec.add_ref_feature_from_frame(x_hat_id, False)
+    P16 = (H // 16) * (W // 16)
+    bits, ec = enc.compress(_to_gpu_nchw(synthetic.chunk_to_input(frames[9:17]), gpu), 32, False, 0, 0)
+    yhat_e = enc.debug_copy(0, P16 * 256 * 2, np.float16).copy()
+    dec.decompress(bits, 32, H, W, ec, False)
+    torch.cuda.synchronize()
+    yhat_d = dec.debug_copy(0, P16 * 256 * 2, np.float16)
+    assert np.array_equal(yhat_e.view(np.uint16), yhat_d.view(np.uint16))

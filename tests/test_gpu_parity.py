@@ -473,3 +473,71 @@ def test_htl_ld_1080p_within_the_fp16_storage_floor(gpu, intra_1080p, model):
     print(f"{model} 1080p bytes gpu {len(bits)} {nb}; max PSNR delta {dp:.4f} dB", flush=True)
     assert dp <= 0.02
     assert abs(len(bits) - nb["fp32"]) <= tol
+
+
+# ------------------------------------------------------------- 3840x2160
+H2160, W2160 = 2160, 3840
+
+
+@pytest.fixture(scope="module")
+def intra_2160p(gpu):
+    """A 3840x2160 DMCI frame: encoder and decoder reconstructions."""
+    from dcvc_amd.inference_extensions_cuda import DMCIProxy
+    pad_r, pad_b = synthetic.get_padding_size(H2160, W2160, 16)
+    print(f"2160p intra: padding bottom {pad_b} right {pad_r}", flush=True)
+    frames = synthetic.motion_sequence_yuv420(H2160, W2160, 5, seed=13)
+    e, d = DMCIProxy(), DMCIProxy()
+    e.set_param(_state("dmci", 0), 0.0)
+    d.set_param(_state("dmci", 0), 0.0)
+    bits, x_hat_e, ec = e.compress(_gpu_nchw(synthetic.frame_to_input(frames[0]), gpu), 32, pad_b, pad_r)
+    x_hat_e = x_hat_e.clone()
+    x_hat_d = d.decompress(bits, 32, H2160, W2160, ec).clone()
+    torch.cuda.synchronize()
+    return dict(frames=frames, x_hat_e=x_hat_e, x_hat_d=x_hat_d, pad_b=pad_b, pad_r=pad_r,
+                Hp=H2160 + pad_b, Wp=W2160 + pad_r, nbytes=len(bits))
+
+
+def test_dmci_2160p_encoder_decoder_bitexact(intra_2160p):
+    t = intra_2160p
+    assert t["nbytes"] > 0
+    assert torch.equal(t["x_hat_e"], t["x_hat_d"]), "decoder reconstruction differs from the encoder's"
+
+
+@pytest.mark.parametrize("model", ["hts", "htl", "ld"])
+def test_p_codecs_2160p_encoder_decoder_bitexact(gpu, intra_2160p, model):
+    """The largest picture size the reference tunes for (3840x2160): from
+    the intra frame above, one chunk of 4 real frames padded to 8 (HT-S,
+    HT-L) or one frame (LD); the decoder rebuilds the encoder's y_hat bit
+    for bit and every reconstruction is finite and in range."""
+    from dcvc_amd.inference_extensions_cuda import DMCHTLProxy, DMCHTSProxy, DMCLDProxy
+    t = intra_2160p
+    pad_b, pad_r, Hp, Wp = t["pad_b"], t["pad_r"], t["Hp"], t["Wp"]
+    cls = {"hts": DMCHTSProxy, "htl": DMCHTLProxy, "ld": DMCLDProxy}[model]
+    sdp = _state(model, {"hts": 1, "htl": 2, "ld": 3}[model])
+    enc, dec = cls(), cls()
+    enc.set_param(sdp, 0.0)
+    dec.set_param(sdp, 0.0)
+    enc.add_ref_feature_from_frame(t["x_hat_e"], True)
+    dec.add_ref_feature_from_frame(t["x_hat_d"], False)
+    P16 = (Hp // 16) * (Wp // 16)
+    x = synthetic.frame_to_input(t["frames"][1]) if model == "ld" else synthetic.chunk_to_input(t["frames"][1:5], 8)
+    print(f"{model} 2160p: input {x.shape}", flush=True)
+    bits, ec = enc.compress(_gpu_nchw(x, gpu), 32, False, pad_b, pad_r)
+    if model == "htl":
+        yhat_e = enc.debug_copy(0, P16 * 256 * 2, np.float16).copy()
+    else:
+        cy = 256 if model == "hts" else 128
+        yhat_e = enc.debug_copy(0, P16 * 512 * 2, np.float16).reshape(P16, 512)[:, :cy].copy()
+    outs = dec.decompress(bits, 32, H2160, W2160, ec, False)
+    outs = [outs] if model == "ld" else outs
+    torch.cuda.synchronize()
+    if model == "htl":
+        yhat_d = dec.debug_copy(0, P16 * 256 * 2, np.float16)
+    else:
+        yhat_d = dec.debug_copy(0, P16 * 512 * 2, np.float16).reshape(P16, 512)[:, :cy]
+    print(f"{model} 2160p: {len(bits)} bytes, ec {ec}", flush=True)
+    assert len(bits) > 0
+    assert np.array_equal(yhat_e.view(np.uint16), yhat_d.view(np.uint16)), "decoder y_hat differs"
+    for f in outs:
+        assert f.shape == (1, 3, Hp, Wp)
+        assert torch.isfinite(f.float()).all() and float(f.abs().max()) <= 0.5
