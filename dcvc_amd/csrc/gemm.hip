@@ -620,7 +620,15 @@ __global__ __launch_bounds__(T::kThreads, T::kWavesPerSimd) void gemm_kernel(Gem
 //   9: 256 x 256, 8 waves (4 x 2, wave 64 x 128), 2 stages
 //   10: 256 x 128 (quad: 128 x 256), 3 stages (1 block / CU)
 //   11: 128 x 128 (quad: waves 4 x 1), 2 stages (2 blocks / CU)
-constexpr int kPolicies = 11;
+//   Tiles for small grids (the 1/16- and 1/64-resolution layers, whose
+//   128 x 128 grids leave CUs idle and whose K loops wait on load latency):
+//   12: 128 x 128 (quad: 128 x 256), 8 waves (4 x 2, wave 32 x 64 / 32 x 128),
+//       5 stages
+//   13: 64 x 128 (quad: 64 x 256), waves 2 x 2 (wave 32 x 64 / 32 x 128),
+//       5 stages
+//   14: as 12 with 64-deep K tiles, 4 stages (quad: 3); plain 1x1 products
+//       only, other launches take 12
+constexpr int kPolicies = 14;
 
 template <int BM, int BN, int WM, int WN, int ST, int EPI, int AMODE, int DMODE, int PREC = PREC_F16, int KB = 32>
 static hipError_t launch_tile(GemmArgs g, hipStream_t stream) {
@@ -640,18 +648,23 @@ static hipError_t launch_tile(GemmArgs g, hipStream_t stream) {
 // Automatic choice, from a sweep of the 1080p HT-S layer shapes on MI355X:
 // large-M layers run 2 blocks per CU (one block's epilogue overlaps the
 // other's K loop), except long-K layers (K >= 1024), which keep the 256 x 256
-// tile; the 1/16- and 1/64-resolution layers take 128 x 128 tiles so the grid
-// still covers the 256 CUs.  The 8-wave 256 x 256 tiles (6, 7) are 2-10 %
-// faster on the 1/8-resolution 512-channel and long-K shapes in isolation,
-// but inside the codecs, where two streams run concurrently, one block per CU
-// cannot share a CU with the other stream's blocks and the layers took
-// 5-17 % longer, so they are only used when forced.
+// tile; smaller grids take 128 x 128 tiles with 8 waves (12), and grids of
+// fewer than 256 such tiles (the 1/16- and 1/64-resolution layers) take
+// 64 x 128 tiles (13), so that the grid still covers the 256 CUs.  Over the
+// autotuner's sweep of every model and resolution, 13 takes 0.89 of the time
+// of the 4-wave 128 x 128 tile on those small grids and 12 takes 0.97 on the
+// others; the quad GEMM keeps 128 x 128 (4 x 1 waves) there.  The 8-wave
+// 256 x 256 tiles (6, 7) are 2-10 % faster on the 1/8-resolution
+// 512-channel and long-K shapes in isolation, but inside the codecs, where
+// two streams run concurrently, one block per CU cannot share a CU with the
+// other stream's blocks and the layers took 5-17 % longer, so they are only
+// used when forced.
 static int heuristic_tile(const GemmArgs& g, bool quad) {
     const long long m256 = ceil_div(g.M, 256), m128 = ceil_div(g.M, 128);
     if (quad) return (m128 * ceil_div(g.N, 64) >= 480) ? 4 : 3;
     if (g.K >= 1024 && m256 * ceil_div(g.N, 256) >= 240) return 1;
     if (m256 * ceil_div(g.N, 128) >= 480) return 4;
-    return 3;
+    return (m128 * ceil_div(g.N, 128) < 256) ? 13 : 12;
 }
 
 // Forced policy, else the tuning table's entry for this shape, else the
@@ -696,6 +709,10 @@ static hipError_t dispatch_tiles(const GemmArgs& g, hipStream_t stream) {
         case 5: return launch_tile<256, 256, 2, 2, 4, EPI, AMODE, DMODE, PREC>(g, stream);
         case 6: return launch_tile<256, 256, 4, 2, 4, EPI, AMODE, DMODE, PREC>(g, stream);
         case 7: return launch_tile<256, 256, 4, 2, 3, EPI, AMODE, DMODE, PREC>(g, stream);
+        case 14: if constexpr (kPlain) return launch_tile<128, 256, 4, 2, 3, EPI, AMODE, DMODE, PREC, 64>(g, stream);
+        [[fallthrough]];
+        case 12: return launch_tile<128, 256, 4, 2, 5, EPI, AMODE, DMODE, PREC>(g, stream);
+        case 13: return launch_tile<64, 256, 2, 2, 5, EPI, AMODE, DMODE, PREC>(g, stream);
         case 8: if constexpr (kPlain) return launch_tile<256, 256, 2, 2, 2, EPI, AMODE, DMODE, PREC, 64>(g, stream);
         [[fallthrough]];
         case 9: if constexpr (kPlain) return launch_tile<256, 256, 4, 2, 2, EPI, AMODE, DMODE, PREC, 64>(g, stream);
@@ -714,6 +731,10 @@ static hipError_t dispatch_tiles(const GemmArgs& g, hipStream_t stream) {
         case 5: return launch_tile<128, 256, 2, 2, 3, EPI, AMODE, DMODE, PREC>(g, stream);
         case 6: return launch_tile<256, 256, 4, 2, 4, EPI, AMODE, DMODE, PREC>(g, stream);
         case 7: return launch_tile<256, 256, 4, 2, 3, EPI, AMODE, DMODE, PREC>(g, stream);
+        case 14: if constexpr (kPlain) return launch_tile<128, 128, 4, 2, 4, EPI, AMODE, DMODE, PREC, 64>(g, stream);
+        [[fallthrough]];
+        case 12: return launch_tile<128, 128, 4, 2, 5, EPI, AMODE, DMODE, PREC>(g, stream);
+        case 13: return launch_tile<64, 128, 2, 2, 5, EPI, AMODE, DMODE, PREC>(g, stream);
         case 8: if constexpr (kPlain) return launch_tile<256, 256, 2, 2, 2, EPI, AMODE, DMODE, PREC, 64>(g, stream);
         [[fallthrough]];
         case 9: if constexpr (kPlain) return launch_tile<256, 256, 4, 2, 2, EPI, AMODE, DMODE, PREC, 64>(g, stream);

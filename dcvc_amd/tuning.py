@@ -42,8 +42,10 @@ RESOLUTIONS = [(2160, 3840), (1080, 1920), (720, 1280), (480, 832), (480, 854), 
 # 16-stream mix (batching.MIX_16), each layer one launch over the group
 BATCHES = [(4, 480, 854), (6, 720, 1280), (4, 1080, 1920), (2, 2160, 3840)]
 # 6 and 7 hold one block per CU: faster alone on some shapes, slower inside
-# the codecs, whose concurrent streams need a second block to share the CU
-GEMM_CANDIDATES = (1, 2, 3, 4, 5)
+# the codecs, whose concurrent streams need a second block to share the CU;
+# 8-11 (64-deep K tiles on large tiles) were no faster there.  12-14 are the
+# small-grid tiles.
+GEMM_CANDIDATES = (1, 2, 3, 4, 5, 12, 13, 14)
 DW_CANDIDATES = (4, 8)
 COLS = 15  # trace record width (include/dcvc_codec.h)
 

@@ -30,7 +30,7 @@ int dcvc_version(void);
  * (def_const.h:6-11, elementwise/stream.cu:77-87).  Host-only. */
 int dcvc_scale_index_thresholds(float* th128);
 
-/* GEMM tile configuration: 0 automatic (default), 1-11 force one of the
+/* GEMM tile configuration: 0 automatic (default), 1-14 force one of the
  * tile configurations for every GEMM launched afterwards (process-wide).
  * No counterpart in the reference: used by tests (all configurations must
  * give bit-identical results) and by the autotuner.  Host-only. */
@@ -41,7 +41,7 @@ int dcvc_gemm_set_tile_policy(int policy);
  * HINT_MAP_D3x3 (cutlass/d3x3.cu:7-441), which launch_cutlass_helper
  * consults (cutlass/cutlass_kernel.h:233-249).  dcvc_gemm_tune_set replaces
  * the GEMM table with n entries of 8 int64 {prec, epi, amode, dmode, M, N, K,
- * config (1-11)} (prec as dcvc_gemm_lowp, or 3: prec 2 on e4m3 activations,
+ * config (1-14)} (prec as dcvc_gemm_lowp, or 3: prec 2 on e4m3 activations,
  * dcvc_gemm_fp8_act): a GEMM of that key launches with that configuration, other
  * shapes keep the built-in heuristic (n = 0 clears).  dcvc_d3x3_tune_set does
  * the same for the depthwise strip length, entries {n_img, H, W, C, rows (4

@@ -139,8 +139,10 @@ def test_gemm_batch_invariance(gpu):
 def test_gemm_tile_configs_bit_identical(gpu, epi):
     """Every tile configuration gives the same bits (same per-element accumulation
     order), including ragged M / N tails, implicit convs and the sub-pixel store.
-    Configurations 8-11 (64-deep K tiles, 128-byte LDS rows) serve plain 1x1
-    products; the other launches fall back to the automatic choice."""
+    Configurations 8-11 and 14 (64-deep K tiles, 128-byte LDS rows) serve
+    plain 1x1 products; the other launches fall back to the automatic choice
+    (14: to 12).  12 and 13 are the small-grid tiles (8 waves on 128 x 128,
+    and 64-row tiles)."""
     from dcvc_amd import _lib, kernels
     lib = _lib.load()
     r = rng(21)
@@ -173,7 +175,7 @@ def test_gemm_tile_configs_bit_identical(gpu, epi):
 
     outs = []
     try:
-        for policy in range(1, 12):
+        for policy in range(1, 15):
             assert lib.dcvc_gemm_set_tile_policy(policy) == 0
             r0 = rng(5)
             outs.append(run())
@@ -201,7 +203,7 @@ def test_bf16_tile_configs_bit_identical(gpu, epi):
     bias = torch.full((rows,), 0.05, device=gpu)
     outs = []
     try:
-        for policy in range(1, 12):
+        for policy in range(1, 15):
             assert lib.dcvc_gemm_set_tile_policy(policy) == 0
             outs.append(kernels.gemm_lowp(a, wq, scale, "bf16", bias, epi=epi, s1=s1 if epi == "shortcut" else None))
     finally:

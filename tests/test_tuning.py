@@ -25,7 +25,7 @@ def test_gemm_table_overrides_the_heuristic_and_clears():
     lib = _lib.load()
     try:
         h = lib.dcvc_gemm_tile_for(F16, EPI_SHORTCUT, 0, 0, 32640, 512, 256)
-        assert 1 <= h <= 7
+        assert 1 <= h <= 14
         other = 5 if h != 5 else 2
         assert _set([[F16, EPI_SHORTCUT, 0, 0, 32640, 512, 256, other]]) == 0
         assert lib.dcvc_gemm_tile_for(F16, EPI_SHORTCUT, 0, 0, 32640, 512, 256) == other
@@ -48,7 +48,7 @@ def test_gemm_table_overrides_the_heuristic_and_clears():
 def test_gemm_table_rejects_malformed_entries():
     lib = _lib.load()
     h = lib.dcvc_gemm_tile_for(F16, EPI_QUAD, 0, 0, 8160, 768, 768)
-    for bad in ([F16, EPI_QUAD, 0, 0, 8160, 768, 768, 0], [F16, EPI_QUAD, 0, 0, 8160, 768, 768, 12],
+    for bad in ([F16, EPI_QUAD, 0, 0, 8160, 768, 768, 0], [F16, EPI_QUAD, 0, 0, 8160, 768, 768, 15],
                 [4, EPI_QUAD, 0, 0, 8160, 768, 768, 1], [F16, EPI_QUAD, 0, 0, 0, 768, 768, 1],
                 [F16, EPI_QUAD, 0, 0, 1 << 24, 768, 768, 1]):
         assert _set([[F16, EPI_QUAD, 0, 0, 8160, 768, 768, 2], bad]) == 1
@@ -79,7 +79,7 @@ def test_committed_database_installs():
     assert db["gemm"] and db["d3x3"]
     assert {"3840x2160", "1920x1080", "1280x720", "832x480", "416x240"} <= set(db["resolutions"])
     for e in db["gemm"]:
-        assert 1 <= e["cfg"] <= 7 and str(e["cfg"]) in e["us"]
+        assert e["cfg"] in tuning.GEMM_CANDIDATES and str(e["cfg"]) in e["us"]
         assert e["us"][str(e["cfg"])] <= e["us"][str(e["heuristic"])] * 1.0001 or e["cfg"] == e["heuristic"]
     try:
         assert tuning.install(db) == (len(db["gemm"]), len(db["d3x3"]))
