@@ -692,13 +692,18 @@ static hipError_t dispatch_tiles(const GemmArgs& g, hipStream_t stream) {
         constexpr bool kShort = (EPI == EPI_SHORTCUT || EPI == EPI_SHORTCUT2 || EPI == EPI_SHORTCUT_QUANT);
         // measured on the 1080p shapes: 256 x 256 pays off only for K >= 1024
         const bool big = !kShort && (g_tile_policy != 0 ? (t == 1 || t == 2) : g.K >= 1024);
+        // grids of fewer than 256 128 x 128 tiles take 64 x 128 tiles, as
+        // configuration 13 does for fp16
+        const bool small = g_tile_policy != 0 ? t == 13 : ceil_div(g.M, 128) * ceil_div(g.N, 128) < 256;
         if constexpr (EPI == EPI_QUAD) {
             if (big) return launch_tile<256, 256, 2, 2, 3, EPI, AMODE, DMODE, PREC>(g, stream);
             return launch_tile<128, 128, 4, 1, 3, EPI, AMODE, DMODE, PREC>(g, stream);
         } else if constexpr (kShort) {
+            if (small) return launch_tile<64, 128, 2, 2, 3, EPI, AMODE, DMODE, PREC>(g, stream);
             return launch_tile<128, 128, 2, 2, 3, EPI, AMODE, DMODE, PREC>(g, stream);
         } else {
             if (big) return launch_tile<256, 256, 2, 2, 3, EPI, AMODE, DMODE, PREC>(g, stream);
+            if (small) return launch_tile<64, 128, 2, 2, 3, EPI, AMODE, DMODE, PREC>(g, stream);
             return launch_tile<128, 128, 2, 2, 3, EPI, AMODE, DMODE, PREC>(g, stream);
         }
     } else if constexpr (EPI == EPI_QUAD) {

@@ -95,7 +95,7 @@ def test_fp8_activation_tile_configs_bit_identical(gpu, epi):
     wq, scale = kernels.weights_to_lowp(w, "fp8")
     outs = []
     try:
-        for policy in (0, 1, 2, 3, 4, 5, 6, 7):
+        for policy in range(0, 15):
             assert lib.dcvc_gemm_set_tile_policy(policy) == 0
             o, _ = kernels.gemm_fp8_act(a8, wq, scale, b, epi=epi, s1=s1 if epi == "shortcut" else None)
             outs.append(o)

@@ -145,7 +145,7 @@ def test_lowp_tile_configs_bit_identical(gpu, prec, epi):
     wq, scale = kernels.weights_to_lowp(f16(w, gpu), prec)
     outs = []
     try:
-        for policy in (1, 2, 3, 4, 5, 6, 7):
+        for policy in range(1, 15):
             assert lib.dcvc_gemm_set_tile_policy(policy) == 0
             outs.append(kernels.gemm_lowp(f16(a, gpu), wq, scale, prec, torch.from_numpy(b).to(gpu), epi=epi,
                                           s1=f16(s1, gpu)))
