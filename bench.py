@@ -95,11 +95,20 @@ def kernel_classes(recs):
     return dict(sorted(out.items(), key=lambda kv: -kv[1]["time_ms"]))
 
 
-# tile configuration id -> Tile<BM, BN, WM, WN, STAGES> of the fp16 dispatch (gemm.hip)
+# tile configuration id -> Tile<BM, BN, WM, WN, STAGES[, K tile 64]> of the
+# fp16 / bf16 dispatch (gemm.hip); the 64-deep K tiles (8-11, 14) serve plain
+# 1x1 products only, and other launches take `TILES_FALLBACK`'s configuration
 TILES_QUAD = {1: (256, 256, 2, 2, 5), 2: (128, 256, 2, 2, 5), 3: (128, 128, 4, 1, 5), 4: (128, 256, 2, 2, 3),
-              5: (256, 256, 2, 2, 4), 6: (256, 256, 4, 2, 4), 7: (256, 256, 4, 2, 3)}
+              5: (256, 256, 2, 2, 4), 6: (256, 256, 4, 2, 4), 7: (256, 256, 4, 2, 3),
+              8: (256, 256, 2, 2, 2, 64), 9: (256, 256, 4, 2, 2, 64), 10: (128, 256, 2, 2, 3, 64),
+              11: (128, 128, 4, 1, 2, 64), 12: (128, 256, 4, 2, 5), 13: (64, 256, 2, 2, 5),
+              14: (128, 256, 4, 2, 3, 64)}
 TILES = {1: (256, 256, 2, 2, 5), 2: (256, 128, 2, 2, 5), 3: (128, 128, 2, 2, 5), 4: (256, 128, 2, 2, 3),
-         5: (128, 256, 2, 2, 3), 6: (256, 256, 4, 2, 4), 7: (256, 256, 4, 2, 3)}
+         5: (128, 256, 2, 2, 3), 6: (256, 256, 4, 2, 4), 7: (256, 256, 4, 2, 3),
+         8: (256, 256, 2, 2, 2, 64), 9: (256, 256, 4, 2, 2, 64), 10: (256, 128, 2, 2, 3, 64),
+         11: (128, 128, 2, 2, 2, 64), 12: (128, 128, 4, 2, 5), 13: (64, 128, 2, 2, 5),
+         14: (128, 128, 4, 2, 4, 64)}
+TILES_FALLBACK = {8: 3, 9: 3, 10: 3, 11: 3, 14: 12}
 
 
 def instance_name(r):
@@ -110,8 +119,11 @@ def instance_name(r):
         return f"dw3x3_kernel<{cfg}>"
     if kind == 4 or prec != 0:
         return f"{KIND_NAMES[kind]}_{EPI_NAMES[epi]}"
+    if (amode, dmode) != (0, 0):
+        cfg = TILES_FALLBACK.get(cfg, cfg)
     t = (TILES_QUAD if epi == 6 else TILES)[cfg]
-    return "gemm_kernel<" + ",".join(str(v) for v in (*t, epi, amode, dmode, prec)) + ">"
+    tag = ",k64" if len(t) == 6 else ""
+    return "gemm_kernel<" + ",".join(str(v) for v in (*t[:5], epi, amode, dmode, prec)) + tag + ">"
 
 
 def kernel_instances(recs):

@@ -101,3 +101,40 @@ def test_profile_kernel_names():
     assert P.short_name(k64) == "gemm_kernel<256,256,4,2,2,1,0,0,1,k64> [conv1x1_wsilu]"
     conv = "void dcvc::gemm_kernel<dcvc::Tile<256, 128, 2, 2, 3, 0, 1, 0, 0, 32> >(dcvc::GemmArgs, int)"
     assert P.gemm_class(conv) == "conv_s2_bias"
+
+
+def test_bench_tile_tables_follow_the_gemm_dispatch():
+    """bench.py names each traced GEMM launch by its tile configuration; its
+    tables must list, for every configuration id, the Tile the fp16 / bf16
+    dispatch of gemm.hip launches (64-deep K tiles with their fallback for
+    launches that are not plain 1x1 products)."""
+    import importlib.util
+    import re
+    from pathlib import Path
+    root = Path(__file__).resolve().parents[1]
+    spec = importlib.util.spec_from_file_location("bench_mod", root / "bench.py")
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    src = (root / "dcvc_amd" / "csrc" / "gemm.hip").read_text()
+    body = src[src.index("static hipError_t dispatch_tiles"):src.index("static hipError_t dispatch_plain")]
+    quad_part = body[body.index("} else if constexpr (EPI == EPI_QUAD) {"):]
+    quad_part, plain_part = quad_part[:quad_part.index("    } else {")], quad_part[quad_part.index("    } else {"):]
+    # configuration 3 is the switch's default
+    pat = re.compile(r"(?:case (\d+)|default):( if constexpr \(kPlain\))? return launch_tile<(\d+), (\d+), (\d+), "
+                     r"(\d+), (\d+), EPI, AMODE, DMODE, PREC(?:, (\d+))?>")
+    n_policies = int(re.search(r"constexpr int kPolicies = (\d+);", src).group(1))
+    for part, table in ((quad_part, bench.TILES_QUAD), (plain_part, bench.TILES)):
+        found = {}
+        for m in pat.finditer(part):
+            t = tuple(int(v) for v in m.group(3, 4, 5, 6, 7))
+            if m.group(8) and m.group(8) != "32":
+                t += (int(m.group(8)),)
+            found[int(m.group(1) or 3)] = (t, bool(m.group(2)))
+        assert set(found) == set(range(1, n_policies + 1)) == set(table)
+        for cfg, (t, plain_only) in found.items():
+            assert table[cfg] == t, cfg
+            assert (cfg in bench.TILES_FALLBACK) == plain_only, cfg
+    # the fallback of a plain-only configuration is the configuration the
+    # dispatch falls through to (the default tile equals configuration 3's)
+    for cfg, fb in bench.TILES_FALLBACK.items():
+        assert len(bench.TILES[cfg]) == 6 and len(bench.TILES[fb]) == 5
