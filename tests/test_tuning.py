@@ -45,6 +45,24 @@ def test_gemm_table_overrides_the_heuristic_and_clears():
         _set([])
 
 
+def test_heuristic_tile_choice():
+    """The automatic choice by grid size (args: prec, epi, amode, dmode, M, N,
+    K): long-K 1/8-resolution layers 256 x 256; large grids 2 blocks per CU;
+    smaller grids the 8-wave 128 x 128 tile; grids of fewer than 256 128 x 128
+    tiles (1/16 and 1/64 resolution) 64 x 128; quads 128 x 256 or 128 x 128."""
+    lib = _lib.load()
+    tile = lib.dcvc_gemm_tile_for
+    assert tile(F16, EPI_BIAS, 0, 0, 32640, 512, 2048) == 1      # K >= 1024, 128 x 2 256-tiles
+    assert tile(F16, EPI_SHORTCUT, 0, 0, 32640, 512, 512) == 4   # 128 x 4 = 512 256 x 128 tiles
+    assert tile(F16, EPI_SHORTCUT, 0, 0, 32640, 256, 256) == 12  # 255 x 2 = 510 128 x 128 tiles
+    assert tile(F16, EPI_SHORTCUT, 0, 0, 8160, 256, 256) == 13   # 64 x 2 = 128 128 x 128 tiles
+    assert tile(F16, EPI_BIAS, 0, 0, 510, 1024, 128) == 13
+    assert tile(F16, EPI_QUAD, 0, 0, 32640, 256, 256) == 4
+    assert tile(F16, EPI_QUAD, 0, 0, 8160, 256, 256) == 3
+    # the same rule for bf16 products
+    assert tile(BF16, EPI_SHORTCUT, 0, 0, 8160, 256, 256) == 13
+
+
 def test_gemm_table_rejects_malformed_entries():
     lib = _lib.load()
     h = lib.dcvc_gemm_tile_for(F16, EPI_QUAD, 0, 0, 8160, 768, 768)
