@@ -48,7 +48,7 @@ def _intra(sdi, frame, gpu, H, W, pad_b, pad_r):
     return x_hat.clone(), d.decompress(bits, 32, H, W, ec).clone()
 
 
-@pytest.mark.parametrize("H,W", [(128, 192), (136, 200), (1080, 1920)])
+@pytest.mark.parametrize("H,W", [(128, 192), (136, 200), (240, 416), (1080, 1920)])
 def test_htl_chunks_encoder_decoder_bitexact(gpu, models, H, W):
     from dcvc_amd.inference_extensions_cuda import DMCHTLProxy
     sdi, sdp = models

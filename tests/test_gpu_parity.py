@@ -541,3 +541,40 @@ def test_p_codecs_2160p_encoder_decoder_bitexact(gpu, intra_2160p, model):
     for f in outs:
         assert f.shape == (1, 3, Hp, Wp)
         assert torch.isfinite(f.float()).all() and float(f.abs().max()) <= 0.5
+
+
+@pytest.mark.parametrize("H,W", [(240, 416), (480, 832), (480, 854), (720, 1280)])
+def test_hts_reference_resolutions_encoder_decoder_bitexact(gpu, H, W):
+    """HT-S at the other resolutions the reference tunes for (416x240 to
+    1280x720; 854 columns need padding, 240 and 480 rows the 1/64-scale
+    padding inside the hyperprior): an intra frame, then one chunk of 4 real
+    frames padded to 8; the decoder rebuilds the encoder's y_hat bit for bit."""
+    from dcvc_amd.inference_extensions_cuda import DMCHTSProxy, DMCIProxy
+    pad_r, pad_b = synthetic.get_padding_size(H, W, 16)
+    Hp, Wp = H + pad_b, W + pad_r
+    print(f"HT-S {H}x{W} padded to {Hp}x{Wp}", flush=True)
+    frames = synthetic.motion_sequence_yuv420(H, W, 5, seed=17)
+    ie, idec = DMCIProxy(), DMCIProxy()
+    ie.set_param(_state("dmci", 0), 0.0)
+    idec.set_param(_state("dmci", 0), 0.0)
+    bits_i, x_hat_e, ec_i = ie.compress(_gpu_nchw(synthetic.frame_to_input(frames[0]), gpu), 32, pad_b, pad_r)
+    x_hat_e = x_hat_e.clone()
+    x_hat_d = idec.decompress(bits_i, 32, H, W, ec_i).clone()
+    assert torch.equal(x_hat_e, x_hat_d)
+    sdp = _state("hts", 1)
+    enc, dec = DMCHTSProxy(), DMCHTSProxy()
+    enc.set_param(sdp, 0.0)
+    dec.set_param(sdp, 0.0)
+    enc.add_ref_feature_from_frame(x_hat_e, True)
+    dec.add_ref_feature_from_frame(x_hat_d, False)
+    P16 = (Hp // 16) * (Wp // 16)
+    bits, ec = enc.compress(_gpu_nchw(synthetic.chunk_to_input(frames[1:5], 8), gpu), 32, False, pad_b, pad_r)
+    yhat_e = enc.debug_copy(0, P16 * 512 * 2, np.float16).reshape(P16, 512)[:, :256].copy()
+    n_e = enc.last_symbol_count()
+    outs = dec.decompress(bits, 32, H, W, ec, False)
+    torch.cuda.synchronize()
+    yhat_d = dec.debug_copy(0, P16 * 512 * 2, np.float16).reshape(P16, 512)[:, :256]
+    assert dec.last_symbol_count() == n_e
+    assert np.array_equal(yhat_e.view(np.uint16), yhat_d.view(np.uint16))
+    for f in outs:
+        assert f.shape == (1, 3, Hp, Wp) and torch.isfinite(f.float()).all()
